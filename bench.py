@@ -1,0 +1,196 @@
+"""Throughput of the SPH step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-cpu-baseline]
+
+A step is one pass of the per-timestep pipeline (hash -> stable radix sort
+-> cell ranges + SoA reorder -> density -> forces + walls + integrate) over
+every particle of a synthetic dam-break scene, driven through the C++
+HeterogeneousSimulator and replayed as a captured HIP graph.  Inputs are
+resident in HBM before the timed region.  `value` is particle-steps/s.
+
+Roofline: algorithmic bytes per particle (hash 24, one sort pass 16, cells +
+reorder 72, density 24, forces 60, integrate 48) plus 4 B per bucket for the
+cellStart fill and 24 B per occupied cell (start/end write, two reads).  The
+dominant kernel's bytes are divided by its average duration, measured with
+HIP events on the simulator's stream around each stage over a second, eager
+pass of the same steps.
+
+cpu_baseline: the oracle (a C++/OpenMP restatement of the reference step;
+the reference itself has no CPU SPH path) timed on this host's cores on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from fluidgrid_amd import abi, lib, scenes  # noqa: E402
+
+HBM_PEAK = 8.0e12
+METRIC = "particle-steps/s at 1M & 8M; % HBM peak; batched/mixed vs plain FP32"
+STAGE_BYTES = dict(hash=24, sort=16, cells=72, density=24, forces=60, integrate=48)
+OCC_BYTES = dict(cells=8, density=8, forces=8)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def occupied_cells(sim):
+    b = sim.buffers
+    start = lib.download_raw(b.gridStart, (b.numBuckets,), np.uint32)
+    return int(np.count_nonzero(start != 0xFFFFFFFF))
+
+
+def stage_bytes(stage, n, c_occ, fused):
+    by = STAGE_BYTES[stage] * n + OCC_BYTES.get(stage, 0) * c_occ
+    if stage == "forces" and fused:
+        by += STAGE_BYTES["integrate"] * n
+    return by
+
+
+def measure(scene, steps, warmup, fused=True):
+    sim = lib.Simulator(scene, fused=fused, use_graph=True)
+    try:
+        sim.step(warmup)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.synchronize()
+        dt = time.perf_counter() - t0
+        c_occ = occupied_cells(sim)
+        # per-stage durations: eager steps with events between stages
+        sim.set_options(fused=fused, use_graph=False, stage_timing=True)
+        acc = None
+        nprof = max(3, min(steps, 10))
+        for _ in range(nprof):
+            sim.step(1)
+            t = sim.stage_times()
+            acc = t if acc is None else {k: acc[k] + t[k] for k in t}
+        stage_ms = {k: v / nprof for k, v in acc.items()}
+        pos, _ = sim.state()
+        finite = bool(np.all(np.isfinite(pos)))
+    finally:
+        sim.close()
+    return dict(seconds=dt, ms_per_step=1e3 * dt / steps, c_occ=c_occ, stage_ms=stage_ms, finite=finite)
+
+
+def cpu_baseline(scene, seconds_budget=15.0):
+    from oracle import oracle
+    threads = oracle.num_threads()
+    pos, vel = scene.pos.copy(), scene.vel.copy()
+    rho_p = np.zeros((scene.n, 2), np.float32)
+    steps, t_total = 0, 0.0
+    while t_total < seconds_budget and steps < 50:
+        t0 = time.perf_counter()
+        st = oracle.step(pos, vel, scene.fluid, scene.common, scene.kernels, scene.key_mode, rho_p_prev=rho_p)
+        t_total += time.perf_counter() - t0
+        pos, vel, rho_p = st.pos, st.vel, st.rho_p
+        steps += 1
+    return dict(value=scene.n * steps / t_total, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"{steps} oracle steps of the {scene.n:,}-particle scene ({t_total:.1f} s, "
+                       f"{threads} OpenMP threads)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-8m", action="store_true")
+    ap.add_argument("--unfused", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (this build runs on one GPU "
+              "and shards nothing across GPUs)", file=sys.stderr)
+        sys.exit(2)
+    lib.load()
+    if lib.gpu_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    log("device:", lib.device_name(0))
+
+    scene = scenes.config_scene(a.config)
+    fused = not a.unfused
+    log(f"scene {scene.name}: n={scene.n:,} buckets={scene.num_buckets:,}")
+    r = measure(scene, a.steps, a.warmup, fused)
+    n = scene.n
+    value = n * a.steps / r["seconds"]
+    stage_ms = r["stage_ms"]
+    dom = max((k for k in stage_ms if stage_ms[k] > 0), key=lambda k: stage_ms[k])
+    dom_bytes = stage_bytes(dom, n, r["c_occ"], fused)
+    achieved = dom_bytes / (stage_ms[dom] * 1e-3)
+    step_bytes = 244 * n + 4 * scene.num_buckets + 24 * r["c_occ"]
+    step_gbs = step_bytes / (r["ms_per_step"] * 1e-3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            t = json.load(open(tpath))
+            traffic = t.get(f"cfg{a.config}", {}).get(dom)
+        except Exception:
+            traffic = None
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain FP32 dam break, config {a.config}: {n:,} particles, "
+                        f"{scene.num_buckets:,} LINEAR buckets, h={scene.info['h']}, dt={scene.info['dt']}, "
+                        "fluid only (no rigid body)",
+            "particles": n,
+            "buckets": scene.num_buckets,
+            "occupied_cells": r["c_occ"],
+            "form": "plain FP32, fused forces+integrate, HIP graph replay" if fused else "plain FP32, unfused",
+            "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+            "step_algorithmic_GBps": step_gbs / 1e9,
+            "step_hbm_frac": step_gbs / HBM_PEAK,
+            "finite": r["finite"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom if not (dom == "forces" and fused) else "forces+integrate (fused)",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": dom_bytes,
+        },
+    }
+    if not a.no_8m and a.config != 2:
+        s8 = scenes.config_scene(2)
+        log(f"scene {s8.name}: n={s8.n:,}")
+        k8 = max(20, a.steps // 4)
+        r8 = measure(s8, k8, max(5, a.warmup // 2), fused)
+        out["config"]["also_8M"] = {
+            "particles": s8.n, "steps": k8, "value": s8.n * k8 / r8["seconds"], "ms_per_step": r8["ms_per_step"],
+            "step_hbm_frac": (244 * s8.n + 4 * s8.num_buckets + 24 * r8["c_occ"]) / (r8["ms_per_step"] * 1e-3)
+            / HBM_PEAK,
+            "stage_ms": {k: round(v, 5) for k, v in r8["stage_ms"].items()},
+        }
+    if not a.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(scene)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

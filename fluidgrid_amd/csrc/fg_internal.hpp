@@ -1,0 +1,154 @@
+// fg_internal.hpp -- shared host state, error handling and device helpers of
+// libfluidgrid_amd.so.  Device code here is written for gfx950 (wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/fluidgrid_amd.h"
+
+namespace fg {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+// ---------------------------------------------------------------------------
+// Host-side state (the reference keeps parameters in __constant__ memory,
+// K_FluidSim.cu:37-40; here they are held on the host and passed by value as
+// kernel arguments, which the hardware reads through the scalar cache).
+// ---------------------------------------------------------------------------
+struct State {
+    fg_common_fluid_params common{};
+    fg_kernels_cache kernels{};
+    bool have_common = false;
+    bool have_kernels = false;
+    int key_mode = FG_KEY_REF_XOR;
+    int math_mode = FG_MATH_FAST;
+    hipStream_t stream = nullptr;
+};
+
+State& state();
+
+// Thread-local error slot.
+void set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+bool check_hip(hipError_t e, const char* what);
+// Records the launch status of the last kernel; returns true on success.
+bool check_launch(const char* what);
+
+// Fill kernel launcher (graph-safe replacement for hipMemsetAsync).
+void launch_fill_u32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t s);
+
+// Geometry of one bucket grid: the fluid grid, or a body-sphere grid.
+struct GridDesc {
+    float3 vmin;       // LINEAR origin
+    float3 cs;         // cell size
+    int3 dims;         // LINEAR cells per axis
+    uint32_t nb;       // number of buckets (key range)
+    int mode;          // FG_KEY_REF_XOR or FG_KEY_LINEAR
+};
+
+inline uint32_t num_buckets(const fg_common_fluid_params& p) {
+    return (uint32_t)p.bucket_size.x * (uint32_t)p.bucket_size.y * (uint32_t)p.bucket_size.z;
+}
+
+inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+// float -> int with truncation, saturation and NaN -> 0, spelled out so the
+// result does not depend on how the compiler lowers an out-of-range cast.
+__device__ __forceinline__ int32_t f2i_sat(float v) {
+    if (v != v) return 0;
+    if (v >= 2147483648.0f) return INT32_MAX;
+    if (v <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)v;
+}
+
+// REF_XOR key: K_DeviceFunctions.cu:17-26.  The add and divide are IEEE
+// single operations (hipcc rounds float division correctly by default), so
+// keys are bit-identical to a host evaluation of the same expression.
+__device__ __forceinline__ uint32_t hash_ref_xor(float px, float py, float pz, float csx, float csy,
+                                                 float csz, uint32_t table) {
+    int32_t x = f2i_sat(__fdiv_rn(__fadd_rn(px, 5000.0f), csx));
+    int32_t y = f2i_sat(__fdiv_rn(__fadd_rn(py, 5000.0f), csy));
+    int32_t z = f2i_sat(__fdiv_rn(__fadd_rn(pz, 5000.0f), csz));
+    uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+    return h % table;
+}
+
+// LINEAR cell coordinate: clamp(floor((p - vmin)/cs), 0, n-1).  Clamping in
+// float before the conversion gives the same answer as converting with
+// saturation first (NaN maps to 0 in both).
+__device__ __forceinline__ int cell_coord(float p, float vmin, float cs, int n) {
+    float q = floorf(__fdiv_rn(__fsub_rn(p, vmin), cs));
+    q = fminf(fmaxf(q, 0.0f), (float)(n - 1));
+    return (int)q;
+}
+
+struct CellXYZ { int x, y, z; };
+
+__device__ __forceinline__ CellXYZ grid_cell(float px, float py, float pz, const GridDesc& g) {
+    return {cell_coord(px, g.vmin.x, g.cs.x, g.dims.x), cell_coord(py, g.vmin.y, g.cs.y, g.dims.y),
+            cell_coord(pz, g.vmin.z, g.cs.z, g.dims.z)};
+}
+
+__device__ __forceinline__ uint32_t grid_linear_key(int cx, int cy, int cz, const GridDesc& g) {
+    return (uint32_t)cx + (uint32_t)g.dims.x * ((uint32_t)cy + (uint32_t)g.dims.y * (uint32_t)cz);
+}
+
+// Key of a position in either mode.
+__device__ __forceinline__ uint32_t grid_key(float px, float py, float pz, const GridDesc& g) {
+    if (g.mode == FG_KEY_REF_XOR) return hash_ref_xor(px, py, pz, g.cs.x, g.cs.y, g.cs.z, g.nb);
+    CellXYZ c = grid_cell(px, py, pz, g);
+    return grid_linear_key(c.x, c.y, c.z, g);
+}
+
+__device__ __forceinline__ CellXYZ linear_cell(float px, float py, float pz, const fg_common_fluid_params& cp) {
+    return {cell_coord(px, cp.volume_min.x, cp.cell_size.x, cp.bucket_size.x),
+            cell_coord(py, cp.volume_min.y, cp.cell_size.y, cp.bucket_size.y),
+            cell_coord(pz, cp.volume_min.z, cp.cell_size.z, cp.bucket_size.z)};
+}
+
+__device__ __forceinline__ uint32_t linear_key(int cx, int cy, int cz, const fg_common_fluid_params& cp) {
+    return (uint32_t)cx + (uint32_t)cp.bucket_size.x * ((uint32_t)cy + (uint32_t)cp.bucket_size.y * (uint32_t)cz);
+}
+
+__device__ __forceinline__ uint32_t dev_num_buckets(const fg_common_fluid_params& cp) {
+    return (uint32_t)cp.bucket_size.x * (uint32_t)cp.bucket_size.y * (uint32_t)cp.bucket_size.z;
+}
+
+// Small float3 helpers with explicit rounding for the reference-arithmetic
+// kernels.  Contraction is switched off for this region so that no multiply
+// and add below is fused after inlining.
+#pragma clang fp contract(off)
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return {__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y), __fadd_rn(a.z, b.z)}; }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y), __fsub_rn(a.z, b.z)}; }
+__device__ __forceinline__ F3 mul(F3 a, float s) { return {__fmul_rn(a.x, s), __fmul_rn(a.y, s), __fmul_rn(a.z, s)}; }
+__device__ __forceinline__ F3 mul(float s, F3 a) { return {__fmul_rn(s, a.x), __fmul_rn(s, a.y), __fmul_rn(s, a.z)}; }
+__device__ __forceinline__ F3 divs(F3 a, float s) { return {__fdiv_rn(a.x, s), __fdiv_rn(a.y, s), __fdiv_rn(a.z, s)}; }
+__device__ __forceinline__ float dot(F3 a, F3 b) {
+    return __fadd_rn(__fadd_rn(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y)), __fmul_rn(a.z, b.z));
+}
+__device__ __forceinline__ float len(F3 a) { return __fsqrt_rn(dot(a, a)); }
+__device__ __forceinline__ F3 xyz(float4 p) { return {p.x, p.y, p.z}; }
+#pragma clang fp contract(fast)
+
+}  // namespace fg
+
+// Reinterpretation between the public POD vector types and HIP's.
+static_assert(sizeof(fg_float4) == sizeof(float4) && alignof(fg_float4) == alignof(float4), "float4 layout");
+static_assert(sizeof(fg_float2) == sizeof(float2) && alignof(fg_float2) == alignof(float2), "float2 layout");
+static_assert(sizeof(fg_float3) == sizeof(float3), "float3 layout");
+static_assert(sizeof(fg_common_fluid_params) == 104, "CommonFluidParams layout");
+static_assert(sizeof(fg_common_rigid_body_params) == 112, "CommonRigidBodyParams layout");
+static_assert(sizeof(fg_kernels_cache) == 12, "KernelsCache layout");
+static_assert(sizeof(fg_fluid_params) == 28, "FluidParams layout");
+
+#define FG_F4(p) reinterpret_cast<float4*>(p)
+#define FG_F2(p) reinterpret_cast<float2*>(p)

@@ -1,0 +1,444 @@
+// fg_sort.hip -- stable LSD radix sort of (u32 key, u32 value) pairs.
+//
+// Replaces the reference's thrust::sort_by_key (GPU_Kernels/K_FluidSim.cu:685-691)
+// and the CUDPP key/value radix sorts (Simulators/RigidBodySimulator.cpp:195,
+// 469, 542).  Ties keep their input order, so the result is the unique stable
+// permutation of the keys.
+//
+// Design (wave64, gfx950):
+//  * one upfront kernel builds the digit histograms of every pass in a single
+//    read of the keys (per-wave LDS sub-histograms, then global atomics);
+//  * each digit pass is a single sweep: a 256-lane workgroup owns a tile of
+//    4096 consecutive elements, ranks them stably with wave-wide ballot
+//    matching (peers with an equal digit found bit by bit), publishes its
+//    per-digit counts and finds its global offsets by decoupled look-back
+//    over the preceding tiles, stages the tile in LDS in digit order and
+//    writes coalesced runs;
+//  * tiles are numbered by an atomic counter at block start, so every tile a
+//    block waits for belongs to a block that is already running;
+//  * digit widths are chosen from the key width (<= 8 bits per pass), so
+//    16-bit keys take two passes, 22-bit keys three, 25/27-bit keys four.
+#include "fg_internal.hpp"
+
+namespace fg {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kMaxPasses = 4;
+constexpr int kMaxRadix = 256;
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagInclusive = 2u << 30;
+constexpr uint32_t kCountMask = (1u << 30) - 1;
+
+struct SortPlan {
+    int passes;
+    int shift[kMaxPasses];
+    int bits[kMaxPasses];
+};
+
+static SortPlan make_plan(int keyBits) {
+    SortPlan p{};
+    if (keyBits < 1) keyBits = 1;
+    if (keyBits > 32) keyBits = 32;
+    p.passes = (keyBits + 7) / 8;
+    const int base = keyBits / p.passes, rem = keyBits % p.passes;
+    int s = 0;
+    for (int i = 0; i < p.passes; i++) {
+        p.bits[i] = base + (i < rem ? 1 : 0);
+        p.shift[i] = s;
+        s += p.bits[i];
+    }
+    return p;
+}
+
+struct SortLayout {
+    size_t hist, offs, counters, status, t1k, t1v, t2k, t2v, total;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static SortLayout make_layout(uint32_t n, const SortPlan& plan) {
+    SortLayout L{};
+    const size_t tiles = (n + kSortTile - 1) / kSortTile;
+    size_t o = 0;
+    L.hist = o; o = align256(o + sizeof(uint32_t) * kMaxPasses * kMaxRadix);
+    L.offs = o; o = align256(o + sizeof(uint32_t) * kMaxPasses * kMaxRadix);
+    L.counters = o; o = align256(o + sizeof(uint32_t) * kMaxPasses + sizeof(uint32_t));
+    L.status = o; o = align256(o + sizeof(uint32_t) * plan.passes * tiles * kMaxRadix);
+    L.t1k = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
+    L.t1v = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
+    if (plan.passes >= 3) {
+        L.t2k = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
+        L.t2v = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
+    } else {
+        L.t2k = L.t2v = 0;
+    }
+    L.total = o;
+    return L;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const uint32_t lane = __lane_id();
+    return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// Zeroes the digit histograms and tile counters of one sort call.
+__global__ void k_sort_zero(uint32_t* __restrict__ hist, uint32_t* __restrict__ counters) {
+    for (int i = threadIdx.x; i < kMaxPasses * kMaxRadix; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x <= kMaxPasses) counters[threadIdx.x] = 0;
+}
+
+// Digit histograms of all passes in one read; also clears the look-back
+// status words of every pass.
+__global__ void __launch_bounds__(kSortThreads) k_sort_histogram(const uint32_t* __restrict__ keys, uint32_t n,
+                                                                 SortPlan plan, uint32_t* __restrict__ hist,
+                                                                 uint32_t* __restrict__ status,
+                                                                 uint64_t status_words) {
+    __shared__ uint32_t s_hist[kSortWaves][kMaxPasses][kMaxRadix];
+    const int wave = threadIdx.x / 64;
+    for (int i = threadIdx.x; i < kSortWaves * kMaxPasses * kMaxRadix; i += blockDim.x)
+        (&s_hist[0][0][0])[i] = 0;
+    const uint64_t gstride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < status_words; i += gstride) status[i] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gstride) {
+        const uint32_t k = keys[i];
+        for (int p = 0; p < plan.passes; p++) {
+            const uint32_t d = (k >> plan.shift[p]) & ((1u << plan.bits[p]) - 1u);
+            atomicAdd(&s_hist[wave][p][d], 1u);
+        }
+    }
+    __syncthreads();
+    for (int p = 0; p < plan.passes; p++) {
+        const int radix = 1 << plan.bits[p];
+        for (int d = threadIdx.x; d < radix; d += blockDim.x) {
+            uint32_t c = 0;
+            for (int w = 0; w < kSortWaves; w++) c += s_hist[w][p][d];
+            if (c) atomicAdd(&hist[p * kMaxRadix + d], c);
+        }
+    }
+}
+
+// Exclusive scan of each pass's digit counts -> global digit offsets.
+__global__ void __launch_bounds__(kMaxRadix) k_sort_offsets(const uint32_t* __restrict__ hist,
+                                                            uint32_t* __restrict__ offs, int passes) {
+    __shared__ uint32_t s[kMaxRadix];
+    const int t = threadIdx.x;
+    for (int p = 0; p < passes; p++) {
+        s[t] = hist[p * kMaxRadix + t];
+        __syncthreads();
+        for (int d = 1; d < kMaxRadix; d <<= 1) {
+            uint32_t v = (t >= d) ? s[t - d] : 0u;
+            __syncthreads();
+            s[t] += v;
+            __syncthreads();
+        }
+        offs[p * kMaxRadix + t] = s[t] - hist[p * kMaxRadix + t];
+        __syncthreads();
+    }
+}
+
+// One digit pass.  Element e of tile T is base + w*64*ITEMS + k*64 + lane,
+// so (wave, item, lane) order is index order and the ranking is stable.
+__global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __restrict__ kin,
+                                                            const uint32_t* __restrict__ vin,
+                                                            uint32_t* __restrict__ kout,
+                                                            uint32_t* __restrict__ vout, uint32_t n, int shift,
+                                                            int bits, const uint32_t* __restrict__ offs,
+                                                            uint32_t* __restrict__ status,
+                                                            uint32_t* __restrict__ counter, int values_iota) {
+    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_wcnt[kSortWaves][kMaxRadix];
+    __shared__ uint32_t s_tstart[kMaxRadix];
+    __shared__ uint32_t s_gbase[kMaxRadix];
+    __shared__ uint32_t s_tile;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t radix = 1u << bits, dmask = radix - 1u;
+    if (tid == 0) s_tile = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = tid; i < kSortWaves * kMaxRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * (uint32_t)kSortTile;
+    const uint32_t wbase = base + (uint32_t)wave * 64u * kSortItems;
+
+    uint32_t key[kSortItems], val[kSortItems], rnk[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+        const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
+        const bool valid = idx < n;
+        key[k] = valid ? kin[idx] : 0u;
+        val[k] = valid ? (values_iota ? idx : vin[idx]) : 0u;
+    }
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+        const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
+        const bool valid = idx < n;
+        const uint32_t d = (key[k] >> shift) & dmask;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < bits; b++) {
+            const bool set = (d >> b) & 1u;
+            const uint64_t bal = __ballot(set);
+            peers &= set ? bal : ~bal;
+        }
+        uint32_t r = 0;
+        if (valid) {
+            const uint32_t before = s_wcnt[wave][d];
+            r = before + (uint32_t)__popcll(peers & lt);
+            if ((uint32_t)(__ffsll((long long)peers) - 1) == (uint32_t)lane)
+                s_wcnt[wave][d] = before + (uint32_t)__popcll(peers);
+        }
+        rnk[k] = r;
+    }
+    __syncthreads();
+
+    // Per digit: offsets of each wave inside the tile, tile total, publish.
+    uint32_t tile_count = 0;
+    if ((uint32_t)tid < radix) {
+        uint32_t run = 0;
+        for (int w = 0; w < kSortWaves; w++) {
+            const uint32_t c = s_wcnt[w][tid];
+            s_wcnt[w][tid] = run;
+            run += c;
+        }
+        tile_count = run;
+        const uint32_t flag = (tile == 0) ? kFlagInclusive : kFlagAggregate;
+        __hip_atomic_store(&status[(size_t)tile * kMaxRadix + tid], flag | tile_count, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        s_tstart[tid] = tile_count;
+    }
+    __syncthreads();
+    // Exclusive scan over digits of the tile counts (Hillis-Steele in LDS).
+    for (uint32_t d = 1; d < radix; d <<= 1) {
+        uint32_t v = 0;
+        if ((uint32_t)tid < radix && (uint32_t)tid >= d) v = s_tstart[tid - d];
+        __syncthreads();
+        if ((uint32_t)tid < radix) s_tstart[tid] += v;
+        __syncthreads();
+    }
+    if ((uint32_t)tid < radix) {
+        s_tstart[tid] -= tile_count;
+        // Decoupled look-back for this digit.
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int p = (int)tile - 1;
+            while (true) {
+                const uint32_t s = __hip_atomic_load(&status[(size_t)p * kMaxRadix + tid], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t flag = s & ~kCountMask;
+                if (flag == 0u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += s & kCountMask;
+                if (flag == kFlagInclusive) break;
+                p--;
+            }
+            __hip_atomic_store(&status[(size_t)tile * kMaxRadix + tid], kFlagInclusive | (excl + tile_count),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_gbase[tid] = offs[tid] + excl;
+    }
+    __syncthreads();
+
+    // Stage the tile in digit order.
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+        const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
+        if (idx < n) {
+            const uint32_t d = (key[k] >> shift) & dmask;
+            const uint32_t pos = s_tstart[d] + s_wcnt[wave][d] + rnk[k];
+            s_keys[pos] = key[k];
+            s_vals[pos] = val[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t count = (n - base) < (uint32_t)kSortTile ? (n - base) : (uint32_t)kSortTile;
+    for (uint32_t q = tid; q < count; q += kSortThreads) {
+        const uint32_t kk = s_keys[q];
+        const uint32_t d = (kk >> shift) & dmask;
+        const uint32_t dest = s_gbase[d] + (q - s_tstart[d]);
+        if (dest < n) {
+            kout[dest] = kk;
+            vout[dest] = s_vals[q];
+        }
+    }
+}
+
+__global__ void k_copy_pairs(uint32_t* __restrict__ kd, uint32_t* __restrict__ vd, const uint32_t* __restrict__ ks,
+                             const uint32_t* __restrict__ vs, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        kd[i] = ks[i];
+        vd[i] = vs[i];
+    }
+}
+
+__global__ void k_key_max(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ out) {
+    uint32_t m = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        m = max(m, keys[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// Persistent scratch for gSort/fg_sort_pairs.
+struct SortScratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+static SortScratch& scratch() {
+    static SortScratch s;
+    return s;
+}
+
+static void* ensure_scratch(size_t bytes) {
+    SortScratch& s = scratch();
+    if (s.bytes >= bytes) return s.ptr;
+    if (s.ptr) {
+        (void)hipStreamSynchronize(state().stream);
+        (void)hipFree(s.ptr);
+        s.ptr = nullptr;
+        s.bytes = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    if (!check_hip(hipMalloc(&s.ptr, want), "sort scratch hipMalloc")) return nullptr;
+    s.bytes = want;
+    return s.ptr;
+}
+
+static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, void* ws, size_t ws_bytes,
+                    bool iota, hipStream_t st) {
+    if (n == 0) return FG_OK;
+    if (n >= (1u << 30)) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "sort: n = %u exceeds 2^30 - 1", n);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    const SortPlan plan = make_plan(keyBits);
+    const SortLayout L = make_layout(n, plan);
+    if (ws_bytes < L.total) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "sort: workspace %zu bytes < %zu needed", ws_bytes, L.total);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    char* w = static_cast<char*>(ws);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(w + L.hist);
+    uint32_t* offs = reinterpret_cast<uint32_t*>(w + L.offs);
+    uint32_t* counters = reinterpret_cast<uint32_t*>(w + L.counters);
+    uint32_t* status = reinterpret_cast<uint32_t*>(w + L.status);
+    uint32_t* t1k = reinterpret_cast<uint32_t*>(w + L.t1k);
+    uint32_t* t1v = reinterpret_cast<uint32_t*>(w + L.t1v);
+    uint32_t* t2k = plan.passes >= 3 ? reinterpret_cast<uint32_t*>(w + L.t2k) : nullptr;
+    uint32_t* t2v = plan.passes >= 3 ? reinterpret_cast<uint32_t*>(w + L.t2v) : nullptr;
+    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t status_words = (uint64_t)plan.passes * tiles * kMaxRadix;
+
+    hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(256), 0, st, hist, counters);
+    unsigned hblocks = grid_for(n, kSortThreads * 8);
+    if (hblocks > 2048) hblocks = 2048;
+    if (hblocks == 0) hblocks = 1;
+    hipLaunchKernelGGL(k_sort_histogram, dim3(hblocks), dim3(kSortThreads), 0, st, keys, n, plan, hist, status,
+                       status_words);
+    hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(kMaxRadix), 0, st, hist, offs, plan.passes);
+
+    const uint32_t* src_k = keys;
+    const uint32_t* src_v = vals;
+    for (int p = 0; p < plan.passes; p++) {
+        uint32_t *dk, *dv;
+        if (plan.passes == 1) {
+            dk = t1k; dv = t1v;
+        } else if (p == plan.passes - 1) {
+            dk = keys; dv = vals;
+        } else if (p % 2 == 0) {
+            dk = t1k; dv = t1v;
+        } else {
+            dk = t2k; dv = t2v;
+        }
+        hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(kSortThreads), 0, st, src_k, src_v, dk, dv, n,
+                           plan.shift[p], plan.bits[p], offs + p * kMaxRadix,
+                           status + (size_t)p * tiles * kMaxRadix, counters + p, (p == 0 && iota) ? 1 : 0);
+        src_k = dk;
+        src_v = dv;
+    }
+    if (plan.passes == 1)
+        hipLaunchKernelGGL(k_copy_pairs, dim3(grid_for(n, 256)), dim3(256), 0, st, keys, vals, t1k, t1v, n);
+    return check_launch("radix sort") ? FG_OK : fg_peek_error();
+}
+
+}  // namespace fg
+
+using namespace fg;
+
+extern "C" {
+
+size_t fg_sort_workspace_size(unsigned int n, int keyBits) { return make_layout(n, make_plan(keyBits)).total; }
+
+int fg_sort_pairs_ws(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits, void* workspace,
+                     size_t workspace_bytes, int values_are_iota) {
+    if (n && (!keys || !values || !workspace)) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sort_pairs_ws: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    if (keyBits < 1 || keyBits > 32) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sort_pairs_ws: keyBits %d outside 1..32", keyBits);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    return run_sort(keys, values, n, keyBits, workspace, workspace_bytes, values_are_iota != 0, state().stream);
+}
+
+static int sort_with_scratch(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits, bool iota) {
+    if (n == 0) return FG_OK;
+    if (!keys || !values) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sort_pairs: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    if (keyBits < 1 || keyBits > 32) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sort_pairs: keyBits %d outside 1..32", keyBits);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    const size_t need = fg_sort_workspace_size(n, keyBits);
+    void* ws = ensure_scratch(need);
+    if (!ws) return fg_peek_error();
+    return run_sort(keys, values, n, keyBits, ws, scratch().bytes, iota, state().stream);
+}
+
+int fg_sort_pairs(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits) {
+    return sort_with_scratch(keys, values, n, keyBits, false);
+}
+
+int fg_sort_pairs_iota(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits) {
+    return sort_with_scratch(keys, values, n, keyBits, true);
+}
+
+// K_FluidSim.cu:685-691.  The key width is read back from the device (one
+// 4-byte copy, which synchronises the stream, as thrust's call does).
+void gSort(unsigned int* _key, unsigned int* _values, int _numParticles) {
+    if (_numParticles <= 0) return;
+    if (!_key || !_values) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "gSort: null pointer");
+        return;
+    }
+    const uint32_t n = (uint32_t)_numParticles;
+    hipStream_t st = state().stream;
+    void* ws = ensure_scratch(fg_sort_workspace_size(n, 32));
+    if (!ws) return;
+    uint32_t* kmax = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + make_layout(n, make_plan(32)).counters) +
+                     kMaxPasses;
+    launch_fill_u32(kmax, 0u, 1, st);
+    unsigned blocks = grid_for(n, 256 * 16);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_key_max, dim3(blocks), dim3(256), 0, st, _key, n, kmax);
+    uint32_t host_max = 0;
+    if (!check_hip(hipMemcpyAsync(&host_max, kmax, 4, hipMemcpyDeviceToHost, st), "gSort readback") ||
+        !check_hip(hipStreamSynchronize(st), "gSort sync"))
+        return;
+    int bits = 1;
+    while (bits < 32 && (host_max >> bits) != 0) bits++;
+    run_sort(_key, _values, n, bits, ws, scratch().bytes, false, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// GPUSPHFluid.cpp -- device buffers of one fluid (GPUSPHFluid.cpp:9-56).
+#include "fluidgrid/GPUSPHFluid.h"
+
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace Simulation {
+namespace Fluid {
+
+namespace {
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* dalloc(size_t count, const char* what) {
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, sizeof(T) * (count ? count : 1)), what);
+    return static_cast<T*>(p);
+}
+
+int key_bits_for(unsigned int numBuckets) {
+    int b = 1;
+    while (b < 32 && ((numBuckets - 1u) >> b) != 0) b++;
+    return b;
+}
+}  // namespace
+
+GPUSPHFluid::GPUSPHFluid(Application::FluidParams fluidParams, Emitter::IEmitter* emitter)
+    : mFluidParams(fluidParams), mEmitter(emitter) {
+    const Application::CommonFluidParams& cp = Application::ApplicationData::instance()->getCommonFluidParams();
+    const int n = mFluidParams.max_particles;
+    if (n < 0) throw std::invalid_argument("GPUSPHFluid: max_particles < 0");
+    const long long nb = (long long)cp.bucket_size.x * cp.bucket_size.y * cp.bucket_size.z;
+    if (nb <= 0 || nb > 0xFFFFFFFFll) throw std::invalid_argument("GPUSPHFluid: invalid bucket_size");
+    mNumBuckets = (unsigned int)nb;
+
+    d_particleId = dalloc<unsigned int>(n, "hipMalloc particleId");
+    d_hashKey = dalloc<unsigned int>(n, "hipMalloc hashKey");
+    d_gridPosStart = dalloc<unsigned int>(mNumBuckets, "hipMalloc gridPosStart");
+    d_gridPosEnd = dalloc<unsigned int>(mNumBuckets, "hipMalloc gridPosEnd");
+    d_position = dalloc<float4>(n, "hipMalloc position");
+    d_ordered_position = dalloc<float4>(n, "hipMalloc ordered_position");
+    d_forces = dalloc<float4>(n, "hipMalloc forces");
+    d_mass_pressure = dalloc<float2>(n, "hipMalloc mass_pressure");
+    d_velocity = dalloc<float4>(n, "hipMalloc velocity");
+    d_ordered_vel = dalloc<float4>(n, "hipMalloc ordered_vel");
+    mSortWorkspaceBytes = fg_sort_workspace_size((unsigned)n, key_bits_for(mNumBuckets));
+    hip_check(hipMalloc(&d_sortWorkspace, mSortWorkspaceBytes ? mSortWorkspaceBytes : 256), "hipMalloc sort ws");
+
+    // GPUSPHFluid.cpp:27-29 zeroes the velocities; the density field is
+    // zeroed here as well (the reference leaves it uninitialised).
+    hip_check(hipMemset(d_velocity, 0, sizeof(float4) * n), "hipMemset velocity");
+    hip_check(hipMemset(d_ordered_vel, 0, sizeof(float4) * n), "hipMemset ordered_vel");
+    hip_check(hipMemset(d_mass_pressure, 0, sizeof(float2) * n), "hipMemset mass_pressure");
+    hip_check(hipMemset(d_forces, 0, sizeof(float4) * n), "hipMemset forces");
+    hip_check(hipMemset(d_gridPosEnd, 0, sizeof(unsigned int) * mNumBuckets), "hipMemset gridPosEnd");
+
+    std::vector<float4> host(n > 0 ? n : 1);
+    if (mEmitter) mEmitter->reset(host.data(), n);
+    if (n > 0) hip_check(hipMemcpy(d_position, host.data(), sizeof(float4) * n, hipMemcpyHostToDevice), "upload");
+    if (auto* ae = dynamic_cast<Emitter::ArrayEmitter*>(mEmitter)) {
+        if (ae->velocities() && n > 0)
+            hip_check(hipMemcpy(d_velocity, ae->velocities(), sizeof(float4) * n, hipMemcpyHostToDevice), "upload");
+    }
+}
+
+GPUSPHFluid::~GPUSPHFluid() {
+    (void)hipFree(d_particleId);
+    (void)hipFree(d_hashKey);
+    (void)hipFree(d_gridPosStart);
+    (void)hipFree(d_gridPosEnd);
+    (void)hipFree(d_position);
+    (void)hipFree(d_ordered_position);
+    (void)hipFree(d_forces);
+    (void)hipFree(d_mass_pressure);
+    (void)hipFree(d_velocity);
+    (void)hipFree(d_ordered_vel);
+    (void)hipFree(d_sortWorkspace);
+}
+
+int GPUSPHFluid::copyPositionToCPU(float4** particlePositions) {
+    const int n = mFluidParams.max_particles;
+    *particlePositions = static_cast<float4*>(std::malloc(sizeof(float4) * (n > 0 ? n : 1)));
+    if (n > 0) hip_check(hipMemcpy(*particlePositions, d_position, sizeof(float4) * n, hipMemcpyDeviceToHost),
+                         "copyPositionToCPU");
+    return n;
+}
+
+void GPUSPHFluid::uploadState(const float4* pos, const float4* vel) {
+    const int n = mFluidParams.max_particles;
+    if (n <= 0) return;
+    if (pos) hip_check(hipMemcpy(d_position, pos, sizeof(float4) * n, hipMemcpyHostToDevice), "uploadState");
+    if (vel) hip_check(hipMemcpy(d_velocity, vel, sizeof(float4) * n, hipMemcpyHostToDevice), "uploadState");
+}
+
+void GPUSPHFluid::downloadState(float4* pos, float4* vel) {
+    const int n = mFluidParams.max_particles;
+    if (n <= 0) return;
+    hip_check(hipDeviceSynchronize(), "downloadState sync");
+    if (pos) hip_check(hipMemcpy(pos, d_position, sizeof(float4) * n, hipMemcpyDeviceToHost), "downloadState");
+    if (vel) hip_check(hipMemcpy(vel, d_velocity, sizeof(float4) * n, hipMemcpyDeviceToHost), "downloadState");
+}
+
+}  // namespace Fluid
+}  // namespace Simulation

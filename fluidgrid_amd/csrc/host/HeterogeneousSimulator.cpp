@@ -1,0 +1,221 @@
+// HeterogeneousSimulator.cpp -- per-step orchestration of the SPH pipeline
+// (HeterogeneousSimulator.cpp:19-212 of the reference).
+#include "fluidgrid/HeterogeneousSimulator.h"
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace Simulation {
+namespace Simulators {
+
+HeterogeneousSimulator* HeterogeneousSimulator::mInstance = nullptr;
+
+namespace {
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void fg_check(const char* what) {
+    int e = fg_peek_error();
+    if (e != FG_OK) {
+        std::string msg = std::string(what) + ": " + fg_last_error_string();
+        (void)fg_last_error();
+        throw std::runtime_error(msg);
+    }
+}
+
+int key_bits_for(unsigned int numBuckets) {
+    int b = 1;
+    while (b < 32 && ((numBuckets - 1u) >> b) != 0) b++;
+    return b;
+}
+
+// Makes `s` the launch stream for the duration of a scope.
+struct StreamScope {
+    void* prev;
+    explicit StreamScope(hipStream_t s) : prev(fg_get_stream()) { fg_set_stream(s); }
+    ~StreamScope() { fg_set_stream(prev); }
+};
+
+fg_float3 abi3(float3 v) { return fg_float3{v.x, v.y, v.z}; }
+}  // namespace
+
+HeterogeneousSimulator* HeterogeneousSimulator::instance() {
+    if (mInstance == nullptr) mInstance = new HeterogeneousSimulator();
+    return mInstance;
+}
+
+void HeterogeneousSimulator::destroy() {
+    delete mInstance;
+    mInstance = nullptr;
+}
+
+// HeterogeneousSimulator.cpp:27-47: kernel constants with the float PI and
+// pow(h, n) in double, one rounding to float; then both parameter blocks
+// are handed to the launchers.
+HeterogeneousSimulator::HeterogeneousSimulator() {
+    using namespace Application;
+    CommonFluidParams& cp = ApplicationData::instance()->getCommonFluidParams();
+    KernelsCache kc;
+    const double h = cp.kernel_radius;
+    kc.kPoly6 = (float)(315.0f / (64.0f * PI * std::pow(h, 9.0)));
+    kc.kSpikyGradient = (float)(-(45.0f / (PI * std::pow(h, 6.0))));
+    kc.kViscosityLaplacian = (float)(45.0f / (PI * std::pow(h, 6.0)));
+    gInitializeKernelParams(abi_cast<fg_kernels_cache>(kc));
+    ApplicationData::instance()->setKernelsCache(kc);
+    gInitializeCommonFluidParams(abi_cast<fg_common_fluid_params>(cp));
+
+    hip_check(hipStreamCreateWithFlags(&mStream, hipStreamNonBlocking), "hipStreamCreate");
+    mOwnStream = true;
+    for (auto& e : mEv) hip_check(hipEventCreate(&e), "hipEventCreate");
+}
+
+HeterogeneousSimulator::~HeterogeneousSimulator() {
+    invalidateGraph();
+    if (mStream) (void)hipStreamSynchronize(mStream);
+    for (auto& e : mEv)
+        if (e) (void)hipEventDestroy(e);
+    if (mOwnStream && mStream) (void)hipStreamDestroy(mStream);
+}
+
+void HeterogeneousSimulator::addSphFluid(Fluid::GPUSPHFluid* fluid) {
+    mFluids.push_back(fluid);
+    invalidateGraph();
+}
+
+void HeterogeneousSimulator::setOptions(const StepOptions& o) {
+    mOptions = o;
+    invalidateGraph();
+}
+
+void HeterogeneousSimulator::setStream(hipStream_t s) {
+    invalidateGraph();
+    if (mOwnStream && mStream) {
+        (void)hipStreamSynchronize(mStream);
+        (void)hipStreamDestroy(mStream);
+    }
+    mStream = s;
+    mOwnStream = false;
+}
+
+void HeterogeneousSimulator::invalidateGraph() {
+    if (mGraphExec) (void)hipGraphExecDestroy(mGraphExec);
+    if (mGraph) (void)hipGraphDestroy(mGraph);
+    mGraphExec = nullptr;
+    mGraph = nullptr;
+}
+
+// HeterogeneousSimulator.cpp:131-150
+void HeterogeneousSimulator::findFluidNeighbourhood(Fluid::GPUSPHFluid* fluid) {
+    const Application::CommonFluidParams& cp = Application::ApplicationData::instance()->getCommonFluidParams();
+    const unsigned int nb = (unsigned int)cp.getNumBuckets();
+    const unsigned int n = (unsigned int)fluid->getNumParticles();
+    if (mOptions.fused) {
+        // Keys only; the first sort pass generates the identity ids.
+        fg_calc_hash_key_only(fluid->getHashKey(), reinterpret_cast<fg_float4*>(fluid->getPosition()), n, nb,
+                              abi3(cp.cell_size));
+    } else {
+        gCalcHashKey(fluid->getHashKey(), fluid->getParticleID(), reinterpret_cast<fg_float4*>(fluid->getPosition()),
+                     n, nb, abi3(cp.cell_size));
+    }
+    if (mOptions.stage_timing) (void)hipEventRecord(mEv[1], mStream);
+    fg_sort_pairs_ws(fluid->getHashKey(), fluid->getParticleID(), n, key_bits_for(nb), fluid->getSortWorkspace(),
+                     fluid->getSortWorkspaceBytes(), mOptions.fused ? 1 : 0);
+    if (mOptions.stage_timing) (void)hipEventRecord(mEv[2], mStream);
+    gFindCellOffsetOrderingPosVel(fluid->getHashStart(), fluid->getHashEnd(), fluid->getHashKey(),
+                                  fluid->getParticleID(), reinterpret_cast<fg_float4*>(fluid->getPosition()),
+                                  reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
+                                  reinterpret_cast<fg_float4*>(fluid->getVelocity()),
+                                  reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()), n, nb);
+}
+
+// HeterogeneousSimulator.cpp:153-159
+void HeterogeneousSimulator::calculateFluidDensity(Fluid::GPUSPHFluid* fluid) {
+    gCalcMassDensity(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
+                     reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashKey(),
+                     fluid->getHashStart(), fluid->getHashEnd(), fluid->getNumParticles(), fluid->getGasConstant(),
+                     fluid->getRestDensity(), fluid->getMass());
+}
+
+// HeterogeneousSimulator.cpp:161-205 (the coupled branch is wired by the
+// rigid-body simulator through gSPHCalculateForcesWithSRB).
+void HeterogeneousSimulator::calculateFluidForces(Fluid::GPUSPHFluid* fluid, bool, bool) {
+    if (mOptions.fused) {
+        fg_sph_forces_integrate(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
+                                reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()),
+                                reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashStart(),
+                                fluid->getHashEnd(), reinterpret_cast<fg_float4*>(fluid->getVelocity()),
+                                reinterpret_cast<fg_float4*>(fluid->getPosition()), fluid->getParticleID(),
+                                (unsigned)fluid->getNumParticles(), fluid->getMass(), fluid->getViscosity());
+    } else {
+        gSPHCalculateForces(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
+                            reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()),
+                            reinterpret_cast<fg_float4*>(fluid->getForces()),
+                            reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashStart(),
+                            fluid->getHashEnd(), reinterpret_cast<fg_float4*>(fluid->getVelocity()),
+                            fluid->getParticleID(), (unsigned)fluid->getNumParticles(), fluid->getMass(),
+                            fluid->getViscosity());
+    }
+}
+
+// HeterogeneousSimulator.cpp:207-212
+void HeterogeneousSimulator::integrateFluidForces(Fluid::GPUSPHFluid* fluid) {
+    gSPHIntegrate(reinterpret_cast<fg_float4*>(fluid->getPosition()),
+                  reinterpret_cast<fg_float4*>(fluid->getVelocity()), reinterpret_cast<fg_float4*>(fluid->getForces()),
+                  (unsigned)fluid->getNumParticles(), reinterpret_cast<fg_float2*>(fluid->getDensityField()),
+                  fluid->getHashStart(), fluid->getHashEnd());
+}
+
+void HeterogeneousSimulator::enqueueStep() {
+    const bool t = mOptions.stage_timing;
+    for (Fluid::GPUSPHFluid* f : mFluids) {
+        f->beginUpdate();
+        if (t) (void)hipEventRecord(mEv[0], mStream);
+        findFluidNeighbourhood(f);
+        if (t) (void)hipEventRecord(mEv[3], mStream);
+        calculateFluidDensity(f);
+        if (t) (void)hipEventRecord(mEv[4], mStream);
+        calculateFluidForces(f, false, false);
+        if (t) (void)hipEventRecord(mEv[5], mStream);
+        if (!mOptions.fused) integrateFluidForces(f);
+        if (t) (void)hipEventRecord(mEv[6], mStream);
+        f->endUpdate();
+    }
+}
+
+// HeterogeneousSimulator.cpp:54-127
+void HeterogeneousSimulator::doTimeStep() {
+    StreamScope scope(mStream);
+    enqueueStep();
+    fg_check("doTimeStep");
+    if (mOptions.stage_timing && !mFluids.empty()) {
+        hip_check(hipEventSynchronize(mEv[6]), "stage timing");
+        float* out[6] = {&mTimes.hash_ms, &mTimes.sort_ms, &mTimes.cells_ms, &mTimes.density_ms, &mTimes.forces_ms,
+                         &mTimes.integrate_ms};
+        for (int k = 0; k < 6; k++) (void)hipEventElapsedTime(out[k], mEv[k], mEv[k + 1]);
+    }
+}
+
+void HeterogeneousSimulator::doTimeSteps(int steps) {
+    if (steps <= 0) return;
+    if (!mOptions.use_graph || mOptions.stage_timing) {
+        for (int s = 0; s < steps; s++) doTimeStep();
+        return;
+    }
+    StreamScope scope(mStream);
+    if (!mGraphExec) {
+        hip_check(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        enqueueStep();
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamEndCapture(mStream, &g), "hipStreamEndCapture");
+        fg_check("capture");
+        mGraph = g;
+        hip_check(hipGraphInstantiate(&mGraphExec, mGraph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    for (int s = 0; s < steps; s++) hip_check(hipGraphLaunch(mGraphExec, mStream), "hipGraphLaunch");
+}
+
+}  // namespace Simulators
+}  // namespace Simulation

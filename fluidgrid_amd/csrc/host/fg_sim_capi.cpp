@@ -1,0 +1,171 @@
+// fg_sim_capi.cpp -- C-ABI over the C++ HeterogeneousSimulator mirror, for
+// callers without a C++ toolchain (bench and tests drive the product path
+// through it).  One process-wide simulator, as in the reference.
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "fluidgrid/HeterogeneousSimulator.h"
+#include "fluidgrid_amd_sim.h"
+
+using namespace Simulation;
+
+namespace {
+struct SimState {
+    std::vector<std::unique_ptr<Emitter::ArrayEmitter>> emitters;
+    std::vector<std::unique_ptr<Fluid::GPUSPHFluid>> fluids;
+};
+
+SimState*& sim() {
+    static SimState* s = nullptr;
+    return s;
+}
+
+int fail(const std::exception& e) {
+    fg_record_error(FG_ERR_HIP, e.what());
+    return FG_ERR_HIP;
+}
+
+Fluid::GPUSPHFluid* fluid_at(int idx) {
+    if (!sim() || idx < 0 || idx >= (int)sim()->fluids.size()) return nullptr;
+    return sim()->fluids[idx].get();
+}
+}  // namespace
+
+extern "C" {
+
+int fg_sim_shutdown(void) {
+    try {
+        Simulators::HeterogeneousSimulator::destroy();
+        delete sim();
+        sim() = nullptr;
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_init(const fg_common_fluid_params* common, int key_mode, int math_mode) {
+    if (!common) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        fg_sim_shutdown();
+        int rc = fg_set_key_mode(key_mode);
+        if (rc) return rc;
+        rc = fg_set_math_mode(math_mode);
+        if (rc) return rc;
+        Application::ApplicationData::instance()->setCommonFluidParams(
+            Application::abi_cast<Application::CommonFluidParams>(*common));
+        sim() = new SimState();
+        (void)Simulators::HeterogeneousSimulator::instance();
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_add_fluid(const fg_fluid_params* params, const fg_float4* positions, const fg_float4* velocities) {
+    if (!sim() || !params || (params->max_particles > 0 && !positions)) return -FG_ERR_INVALID_ARGUMENT;
+    try {
+        auto em = std::make_unique<Emitter::ArrayEmitter>(reinterpret_cast<const float4*>(positions),
+                                                          reinterpret_cast<const float4*>(velocities),
+                                                          params->max_particles);
+        auto fl = std::make_unique<Fluid::GPUSPHFluid>(Application::abi_cast<Application::FluidParams>(*params),
+                                                       em.get());
+        Simulators::HeterogeneousSimulator::instance()->addSphFluid(fl.get());
+        sim()->emitters.push_back(std::move(em));
+        sim()->fluids.push_back(std::move(fl));
+        return (int)sim()->fluids.size() - 1;
+    } catch (const std::exception& e) {
+        return -fail(e);
+    }
+}
+
+int fg_sim_set_options(int fused, int use_graph, int stage_timing) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    Simulators::StepOptions o;
+    o.fused = fused != 0;
+    o.use_graph = use_graph != 0;
+    o.stage_timing = stage_timing != 0;
+    Simulators::HeterogeneousSimulator::instance()->setOptions(o);
+    return FG_OK;
+}
+
+int fg_sim_step(int steps) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        Simulators::HeterogeneousSimulator::instance()->doTimeSteps(steps);
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_synchronize(void) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    hipError_t e = hipStreamSynchronize(Simulators::HeterogeneousSimulator::instance()->stream());
+    if (e != hipSuccess) {
+        fg_record_error(FG_ERR_HIP, hipGetErrorString(e));
+        return FG_ERR_HIP;
+    }
+    return FG_OK;
+}
+
+void* fg_sim_stream(void) {
+    if (!sim()) return nullptr;
+    return reinterpret_cast<void*>(Simulators::HeterogeneousSimulator::instance()->stream());
+}
+
+int fg_sim_download(int idx, fg_float4* positions, fg_float4* velocities) {
+    Fluid::GPUSPHFluid* f = fluid_at(idx);
+    if (!f) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        f->downloadState(reinterpret_cast<float4*>(positions), reinterpret_cast<float4*>(velocities));
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_upload(int idx, const fg_float4* positions, const fg_float4* velocities) {
+    Fluid::GPUSPHFluid* f = fluid_at(idx);
+    if (!f) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        (void)hipStreamSynchronize(Simulators::HeterogeneousSimulator::instance()->stream());
+        f->uploadState(reinterpret_cast<const float4*>(positions), reinterpret_cast<const float4*>(velocities));
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_fluid_buffers(int idx, fg_fluid_buffers* out) {
+    Fluid::GPUSPHFluid* f = fluid_at(idx);
+    if (!f || !out) return FG_ERR_INVALID_ARGUMENT;
+    out->hashKey = f->getHashKey();
+    out->particleId = f->getParticleID();
+    out->gridStart = f->getHashStart();
+    out->gridEnd = f->getHashEnd();
+    out->position = f->getPosition();
+    out->orderedPosition = f->getOrderedPosition();
+    out->velocity = f->getVelocity();
+    out->orderedVelocity = f->getOrderedVelocities();
+    out->densityPressure = f->getDensityField();
+    out->forces = f->getForces();
+    out->numParticles = (unsigned)f->getNumParticles();
+    out->numBuckets = f->getNumBuckets();
+    return FG_OK;
+}
+
+int fg_sim_stage_times(float* out6) {
+    if (!sim() || !out6) return FG_ERR_INVALID_ARGUMENT;
+    const auto& t = Simulators::HeterogeneousSimulator::instance()->lastStageTimes();
+    out6[0] = t.hash_ms;
+    out6[1] = t.sort_ms;
+    out6[2] = t.cells_ms;
+    out6[3] = t.density_ms;
+    out6[4] = t.forces_ms;
+    out6[5] = t.integrate_ms;
+    return FG_OK;
+}
+
+}  // extern "C"

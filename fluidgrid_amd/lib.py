@@ -1,0 +1,255 @@
+"""ctypes bindings of libfluidgrid_amd.so (the C-ABI in include/*.h).
+
+Loading fails with an ImportError-like RuntimeError when the library has not
+been built: there is no fallback compute path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfluidgrid_amd.so")
+
+_lib = None
+
+vp = C.c_void_p
+u32 = C.c_uint
+i32 = C.c_int
+f32 = C.c_float
+sz = C.c_size_t
+
+# name -> (restype, argtypes) for every exported C function.
+SIGNATURES = {
+    # reference launchers
+    "gInitializeCommonFluidParams": (None, [abi.CommonFluidParams]),
+    "gInitializeKernelParams": (None, [abi.KernelsCache]),
+    "gCalcHashKey": (None, [vp, vp, vp, u32, u32, abi.Float3]),
+    "gCalcHashKeyWithOffset": (None, [vp, vp, vp, vp, vp, vp, u32, abi.Float3, i32]),
+    "gSort": (None, [vp, vp, i32]),
+    "gFindCellOffsetOrderingPosVel": (None, [vp, vp, vp, vp, vp, vp, vp, vp, u32, u32]),
+    "gFindCellOffsetOrderingPos": (None, [vp, vp, vp, vp, vp, vp, u32, u32]),
+    "gCalcMassDensity": (None, [vp, vp, vp, vp, vp, i32, f32, f32, f32]),
+    "gSPHCalculateForces": (None, [vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32]),
+    "gSPHIntegrate": (None, [vp, vp, vp, u32, vp, vp, vp]),
+    # extensions
+    "fg_last_error": (i32, []),
+    "fg_peek_error": (i32, []),
+    "fg_last_error_string": (C.c_char_p, []),
+    "fg_record_error": (None, [i32, C.c_char_p]),
+    "fg_version": (C.c_char_p, []),
+    "fg_set_stream": (i32, [vp]),
+    "fg_get_stream": (vp, []),
+    "fg_set_key_mode": (i32, [i32]),
+    "fg_get_key_mode": (i32, []),
+    "fg_set_math_mode": (i32, [i32]),
+    "fg_get_math_mode": (i32, []),
+    "fg_get_common_fluid_params": (i32, [C.POINTER(abi.CommonFluidParams)]),
+    "fg_get_kernels_cache": (i32, [C.POINTER(abi.KernelsCache)]),
+    "fg_sort_pairs": (i32, [vp, vp, u32, i32]),
+    "fg_sort_pairs_iota": (i32, [vp, vp, u32, i32]),
+    "fg_sort_workspace_size": (sz, [u32, i32]),
+    "fg_sort_pairs_ws": (i32, [vp, vp, u32, i32, vp, sz, i32]),
+    "fg_calc_hash_key_only": (i32, [vp, vp, u32, u32, abi.Float3]),
+    "fg_sph_forces_integrate": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32]),
+    "fg_device_count": (i32, [C.POINTER(C.c_int)]),
+    "fg_device_name": (i32, [i32, C.c_char_p, sz]),
+    "fg_malloc": (i32, [C.POINTER(C.c_void_p), sz]),
+    "fg_free": (i32, [vp]),
+    "fg_memcpy_htod": (i32, [vp, vp, sz]),
+    "fg_memcpy_dtoh": (i32, [vp, vp, sz]),
+    "fg_memcpy_dtod": (i32, [vp, vp, sz]),
+    "fg_memset_u32": (i32, [vp, u32, sz]),
+    "fg_synchronize": (i32, []),
+    "fg_event_create": (i32, [C.POINTER(C.c_void_p)]),
+    "fg_event_destroy": (i32, [vp]),
+    "fg_event_record": (i32, [vp]),
+    "fg_event_elapsed_ms": (i32, [vp, vp, C.POINTER(C.c_float)]),
+    # simulator
+    "fg_sim_init": (i32, [C.POINTER(abi.CommonFluidParams), i32, i32]),
+    "fg_sim_add_fluid": (i32, [C.POINTER(abi.FluidParams), vp, vp]),
+    "fg_sim_set_options": (i32, [i32, i32, i32]),
+    "fg_sim_step": (i32, [i32]),
+    "fg_sim_synchronize": (i32, []),
+    "fg_sim_stream": (vp, []),
+    "fg_sim_download": (i32, [i32, vp, vp]),
+    "fg_sim_upload": (i32, [i32, vp, vp]),
+    "fg_sim_fluid_buffers": (i32, [i32, vp]),
+    "fg_sim_stage_times": (i32, [vp]),
+    "fg_sim_shutdown": (i32, []),
+}
+
+
+class FluidBuffers(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("hashKey", "particleId", "gridStart", "gridEnd", "position",
+                                          "orderedPosition", "velocity", "orderedVelocity", "densityPressure",
+                                          "forces")] + [("numParticles", C.c_uint), ("numBuckets", C.c_uint)]
+
+
+class FGError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libfluidgrid_amd.so; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FGError(f"{LIB_PATH} is missing: build it with `python -m fluidgrid_amd.build` "
+                          "(there is no CPU fallback for the HIP path)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc=None, what="fluidgrid_amd"):
+    """Raise FGError if rc is non-zero or the thread's error slot is set."""
+    L = load()
+    err = L.fg_last_error()
+    if (rc not in (None, 0)) or err:
+        msg = L.fg_last_error_string().decode(errors="replace")
+        raise FGError(f"{what}: rc={rc} err={err} {msg}")
+
+
+def gpu_count():
+    n = C.c_int(0)
+    load().fg_device_count(C.byref(n))
+    return n.value
+
+
+def device_name(dev=0):
+    buf = C.create_string_buffer(256)
+    rc = load().fg_device_name(dev, buf, 256)
+    check(rc, "fg_device_name")
+    return buf.value.decode()
+
+
+class DeviceArray:
+    """A device allocation mirrored by a numpy dtype/shape."""
+
+    def __init__(self, shape, dtype, data=None):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(load().fg_malloc(C.byref(p), max(self.nbytes, 16)), "fg_malloc")
+        self.ptr = p.value
+        if data is not None:
+            self.upload(data)
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        return cls(a.shape, a.dtype, a)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        check(load().fg_memcpy_htod(self.ptr, a.ctypes.data, self.nbytes), "htod")
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        check(load().fg_synchronize(), "sync")
+        check(load().fg_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes), "dtoh")
+        return out
+
+    def fill_u32(self, value):
+        check(load().fg_memset_u32(self.ptr, value, self.nbytes // 4), "fill")
+
+    def free(self):
+        if self.ptr:
+            load().fg_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def init_params(common, kernels, key_mode, math_mode=abi.MATH_FAST):
+    L = load()
+    L.gInitializeCommonFluidParams(common)
+    L.gInitializeKernelParams(kernels)
+    check(L.fg_set_key_mode(key_mode), "key mode")
+    check(L.fg_set_math_mode(math_mode), "math mode")
+
+
+def download_raw(ptr, shape, dtype):
+    out = np.empty(shape, dtype)
+    check(load().fg_memcpy_dtoh(out.ctypes.data, ptr, out.nbytes), "dtoh")
+    return out
+
+
+class Simulator:
+    """The C++ HeterogeneousSimulator (one process-wide instance) driven
+    through include/fluidgrid_amd_sim.h."""
+
+    def __init__(self, scene, math_mode=abi.MATH_FAST, fused=True, use_graph=True, stage_timing=False):
+        L = load()
+        self.scene = scene
+        self.common = scene.common
+        check(L.fg_sim_init(C.byref(scene.common), scene.key_mode, math_mode), "fg_sim_init")
+        pos = np.ascontiguousarray(scene.pos, np.float32)
+        vel = np.ascontiguousarray(scene.vel, np.float32)
+        idx = L.fg_sim_add_fluid(C.byref(scene.fluid), pos.ctypes.data, vel.ctypes.data)
+        if idx < 0:
+            check(-idx, "fg_sim_add_fluid")
+        self.idx = idx
+        self.n = scene.n
+        check(L.fg_sim_set_options(int(fused), int(use_graph), int(stage_timing)), "options")
+        self.buffers = FluidBuffers()
+        check(L.fg_sim_fluid_buffers(idx, C.byref(self.buffers)), "buffers")
+
+    def set_options(self, fused=True, use_graph=True, stage_timing=False):
+        check(load().fg_sim_set_options(int(fused), int(use_graph), int(stage_timing)), "options")
+
+    def step(self, steps=1):
+        check(load().fg_sim_step(int(steps)), "fg_sim_step")
+
+    def synchronize(self):
+        check(load().fg_sim_synchronize(), "fg_sim_synchronize")
+
+    def state(self):
+        pos = np.empty((self.n, 4), np.float32)
+        vel = np.empty((self.n, 4), np.float32)
+        check(load().fg_sim_download(self.idx, pos.ctypes.data, vel.ctypes.data), "download")
+        return pos, vel
+
+    def upload(self, pos, vel):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        check(load().fg_sim_upload(self.idx, pos.ctypes.data, vel.ctypes.data), "upload")
+
+    def arrays(self):
+        """Intermediate arrays of the last step (keys, ids, cells, sorted
+        state, density) as numpy."""
+        self.synchronize()
+        b, n, nb = self.buffers, self.n, self.buffers.numBuckets
+        return dict(
+            skey=download_raw(b.hashKey, (n,), np.uint32),
+            spid=download_raw(b.particleId, (n,), np.uint32),
+            start=download_raw(b.gridStart, (nb,), np.uint32),
+            end=download_raw(b.gridEnd, (nb,), np.uint32),
+            spos=download_raw(b.orderedPosition, (n, 4), np.float32),
+            svel=download_raw(b.orderedVelocity, (n, 4), np.float32),
+            rho_p=download_raw(b.densityPressure, (n, 2), np.float32),
+        )
+
+    def stage_times(self):
+        out = (C.c_float * 6)()
+        check(load().fg_sim_stage_times(out), "stage times")
+        return dict(zip(("hash", "sort", "cells", "density", "forces", "integrate"), list(out)))
+
+    def stream(self):
+        return load().fg_sim_stream()
+
+    def close(self):
+        load().fg_sim_shutdown()

@@ -1,0 +1,218 @@
+"""Seeded synthetic scenes for the SPH step.
+
+Dam-break blocks follow the reference emitter's particle order: y-layer
+outermost, then z, then x innermost (Emitter.cpp:16-42).  Each lattice point
+is jittered by (2u-1)*0.01h per axis with u drawn from a SplitMix64 stream
+(seed = 20261202 + config id), three draws per particle in id order.
+
+`appmain_scene()` restates the reference application's hard-coded scene
+(AppMain.cpp:78-133 with BlockEmitter, Emitter.cpp:16-42) in float32
+arithmetic; it is the parity scene for the REF_XOR key mode.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+SEED_BASE = 20261202
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed, count, offset=0):
+    """The first `count` outputs (after skipping `offset`) of SplitMix64."""
+    with np.errstate(over="ignore"):
+        k = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed, count, offset=0):
+    """Uniform floats in [0,1) built from the top 24 bits of SplitMix64."""
+    return (splitmix64(seed, count, offset) >> np.uint64(40)).astype(np.float64) * (1.0 / (1 << 24))
+
+
+def kernels_cache(h):
+    """KernelsCache as HeterogeneousSimulator.cpp:33-39 computes it: the
+    float PI constant, pow(h, n) in double, one rounding to float."""
+    pi_f = float(np.float32(3.14159265))
+    h = float(np.float32(h))
+    kc = abi.KernelsCache()
+    kc.kPoly6 = float(np.float32(315.0 / (float(np.float32(64.0 * pi_f)) * h ** 9)))
+    kc.kSpikyGradient = float(np.float32(-(45.0 / (pi_f * h ** 6))))
+    kc.kViscosityLaplacian = float(np.float32(45.0 / (pi_f * h ** 6)))
+    return kc
+
+
+def key_bits(num_buckets):
+    return max(1, int(num_buckets - 1).bit_length())
+
+
+@dataclass
+class Scene:
+    name: str
+    pos: np.ndarray            # (N,4) float32, w = active flag
+    vel: np.ndarray            # (N,4) float32
+    fluid: abi.FluidParams
+    common: abi.CommonFluidParams
+    kernels: abi.KernelsCache
+    key_mode: int
+    steps: int
+    seed: int = 0
+    info: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return self.pos.shape[0]
+
+    @property
+    def num_buckets(self):
+        return self.common.num_buckets()
+
+
+# Dam-break configurations: (lattice nx, ny, nz), h, box extents, dt, steps.
+CONFIGS = {
+    0: dict(lattice=(32, 16, 32), h=0.025, box=(1.0, 1.0, 1.0), dt=4e-4, steps=100),
+    1: dict(lattice=(128, 64, 128), h=0.01, box=(1.5, 1.5, 1.5), dt=1.5e-4, steps=200),
+    2: dict(lattice=(256, 128, 256), h=0.005, box=(1.5, 1.5, 1.5), dt=7.5e-5, steps=200),
+    4: dict(lattice=(512, 256, 256), h=0.003, box=(2.0, 1.0, 1.0), dt=8e-5, steps=100),
+}
+
+
+def _cells(extent, h):
+    return int(math.ceil(extent / h - 1e-9))
+
+
+def dam_break(lattice, h, box, dt, steps=100, seed=SEED_BASE, name="dam_break",
+              key_mode=abi.KEY_LINEAR, jitter=0.01, stiffness_k=3.0, mu=3.5, rho0=1000.0,
+              velocity_limit=600.0, gravity=-9.81):
+    """Jittered block of nx*ny*nz particles at spacing h/2 from the box corner."""
+    nx, ny, nz = lattice
+    n = nx * ny * nz
+    half = 0.5 * h
+    iy, iz, ix = np.meshgrid(np.arange(ny), np.arange(nz), np.arange(nx), indexing="ij")
+    lat = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1).astype(np.float64)
+    base = half * (1.0 + lat)
+    if jitter:
+        u = uniform01(seed, 3 * n).reshape(n, 3)
+        base = base + (2.0 * u - 1.0) * (jitter * h)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = base.astype(np.float32)
+    pos[:, 3] = 1.0
+    vel = np.zeros((n, 4), np.float32)
+
+    cp = abi.CommonFluidParams()
+    hf = float(np.float32(h))
+    cp.cell_size = abi.f3(hf, hf, hf)
+    cells = tuple(_cells(e, h) for e in box)
+    cp.bucket_size = abi.i3(*cells)
+    cp.keyBits = key_bits(cells[0] * cells[1] * cells[2])
+    cp.kernel_radius = hf
+    cp.kernel_radius_squared = float(np.float32(hf) * np.float32(hf))
+    cp.pointRadius = float(np.float32(half))
+    cp.pointScale = 1.0
+    cp.velocity_limit = velocity_limit
+    cp.volume_min = abi.f3(0.0, 0.0, 0.0)
+    cp.volume_max = abi.f3(*box)
+    cp.global_external_force = abi.f3(0.0, gravity, 0.0)
+    cp.simulation_scale = 1.0
+    cp.deltaTimeIntegration = dt
+    cp.fluid_to_srb_relation = abi.i3(1, 1, 1)
+
+    fp = abi.FluidParams()
+    fp.mass = float(np.float32(rho0 * half ** 3))
+    fp.restDensity = rho0
+    fp.viscosity = mu
+    fp.stiffness = 0.0
+    fp.particle_radius = float(np.float32(half))
+    fp.gasConstant = stiffness_k
+    fp.max_particles = n
+    return Scene(name, pos, vel, fp, cp, kernels_cache(hf), key_mode, steps, seed,
+                 dict(lattice=lattice, h=h, box=box, dt=dt))
+
+
+def config_scene(cfg_id, key_mode=abi.KEY_LINEAR):
+    c = CONFIGS[cfg_id]
+    return dam_break(c["lattice"], c["h"], c["box"], c["dt"], c["steps"], seed=SEED_BASE + cfg_id,
+                     name=f"dam_break_cfg{cfg_id}", key_mode=key_mode)
+
+
+def small_dam_break(lattice=(12, 8, 10), h=0.05, box=(1.0, 1.0, 1.0), dt=4e-4, seed=SEED_BASE + 100,
+                    key_mode=abi.KEY_LINEAR):
+    return dam_break(lattice, h, box, dt, 10, seed=seed, name="small_dam_break", key_mode=key_mode)
+
+
+def appmain_scene():
+    """The reference application's scene (AppMain.cpp:84-123) in float32."""
+    f = np.float32
+    cp = abi.CommonFluidParams()
+    cp.volume_min = abi.f3(-20.0, -20.0, -20.0)
+    cp.volume_max = abi.f3(20.0, 85.0, 40.0)
+    cp.global_external_force = abi.f3(0.0, float(f(-9.8)), 0.0)
+    cp.simulation_scale = float(f(0.009))
+    cp.deltaTimeIntegration = float(f(0.004))
+    cp.bucket_size = abi.i3(64, 64, 64)
+    cp.keyBits = 24
+    cp.velocity_limit = 600.0
+    cp.pointScale = float(f(800.0 / math.tan(90.0 * 0.5 * float(f(3.1415)) / 180.0)))
+    cp.fluid_to_srb_relation = abi.i3(1, 1, 1)
+
+    fp = abi.FluidParams()
+    fp.max_particles = 24000
+    mass = f(f(128 * 1024.0) / f(24000)) * f(0.0002)
+    fp.mass = float(mass)
+    fp.restDensity = 1000.0
+    fp.stiffness = float(f(0.2))
+    fp.viscosity = 1.0
+    fp.gasConstant = 1.5
+    fp.particle_radius = float(f(0.3))
+
+    dist = np.power(f(mass / f(1000.0)), f(f(1.0) / f(3.0)))
+    spacing = f(f(dist * f(0.87)) / f(cp.simulation_scale))
+    kr = f(f(f(2.0) * dist) * f(0.87))
+    cp.kernel_radius = float(kr)
+    cp.kernel_radius_squared = float(f(kr * kr))
+    cp.pointRadius = float(spacing)
+    pv = f(kr / f(cp.simulation_scale))
+    cp.cell_size = abi.f3(float(pv), float(pv), float(pv))
+
+    pos = block_emitter((f(-20.0), f(-20.0), f(-20.0)), (f(20.0), f(85.0), f(0.0)), spacing, fp.max_particles)
+    vel = np.zeros_like(pos)
+    return Scene("appmain_ref_xor", pos, vel, fp, cp, kernels_cache(float(kr)), abi.KEY_REF_XOR, 100, 0,
+                 dict(spacing=float(spacing)))
+
+
+def block_emitter(bmin, bmax, spacing, count):
+    """BlockEmitter::reset (Emitter.cpp:16-42) with float-accumulated loops."""
+    f = np.float32
+    spacing = f(spacing)
+    xs, zs = [], []
+    x = f(bmin[0])
+    while x < f(bmax[0]):
+        xs.append(x)
+        x = f(x + spacing)
+    z = f(bmin[2])
+    while z < f(bmax[2]):
+        zs.append(z)
+        z = f(z + spacing)
+    out = np.zeros((count, 4), np.float32)
+    out[:, 3] = -1.0
+    used = 0
+    pos_y = f(bmin[1])
+    per_layer = len(xs) * len(zs)
+    if per_layer == 0:
+        return out
+    while used < count:
+        for zz in zs:
+            for xx in xs:
+                if used < count:
+                    out[used] = (xx, pos_y, zz, 1.0)
+                    used += 1
+        pos_y = f(pos_y + spacing)
+    return out

@@ -1,0 +1,73 @@
+// HeterogeneousSimulator.h -- the step orchestrator of the reference
+// (HeterogeneousSimulator.h:41-45, .cpp:27-212): a singleton that computes
+// the kernel constants, uploads the parameter blocks and advances every
+// registered fluid by one step per doTimeStep().
+//
+// Differences from the reference, all outside the arithmetic:
+//  * no host synchronisation or printf between stages; work goes to one
+//    stream (the simulator's own, or the one given to setStream);
+//  * the sort uses the key width implied by the bucket count and a
+//    persistent workspace; forces and integration run as one pass when
+//    `fused` is set (bit-identical results);
+//  * doTimeSteps(n) can replay a captured HIP graph of one step.
+#pragma once
+
+#include <vector>
+
+#include "ApplicationData.h"
+#include "GPUSPHFluid.h"
+
+namespace Simulation {
+namespace Simulators {
+
+struct StepOptions {
+    bool fused = true;       // forces + integrate in one pass
+    bool use_graph = true;   // doTimeSteps replays a captured step
+    bool stage_timing = false;
+};
+
+struct StageTimes {
+    float hash_ms = 0, sort_ms = 0, cells_ms = 0, density_ms = 0, forces_ms = 0, integrate_ms = 0;
+};
+
+class HeterogeneousSimulator {
+public:
+    static HeterogeneousSimulator* instance();
+    // Extension: drops the singleton (and its captured graph).
+    static void destroy();
+
+    void addSphFluid(Fluid::GPUSPHFluid* fluid);
+    void doTimeStep();
+
+    // Extensions.
+    void doTimeSteps(int steps);
+    void setOptions(const StepOptions& o);
+    const StepOptions& options() const { return mOptions; }
+    void setStream(hipStream_t s);
+    hipStream_t stream() const { return mStream; }
+    const StageTimes& lastStageTimes() const { return mTimes; }
+    void invalidateGraph();
+
+private:
+    HeterogeneousSimulator();
+    ~HeterogeneousSimulator();
+
+    void findFluidNeighbourhood(Fluid::GPUSPHFluid* fluid);
+    void calculateFluidDensity(Fluid::GPUSPHFluid* fluid);
+    void calculateFluidForces(Fluid::GPUSPHFluid* fluid, bool useStaticRB, bool useDynamicRB);
+    void integrateFluidForces(Fluid::GPUSPHFluid* fluid);
+    void enqueueStep();
+
+    std::vector<Fluid::GPUSPHFluid*> mFluids;
+    StepOptions mOptions;
+    hipStream_t mStream = nullptr;
+    bool mOwnStream = false;
+    hipGraph_t mGraph = nullptr;
+    hipGraphExec_t mGraphExec = nullptr;
+    hipEvent_t mEv[7] = {};
+    StageTimes mTimes;
+    static HeterogeneousSimulator* mInstance;
+};
+
+}  // namespace Simulators
+}  // namespace Simulation

@@ -1,0 +1,254 @@
+/*
+ * fluidgrid_amd.h -- C-ABI of the MI355X (gfx950) per-timestep SPH pipeline.
+ *
+ * Two groups of entry points live here:
+ *
+ *  1. The reference launchers, same names, same argument order and meaning,
+ *     same `void` return.  Each declaration cites the reference header line it
+ *     replaces (paths relative to the FluidGrid source tree).  A host program
+ *     written against GPU_Kernels/K_HashGrid.cuh, K_FluidSim.cuh and
+ *     K_RigidBodySim.cuh links against libfluidgrid_amd.so unchanged.
+ *
+ *  2. `fg_*` extensions: stream selection, error query, key/math modes, an
+ *     explicit-key-width sort, the fused force+integrate launcher, device
+ *     memory helpers for callers that do not link a HIP runtime themselves,
+ *     per-body FP64 reductions, and the opaque simulator handle used by the
+ *     C++ `HeterogeneousSimulator` mirror.
+ *
+ * Vector types.  fg_float2/3/4 and fg_int3 are plain structs with the size
+ * and alignment of the HIP/CUDA vector types (8/8, 12/4, 16/16, 12/4); they
+ * are passed by value exactly like HIP's float3/float4 on x86-64 SysV.
+ *
+ * Conventions shared by every launcher:
+ *  - all pointers are device pointers owned by the caller; launchers never
+ *    allocate, except gSort/fg_sort_pairs which keep a persistent scratch
+ *    buffer that grows on demand (fg_sort_pairs_ws takes caller workspace);
+ *  - work is enqueued on the stream chosen by fg_set_stream (default: the
+ *    null stream) and launchers do not synchronise;
+ *  - launchers validate arguments and record failures in a thread-local
+ *    error slot read by fg_last_error(); they never abort the process.
+ */
+#ifndef FLUIDGRID_AMD_H
+#define FLUIDGRID_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fg_float2 { float x, y; } __attribute__((aligned(8))) fg_float2;
+typedef struct fg_float3 { float x, y, z; } fg_float3;
+typedef struct fg_float4 { float x, y, z, w; } __attribute__((aligned(16))) fg_float4;
+typedef struct fg_int3 { int x, y, z; } fg_int3;
+typedef struct fg_double4 { double x, y, z, w; } fg_double4;
+
+/* ApplicationData.cuh:32-41 (28 bytes) */
+typedef struct fg_fluid_params {
+    float mass;
+    float restDensity;
+    float viscosity;
+    float stiffness;
+    float particle_radius;
+    float gasConstant;
+    int   max_particles;
+} fg_fluid_params;
+
+/* ApplicationData.cuh:46-67 (104 bytes, align 4) */
+typedef struct fg_common_fluid_params {
+    fg_float3    cell_size;
+    fg_int3      bucket_size;
+    unsigned int keyBits;
+    float        kernel_radius;
+    float        kernel_radius_squared;
+    float        pointRadius;
+    float        pointScale;
+    float        velocity_limit;
+    fg_float3    volume_min;
+    fg_float3    volume_max;
+    fg_float3    global_external_force;
+    float        simulation_scale;
+    float        deltaTimeIntegration;
+    fg_int3      fluid_to_srb_relation;
+} fg_common_fluid_params;
+
+/* ApplicationData.cuh:73-102 (112 bytes, align 16) */
+typedef struct fg_common_rigid_body_params {
+    fg_int3      rb_to_fluid_relation;
+    fg_int3      st_bucket_size;
+    fg_int3      dn_bucket_size;
+    unsigned int st_keyBits;
+    float        velocity_limit;
+    fg_float3    volume_min;
+    fg_float3    volume_max;
+    fg_float4    global_external_force;
+    float        simulation_scale;
+    float        deltaTimeIntegration;
+    float        particleRadius;
+} fg_common_rigid_body_params;
+
+/* ApplicationData.cuh:107-112 (12 bytes) */
+typedef struct fg_kernels_cache {
+    float kPoly6;
+    float kSpikyGradient;
+    float kViscosityLaplacian;
+} fg_kernels_cache;
+
+/* ------------------------------------------------------------------------ */
+/* 1. Reference launchers                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* K_FluidSim.cuh:43-44 -- makes `p` the parameter block of every later
+ * launch (held host-side and passed as a kernel argument). */
+void gInitializeCommonFluidParams(fg_common_fluid_params p);
+
+/* K_FluidSim.cuh:18-19 -- kPoly6, kSpikyGradient, kViscosityLaplacian. */
+void gInitializeKernelParams(fg_kernels_cache k);
+
+/* K_HashGrid.cuh:5-8 -- key[i] = cell key of positions[i]; particleID[i] = i.
+ * The key function is the one selected by fg_set_key_mode. */
+void gCalcHashKey(unsigned int* hashKey, unsigned int* particleID, fg_float4* positions,
+                  unsigned int numParticles, unsigned int numBuckets, fg_float3 cell_size);
+
+/* K_HashGrid.cuh:10-20 -- body-sphere world transform conj(q)*p*q + cm[w],
+ * then key and identity id. */
+void gCalcHashKeyWithOffset(unsigned int* _out_hashKey, unsigned int* _out_particleID,
+                            fg_float4* _in_vectorOffset, fg_float4* _in_orientation,
+                            fg_float4* _in_vectorPos, fg_float4* _out_positions,
+                            unsigned int _in_numParticles, fg_float3 _in_cellSize,
+                            int _in_hashTableSize);
+
+/* K_FluidSim.cuh:81-82 -- stable in-place key/value sort (ties keep input
+ * order).  The key width is found on the device; one 4-byte readback. */
+void gSort(unsigned int* _key, unsigned int* _values, int _numParticles);
+
+/* K_HashGrid.cuh:23-34 -- cellStart := 0xFFFFFFFF over numBuckets; cell
+ * start/end from key transitions of the sorted keys; SoA gather of pos/vel
+ * through the sorted ids.  cellEnd is not cleared. */
+void gFindCellOffsetOrderingPosVel(unsigned int* gridPosStart, unsigned int* gridPosEnd,
+                                   unsigned int* hashTable, unsigned int* particleID,
+                                   fg_float4* particlePositions, fg_float4* orderedParticlePositions,
+                                   fg_float4* particleVel, fg_float4* particleOrderedVel,
+                                   unsigned int numParticles, unsigned int numBuckets);
+
+/* K_HashGrid.cuh:37-56 -- as above, positions only. */
+void gFindCellOffsetOrderingPos(unsigned int* gridPosStart, unsigned int* gridPosEnd,
+                                unsigned int* hashTable, unsigned int* particleID,
+                                fg_float4* particlePositions, fg_float4* orderedParticlePositions,
+                                unsigned int numParticles, unsigned int numBuckets);
+
+/* K_FluidSim.cuh:7-16 -- (max(1,rho), k*(rho-rho0)) for active particles,
+ * poly6 over the 27-cell stencil; hashTable is accepted and unused. */
+void gCalcMassDensity(fg_float4* orderedPositions, fg_float2* mass_pressure,
+                      unsigned int* hashTable, unsigned int* gridStart, unsigned int* gridEnd,
+                      int numParticles, float gasConstant, float restDensity, float mass);
+
+/* K_FluidSim.cuh:21-32 -- spiky pressure + Laplacian viscosity + gravity +
+ * penalty walls + clamp; vel[particleId[i]] = orderedVelocity[i] + a*dt.
+ * `forces` is accepted and unused. */
+void gSPHCalculateForces(fg_float4* orderedPositions, fg_float4* orderedVelocity,
+                         fg_float4* forces, fg_float2* density_pressure,
+                         unsigned int* gridStart, unsigned int* gridEnd, fg_float4* vel,
+                         unsigned int* particleId, unsigned int numParticles,
+                         float mass, float viscosity);
+
+/* K_FluidSim.cuh:34-41 -- pos += vel*(dt/scale) where pos.w > 0.
+ * forces/density/grid arguments are accepted and unused. */
+void gSPHIntegrate(fg_float4* _in_sph_positions, fg_float4* _in_sph_velocity,
+                   fg_float4* _in_sph_forces, unsigned int _in_num_particles,
+                   fg_float2* _in_density_press, unsigned int* _in_grid_start,
+                   unsigned int* _in_grid_end);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Extensions (no counterpart in the reference)                           */
+/* ------------------------------------------------------------------------ */
+
+enum {
+    FG_OK = 0,
+    FG_ERR_INVALID_ARGUMENT = 1,
+    FG_ERR_HIP = 2,
+    FG_ERR_OUT_OF_MEMORY = 3,
+    FG_ERR_NOT_INITIALIZED = 4,
+    FG_ERR_UNSUPPORTED = 5
+};
+
+/* Cell key function used by gCalcHashKey and by the neighbour stencil of
+ * the density/force kernels.
+ *  FG_KEY_REF_XOR: ((xi*73856093)^(yi*19349663)^(zi*83492791)) % numBuckets
+ *    with xi = (int)((p.x+5000)/cs.x)  -- K_DeviceFunctions.cu:17-26;
+ *    the stencil rehashes p + k*cs (K_FluidSim.cu:185-203).
+ *  FG_KEY_LINEAR:  key = cx + nx*(cy + ny*cz), ci = clamp(floor((p-vmin)/cs),
+ *    0, n-1) with n = bucket_size; the stencil is ci-1..ci+1, clipped. */
+enum { FG_KEY_REF_XOR = 0, FG_KEY_LINEAR = 1 };
+
+/* Arithmetic of the density/force kernels.
+ *  FG_MATH_FAST: neighbour rows walked as contiguous sorted ranges, cell
+ *    culling, common factors hoisted out of the pair loop (default).
+ *  FG_MATH_REFERENCE: per-bucket partial sums and the reference's per-pair
+ *    expressions, no contraction. */
+enum { FG_MATH_FAST = 0, FG_MATH_REFERENCE = 1 };
+
+int         fg_last_error(void);          /* last error code of this thread, then FG_OK */
+const char* fg_last_error_string(void);   /* message of the last error of this thread */
+int         fg_peek_error(void);          /* last error code without clearing */
+void        fg_record_error(int code, const char* message); /* sets this thread's error slot */
+const char* fg_version(void);
+
+int   fg_set_stream(void* hip_stream);    /* hipStream_t; NULL = null stream */
+void* fg_get_stream(void);
+int   fg_set_key_mode(int mode);
+int   fg_get_key_mode(void);
+int   fg_set_math_mode(int mode);
+int   fg_get_math_mode(void);
+int   fg_get_common_fluid_params(fg_common_fluid_params* out);
+int   fg_get_kernels_cache(fg_kernels_cache* out);
+
+/* Stable in-place key/value sort of keys < 2^keyBits (1..32).  Passes are
+ * chosen from keyBits; no host synchronisation. */
+int fg_sort_pairs(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits);
+/* Same, values are taken to be 0..n-1 on input (not read). */
+int fg_sort_pairs_iota(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits);
+/* Workspace bytes needed by fg_sort_pairs_ws for n elements. */
+size_t fg_sort_workspace_size(unsigned int n, int keyBits);
+int fg_sort_pairs_ws(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits,
+                     void* workspace, size_t workspace_bytes, int values_are_iota);
+
+/* gCalcHashKey without the id output (the sort that follows generates the
+ * identity ids itself, see fg_sort_pairs_iota). */
+int fg_calc_hash_key_only(unsigned int* hashKey, fg_float4* positions, unsigned int numParticles,
+                          unsigned int numBuckets, fg_float3 cell_size);
+
+/* gSPHCalculateForces followed by gSPHIntegrate, in one pass over the sorted
+ * arrays: vel[id[i]] = v_i + a_i*dt and, where pos[id[i]].w > 0,
+ * pos[id[i]] = orderedPositions[i] + vel[id[i]]*(dt/scale).  Results are
+ * bit-identical to the two launches in sequence. */
+int fg_sph_forces_integrate(fg_float4* orderedPositions, fg_float4* orderedVelocity,
+                            fg_float2* density_pressure, unsigned int* gridStart,
+                            unsigned int* gridEnd, fg_float4* vel, fg_float4* pos,
+                            unsigned int* particleId, unsigned int numParticles,
+                            float mass, float viscosity);
+
+/* Device memory and timing helpers (thin wrappers over the HIP runtime). */
+int fg_device_count(int* count);
+int fg_device_name(int device, char* buf, size_t len);
+int fg_malloc(void** ptr, size_t bytes);
+int fg_free(void* ptr);
+int fg_memcpy_htod(void* dst, const void* src, size_t bytes);
+int fg_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int fg_memcpy_dtod(void* dst, const void* src, size_t bytes);
+int fg_memset_u32(unsigned int* dst, unsigned int value, size_t count); /* fill kernel */
+int fg_synchronize(void);                  /* the selected stream */
+int fg_event_create(void** ev);
+int fg_event_destroy(void* ev);
+int fg_event_record(void* ev);             /* on the selected stream */
+int fg_event_elapsed_ms(void* start, void* stop, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDGRID_AMD_H */

@@ -1,0 +1,55 @@
+/*
+ * fluidgrid_amd_sim.h -- C-ABI over the C++ HeterogeneousSimulator mirror
+ * (include/fluidgrid/HeterogeneousSimulator.h).  No counterpart in the
+ * reference: it lets callers without a C++ toolchain (FFI, Python ctypes)
+ * drive the whole step the way the reference's application does
+ * (AppMain.cpp:78-133 then HeterogeneousSimulator::doTimeStep per frame).
+ * One process-wide simulator; not reentrant.
+ */
+#ifndef FLUIDGRID_AMD_SIM_H
+#define FLUIDGRID_AMD_SIM_H
+
+#include "fluidgrid_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fg_fluid_buffers {
+    void* hashKey;          /* u32[n]      sorted cell keys            */
+    void* particleId;       /* u32[n]      sorted particle ids         */
+    void* gridStart;        /* u32[nb]     first sorted index per cell */
+    void* gridEnd;          /* u32[nb]     one past the last           */
+    void* position;         /* float4[n]   particle order, w = active  */
+    void* orderedPosition;  /* float4[n]   sorted order                */
+    void* velocity;         /* float4[n]   particle order              */
+    void* orderedVelocity;  /* float4[n]   sorted order                */
+    void* densityPressure;  /* float2[n]   (rho, p), sorted order      */
+    void* forces;           /* float4[n]   unused by the step          */
+    unsigned int numParticles;
+    unsigned int numBuckets;
+} fg_fluid_buffers;
+
+/* Resets the simulator and makes `common` the parameter block
+ * (ApplicationData::setCommonFluidParams, then the simulator constructor). */
+int fg_sim_init(const fg_common_fluid_params* common, int key_mode, int math_mode);
+/* Adds a fluid with host-side initial state; returns its index, or a
+ * negated FG_ERR_* code. */
+int fg_sim_add_fluid(const fg_fluid_params* params, const fg_float4* positions, const fg_float4* velocities);
+int fg_sim_set_options(int fused, int use_graph, int stage_timing);
+int fg_sim_step(int steps);                       /* asynchronous */
+int fg_sim_synchronize(void);
+void* fg_sim_stream(void);                        /* hipStream_t of the simulator */
+int fg_sim_download(int fluid, fg_float4* positions, fg_float4* velocities);
+int fg_sim_upload(int fluid, const fg_float4* positions, const fg_float4* velocities);
+int fg_sim_fluid_buffers(int fluid, fg_fluid_buffers* out);
+/* hash, sort, cells, density, forces, integrate (ms) of the last step run
+ * with stage_timing on. */
+int fg_sim_stage_times(float* out6);
+int fg_sim_shutdown(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
