@@ -64,8 +64,9 @@ def test_sort_pairs_stable(fglib, n, bits):
     rng = np.random.default_rng(n * 31 + bits)
     hi = 1 << bits
     # Few distinct keys in part of the array so ties are common.
-    keys = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
-    keys[: n // 3] = (keys[: n // 3] % 17) * (hi // 17 if hi >= 17 else 1) % hi
+    k64 = rng.integers(0, hi, n, dtype=np.uint64)
+    k64[: n // 3] = (k64[: n // 3] % 17) * np.uint64(hi // 17 if hi >= 17 else 1) % np.uint64(hi)
+    keys = k64.astype(np.uint32)
     vals = rng.permutation(n).astype(np.uint32)
     order = np.argsort(keys, kind="stable")
     dk, dv = fglib.DeviceArray.from_numpy(keys), fglib.DeviceArray.from_numpy(vals)
