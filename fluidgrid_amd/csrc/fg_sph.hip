@@ -272,11 +272,27 @@ __device__ __forceinline__ bool row_range(const RowWalker& w, int dy, int dz, fl
     return rb < re;
 }
 
+// Blocks are handed to the eight XCDs round-robin; remapping them so that each
+// XCD walks one contiguous slice of the sorted particles keeps the neighbour
+// rows of consecutive blocks in the same L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = nblocks / kXcd, rem = nblocks % kXcd;
+    const uint32_t x = b % kXcd, idx = b / kXcd;
+    return x * per + min(x, rem) + idx;
+}
+
+// Neighbour bitmasks: for each of the 9 stencil rows of a particle, bit k of
+// masks[row * n + i] is set when sorted particle rowBegin + k lies within h.
+// Rows longer than 64 particles are not encoded and are re-tested.
+constexpr uint32_t kRowMaskBits = 64;
+
+template <bool MASKS>
 __global__ void __launch_bounds__(kSphThreads)
     k_density_fast(const float4* __restrict__ sp, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
                    const uint32_t* __restrict__ end, uint32_t n, float k, float rho0, float m,
-                   fg_common_fluid_params cp, fg_kernels_cache kc) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                   fg_common_fluid_params cp, fg_kernels_cache kc, unsigned long long* __restrict__ masks) {
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = sp[i];
     if (!(p.w > 0.0f)) return;
@@ -285,28 +301,70 @@ __global__ void __launch_bounds__(kSphThreads)
     const float cut = hh * 1.0001f;
     const RowWalker w = make_rows(p, cp);
     float acc = 0.0f;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++) {
-            uint32_t rb, re;
-            if (!row_range(w, dy, dz, cut, cp, start, end, n, rb, re)) continue;
-            for (uint32_t j = rb; j < re; j++) {
-                const float4 q = sp[j];
-                const float dx = p.x - q.x, dyv = p.y - q.y, dzv = p.z - q.z;
-                const float r2 = (dx * dx + dyv * dyv + dzv * dzv) * s2;
-                const float t = h2 - r2;
-                acc += (r2 < hh) ? t * t * t : 0.0f;
+#pragma unroll
+    for (int row = 0; row < 9; row++) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        unsigned long long msk = 0ull;
+        uint32_t rb, re;
+        if (row_range(w, dy, dz, cut, cp, start, end, n, rb, re)) {
+            const uint32_t len = re - rb, last = re - 1;
+            for (uint32_t kk = 0; kk < len; kk += 4) {
+                // Four independent loads in flight per lane.
+                const float4 q0 = sp[rb + kk];
+                const float4 q1 = sp[min(rb + kk + 1, last)];
+                const float4 q2 = sp[min(rb + kk + 2, last)];
+                const float4 q3 = sp[min(rb + kk + 3, last)];
+                const float4 qs[4] = {q0, q1, q2, q3};
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float dx = p.x - qs[u].x, dyv = p.y - qs[u].y, dzv = p.z - qs[u].z;
+                    const float r2 = (dx * dx + dyv * dyv + dzv * dzv) * s2;
+                    const bool hit = (r2 < hh) && (kk + u < len);
+                    const float t = h2 - r2;
+                    acc += hit ? t * t * t : 0.0f;
+                    if (MASKS && hit && kk + u < kRowMaskBits) msk |= 1ull << (kk + u);
+                }
             }
         }
+        if (MASKS) masks[(size_t)row * n + i] = msk;
+    }
     const float rho = m * kc.kPoly6 * acc;
     rho_p[i] = make_float2(fmaxf(1.0f, rho), k * (rho - rho0));
 }
 
+// Pressure and viscosity terms of one neighbour j (r < h checked again, so
+// the same code serves rows without a mask).
+struct PairAcc {
+    float px, py, pz, vx, vy, vz;
+};
+
+__device__ __forceinline__ void pair_term(PairAcc& A, uint32_t i, uint32_t j, const float4& pos, const float4& vel,
+                                          float pri, float4 q, float4 vj, float2 dpj, float h, float hh, float s2) {
+    const float dx = pos.x - q.x, dy = pos.y - q.y, dz = pos.z - q.z;
+    const float r2 = (dx * dx + dy * dy + dz * dz) * s2;
+    if (r2 < hh && j != i) {
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * ir;
+        const float hr = h - r;
+        const float irj = __builtin_amdgcn_rcpf(dpj.x);
+        const float cp_ = hr * hr * (pri + dpj.y) * irj * ir;
+        A.px += dx * cp_;
+        A.py += dy * cp_;
+        A.pz += dz * cp_;
+        const float cv = hr * irj;
+        A.vx += (vj.x - vel.x) * cv;
+        A.vy += (vj.y - vel.y) * cv;
+        A.vz += (vj.z - vel.z) * cv;
+    }
+}
+
+template <bool MASKS>
 __global__ void __launch_bounds__(kSphThreads)
     k_forces_fast(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, float4* __restrict__ vout,
                   float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
-                  fg_common_fluid_params cp, fg_kernels_cache kc) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                  fg_common_fluid_params cp, fg_kernels_cache kc, const unsigned long long* __restrict__ masks) {
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pos = sp[i];
     const float4 vel = sv[i];
@@ -317,37 +375,34 @@ __global__ void __launch_bounds__(kSphThreads)
         const float cut = hh * 1.0001f;
         const float2 dpi = rho_p[i];
         const RowWalker w = make_rows(pos, cp);
-        float px = 0.0f, py = 0.0f, pz = 0.0f;  // sum d*(h-r)^2*(p_i+p_j)/(rho_j r)
-        float vx = 0.0f, vy = 0.0f, vz = 0.0f;  // sum (v_j-v_i)*(h-r)/rho_j
-        for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++) {
-                uint32_t rb, re;
-                if (!row_range(w, dy, dz, cut, cp, start, end, n, rb, re)) continue;
-                for (uint32_t j = rb; j < re; j++) {
-                    const float4 q = sp[j];
-                    const float dx = pos.x - q.x, dyv = pos.y - q.y, dzv = pos.z - q.z;
-                    const float r2 = (dx * dx + dyv * dyv + dzv * dzv) * s2;
-                    if (r2 < hh && j != i) {
-                        const float2 dpj = rho_p[j];
-                        const float4 vj = sv[j];
-                        const float r = sqrtf(r2);
-                        const float hr = h - r;
-                        const float irj = 1.0f / dpj.x;
-                        const float cp_ = hr * hr * (dpi.y + dpj.y) * irj / r;
-                        px += dx * cp_;
-                        py += dyv * cp_;
-                        pz += dzv * cp_;
-                        const float cv = hr * irj;
-                        vx += (vj.x - vel.x) * cv;
-                        vy += (vj.y - vel.y) * cv;
-                        vz += (vj.z - vel.z) * cv;
-                    }
+        PairAcc A{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (int row = 0; row < 9; row++) {
+            const int dz = row / 3 - 1, dy = row % 3 - 1;
+            uint32_t rb, re;
+            if (!row_range(w, dy, dz, cut, cp, start, end, n, rb, re)) continue;
+            if (MASKS && re - rb <= kRowMaskBits) {
+                unsigned long long mm = masks[(size_t)row * n + i];
+                while (mm) {
+                    const uint32_t k1 = (uint32_t)__builtin_ctzll(mm);
+                    mm &= mm - 1ull;
+                    const bool two = mm != 0ull;
+                    const uint32_t k2 = two ? (uint32_t)__builtin_ctzll(mm) : k1;
+                    if (two) mm &= mm - 1ull;
+                    const uint32_t j1 = rb + k1, j2 = rb + k2;
+                    const float4 q1 = sp[j1], q2 = sp[j2];
+                    const float4 v1 = sv[j1], v2 = sv[j2];
+                    const float2 d1 = rho_p[j1], d2 = rho_p[j2];
+                    pair_term(A, i, j1, pos, vel, dpi.y, q1, v1, d1, h, hh, s2);
+                    if (two) pair_term(A, i, j2, pos, vel, dpi.y, q2, v2, d2, h, hh, s2);
                 }
+            } else {
+                for (uint32_t j = rb; j < re; j++) pair_term(A, i, j, pos, vel, dpi.y, sp[j], sv[j], rho_p[j], h, hh, s2);
             }
+        }
         const float iri = 1.0f / dpi.x;
         const float cP = -s * kc.kSpikyGradient * iri;
         const float cV = kc.kViscosityLaplacian * mu * iri;
-        const F3 internal = f3(cP * px + cV * vx, cP * py + cV * vy, cP * pz + cV * vz);
+        const F3 internal = f3(cP * A.px + cV * A.vx, cP * A.py + cV * A.vy, cP * A.pz + cV * A.vz);
         a = finish_accel(internal, m, pos, vel, cp);
     }
     write_back(id[i], n, pos, vel, a, cp, vout, pout);
@@ -368,12 +423,15 @@ static bool have_params(const char* who) {
 static bool use_fast() { return state().key_mode == FG_KEY_LINEAR && state().math_mode == FG_MATH_FAST; }
 
 int launch_density(float4* sp, float2* rho_p, uint32_t* start, uint32_t* end, uint32_t n, float k, float rho0,
-                   float m) {
+                   float m, unsigned long long* masks = nullptr) {
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
-    if (use_fast())
-        hipLaunchKernelGGL(k_density_fast, grid, block, 0, st.stream, sp, rho_p, start, end, n, k, rho0, m, st.common,
-                           st.kernels);
+    if (use_fast() && masks)
+        hipLaunchKernelGGL(k_density_fast<true>, grid, block, 0, st.stream, sp, rho_p, start, end, n, k, rho0, m,
+                           st.common, st.kernels, masks);
+    else if (use_fast())
+        hipLaunchKernelGGL(k_density_fast<false>, grid, block, 0, st.stream, sp, rho_p, start, end, n, k, rho0, m,
+                           st.common, st.kernels, (unsigned long long*)nullptr);
     else if (st.key_mode == FG_KEY_LINEAR)
         hipLaunchKernelGGL(k_density_ref<FG_KEY_LINEAR>, grid, block, 0, st.stream, sp, rho_p, start, end, n, k,
                            rho0, m, st.common, st.kernels);
@@ -384,12 +442,15 @@ int launch_density(float4* sp, float2* rho_p, uint32_t* start, uint32_t* end, ui
 }
 
 int launch_forces(float4* sp, float4* sv, float2* rho_p, uint32_t* start, uint32_t* end, float4* vout,
-                  float4* pout, uint32_t* id, uint32_t n, float m, float mu) {
+                  float4* pout, uint32_t* id, uint32_t n, float m, float mu, const unsigned long long* masks = nullptr) {
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
-    if (use_fast())
-        hipLaunchKernelGGL(k_forces_fast, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout, pout, id, n, m,
-                           mu, st.common, st.kernels);
+    if (use_fast() && masks)
+        hipLaunchKernelGGL(k_forces_fast<true>, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout, pout, id,
+                           n, m, mu, st.common, st.kernels, masks);
+    else if (use_fast())
+        hipLaunchKernelGGL(k_forces_fast<false>, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout, pout, id,
+                           n, m, mu, st.common, st.kernels, (const unsigned long long*)nullptr);
     else if (st.key_mode == FG_KEY_LINEAR)
         hipLaunchKernelGGL(k_forces_ref<FG_KEY_LINEAR>, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout,
                            pout, id, n, m, mu, st.common, st.kernels);
@@ -477,6 +538,39 @@ int fg_sph_forces_integrate(fg_float4* orderedPositions, fg_float4* orderedVeloc
     }
     return launch_forces(FG_F4(orderedPositions), FG_F4(orderedVelocity), FG_F2(density_pressure), gridStart, gridEnd,
                          FG_F4(vel), FG_F4(pos), particleId, numParticles, mass, viscosity);
+}
+
+size_t fg_neighbor_mask_bytes(unsigned int numParticles) {
+    return (size_t)9 * sizeof(unsigned long long) * (size_t)numParticles;
+}
+
+int fg_sph_density_nbr(fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,
+                       unsigned int* gridEnd, unsigned long long* neighborMasks, unsigned int numParticles,
+                       float gasConstant, float restDensity, float mass) {
+    if (numParticles == 0) return FG_OK;
+    if (!have_params("fg_sph_density_nbr")) return FG_ERR_NOT_INITIALIZED;
+    if (!orderedPositions || !mass_pressure || !gridStart || !gridEnd || !neighborMasks) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sph_density_nbr: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    return launch_density(FG_F4(orderedPositions), FG_F2(mass_pressure), gridStart, gridEnd, numParticles,
+                          gasConstant, restDensity, mass, neighborMasks);
+}
+
+int fg_sph_forces_integrate_nbr(fg_float4* orderedPositions, fg_float4* orderedVelocity,
+                                fg_float2* density_pressure, unsigned int* gridStart, unsigned int* gridEnd,
+                                const unsigned long long* neighborMasks, fg_float4* vel, fg_float4* pos,
+                                unsigned int* particleId, unsigned int numParticles, float mass, float viscosity) {
+    if (numParticles == 0) return FG_OK;
+    if (!have_params("fg_sph_forces_integrate_nbr")) return FG_ERR_NOT_INITIALIZED;
+    if (!orderedPositions || !orderedVelocity || !density_pressure || !gridStart || !gridEnd || !neighborMasks ||
+        !vel || !particleId) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sph_forces_integrate_nbr: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    return launch_forces(FG_F4(orderedPositions), FG_F4(orderedVelocity), FG_F2(density_pressure), gridStart, gridEnd,
+                         FG_F4(vel), pos ? FG_F4(pos) : nullptr, particleId, numParticles, mass, viscosity,
+                         neighborMasks);
 }
 
 }  // extern "C"

@@ -49,6 +49,7 @@ GPUSPHFluid::GPUSPHFluid(Application::FluidParams fluidParams, Emitter::IEmitter
     d_ordered_vel = dalloc<float4>(n, "hipMalloc ordered_vel");
     mSortWorkspaceBytes = fg_sort_workspace_size((unsigned)n, key_bits_for(mNumBuckets));
     hip_check(hipMalloc(&d_sortWorkspace, mSortWorkspaceBytes ? mSortWorkspaceBytes : 256), "hipMalloc sort ws");
+    hip_check(hipMalloc((void**)&d_neighborMasks, fg_neighbor_mask_bytes((unsigned)n) + 256), "hipMalloc masks");
 
     // GPUSPHFluid.cpp:27-29 zeroes the velocities; the density field is
     // zeroed here as well (the reference leaves it uninitialised).
@@ -79,6 +80,7 @@ GPUSPHFluid::~GPUSPHFluid() {
     (void)hipFree(d_velocity);
     (void)hipFree(d_ordered_vel);
     (void)hipFree(d_sortWorkspace);
+    (void)hipFree(d_neighborMasks);
 }
 
 int GPUSPHFluid::copyPositionToCPU(float4** particlePositions) {

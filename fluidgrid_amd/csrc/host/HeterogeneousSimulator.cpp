@@ -133,6 +133,14 @@ void HeterogeneousSimulator::findFluidNeighbourhood(Fluid::GPUSPHFluid* fluid) {
 
 // HeterogeneousSimulator.cpp:153-159
 void HeterogeneousSimulator::calculateFluidDensity(Fluid::GPUSPHFluid* fluid) {
+    if (mOptions.fused) {
+        // Same density, plus the neighbour bitmasks the force pass reuses.
+        fg_sph_density_nbr(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
+                           reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashStart(),
+                           fluid->getHashEnd(), fluid->getNeighborMasks(), (unsigned)fluid->getNumParticles(),
+                           fluid->getGasConstant(), fluid->getRestDensity(), fluid->getMass());
+        return;
+    }
     gCalcMassDensity(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
                      reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashKey(),
                      fluid->getHashStart(), fluid->getHashEnd(), fluid->getNumParticles(), fluid->getGasConstant(),
@@ -143,12 +151,13 @@ void HeterogeneousSimulator::calculateFluidDensity(Fluid::GPUSPHFluid* fluid) {
 // rigid-body simulator through gSPHCalculateForcesWithSRB).
 void HeterogeneousSimulator::calculateFluidForces(Fluid::GPUSPHFluid* fluid, bool, bool) {
     if (mOptions.fused) {
-        fg_sph_forces_integrate(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
-                                reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()),
-                                reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashStart(),
-                                fluid->getHashEnd(), reinterpret_cast<fg_float4*>(fluid->getVelocity()),
-                                reinterpret_cast<fg_float4*>(fluid->getPosition()), fluid->getParticleID(),
-                                (unsigned)fluid->getNumParticles(), fluid->getMass(), fluid->getViscosity());
+        fg_sph_forces_integrate_nbr(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
+                                    reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()),
+                                    reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashStart(),
+                                    fluid->getHashEnd(), fluid->getNeighborMasks(),
+                                    reinterpret_cast<fg_float4*>(fluid->getVelocity()),
+                                    reinterpret_cast<fg_float4*>(fluid->getPosition()), fluid->getParticleID(),
+                                    (unsigned)fluid->getNumParticles(), fluid->getMass(), fluid->getViscosity());
     } else {
         gSPHCalculateForces(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
                             reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()),

@@ -54,6 +54,9 @@ SIGNATURES = {
     "fg_sort_pairs_ws": (i32, [vp, vp, u32, i32, vp, sz, i32]),
     "fg_calc_hash_key_only": (i32, [vp, vp, u32, u32, abi.Float3]),
     "fg_sph_forces_integrate": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32]),
+    "fg_neighbor_mask_bytes": (sz, [u32]),
+    "fg_sph_density_nbr": (i32, [vp, vp, vp, vp, vp, u32, f32, f32, f32]),
+    "fg_sph_forces_integrate_nbr": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32]),
     "fg_device_count": (i32, [C.POINTER(C.c_int)]),
     "fg_device_name": (i32, [i32, C.c_char_p, sz]),
     "fg_malloc": (i32, [C.POINTER(C.c_void_p), sz]),

@@ -91,6 +91,7 @@ public:
     void uploadState(const float4* pos, const float4* vel);
     void downloadState(float4* pos, float4* vel);
     void* getSortWorkspace() { return d_sortWorkspace; }
+    unsigned long long* getNeighborMasks() { return d_neighborMasks; }
     size_t getSortWorkspaceBytes() const { return mSortWorkspaceBytes; }
 
     Application::FluidParams mFluidParams;
@@ -109,6 +110,7 @@ private:
     float4* d_velocity = nullptr;
     float4* d_ordered_vel = nullptr;
     void* d_sortWorkspace = nullptr;
+    unsigned long long* d_neighborMasks = nullptr;
     size_t mSortWorkspaceBytes = 0;
 };
 

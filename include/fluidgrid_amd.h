@@ -232,6 +232,22 @@ int fg_sph_forces_integrate(fg_float4* orderedPositions, fg_float4* orderedVeloc
                             unsigned int* particleId, unsigned int numParticles,
                             float mass, float viscosity);
 
+/* Density and forces with per-particle neighbour bitmasks (FG_KEY_LINEAR
+ * with FG_MATH_FAST; other modes ignore the masks).  The density launch
+ * records, for each of a particle's 9 stencil rows, which particles of the
+ * row lie within h; the force launch then evaluates only those pairs.
+ * neighborMasks holds fg_neighbor_mask_bytes(n) bytes; the force launch must
+ * follow the density launch on the same sorted arrays.  pos may be NULL
+ * (no integration). */
+size_t fg_neighbor_mask_bytes(unsigned int numParticles);
+int fg_sph_density_nbr(fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,
+                       unsigned int* gridEnd, unsigned long long* neighborMasks, unsigned int numParticles,
+                       float gasConstant, float restDensity, float mass);
+int fg_sph_forces_integrate_nbr(fg_float4* orderedPositions, fg_float4* orderedVelocity,
+                                fg_float2* density_pressure, unsigned int* gridStart, unsigned int* gridEnd,
+                                const unsigned long long* neighborMasks, fg_float4* vel, fg_float4* pos,
+                                unsigned int* particleId, unsigned int numParticles, float mass, float viscosity);
+
 /* Device memory and timing helpers (thin wrappers over the HIP runtime). */
 int fg_device_count(int* count);
 int fg_device_name(int device, char* buf, size_t len);

@@ -153,3 +153,40 @@ def test_reference_math_matches_oracle_closely(fglib, orc):
     assert np.max(np.abs(rp[:, 0] - ref_rp[:, 0]) / ref_rp[:, 0]) < 2e-6
     exact = np.mean(rp[:, 0] == ref_rp[:, 0])
     print(f"reference-math density bit-exact fraction: {exact:.4f}")
+
+
+@pytest.mark.parametrize("which", ["cfg0_moving", "near_walls", "inactive", "compressed", "appmain_linear"])
+def test_neighbor_mask_path_matches_oracle(fglib, orc, which):
+    """fg_sph_density_nbr + fg_sph_forces_integrate_nbr (the simulator's
+    path): density within 1e-5, velocities within the force tolerance,
+    against the oracle; 'compressed' has rows longer than 64 particles and
+    exercises the unmasked fallback."""
+    if which == "appmain_linear":
+        s = scenes.appmain_scene()
+        s.key_mode = abi.KEY_LINEAR
+        s.vel[:, :3] = np.random.default_rng(8).normal(0, 5, (s.n, 3)).astype(np.float32)
+    else:
+        s = scene_case(which)
+    L = fglib.load()
+    sk, sp, start, end, spos, svel = grid_cpu(orc, s)
+    rho_init = np.full((s.n, 2), 3.25, np.float32)
+    ref_rp = orc.density(spos, start, end, s.fluid, s.common, s.kernels, s.key_mode, rho_p_init=rho_init)
+    fglib.init_params(s.common, s.kernels, s.key_mode, abi.MATH_FAST)
+    d = {k: fglib.DeviceArray.from_numpy(v) for k, v in dict(spos=spos, svel=svel, start=start, end=end, sp=sp,
+                                                             rp=rho_init, vel=s.vel).items()}
+    masks = fglib.DeviceArray((L.fg_neighbor_mask_bytes(s.n),), np.uint8)
+    fglib.check(L.fg_sph_density_nbr(d["spos"].ptr, d["rp"].ptr, d["start"].ptr, d["end"].ptr, masks.ptr, s.n,
+                                     s.fluid.gasConstant, s.fluid.restDensity, s.fluid.mass), "density_nbr")
+    rp = d["rp"].numpy()
+    active = spos[:, 3] > 0
+    check_density(rp[active], ref_rp[active], s)
+    assert np.array_equal(rp[~active], rho_init[~active])
+    # forces from the GPU density, compared with the oracle forces on the same density
+    v_ref, acc, _ = orc.forces(spos, svel, rp, start, end, sp, s.fluid, s.common, s.kernels, s.key_mode,
+                               v_out_init=s.vel, want_acc=True)
+    _, _, mag = orc.forces(spos, svel, rp.astype(np.float64), start, end, sp, s.fluid, s.common, s.kernels,
+                           s.key_mode, f64=True, want_mag=True)
+    fglib.check(L.fg_sph_forces_integrate_nbr(d["spos"].ptr, d["svel"].ptr, d["rp"].ptr, d["start"].ptr,
+                                              d["end"].ptr, masks.ptr, d["vel"].ptr, None, d["sp"].ptr, s.n,
+                                              s.fluid.mass, s.fluid.viscosity), "forces_nbr")
+    check_velocity(d["vel"].numpy(), v_ref.astype(np.float64), acc, mag, sp, s.common.deltaTimeIntegration)
