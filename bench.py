@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-8m", action="store_true")
     ap.add_argument("--unfused", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB ten times (reference for HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (this build runs on one GPU "
@@ -117,6 +119,15 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     log("device:", lib.device_name(0))
+    if a.copy_calibration:
+        nbytes = a.copy_calibration << 20
+        src, dst = lib.DeviceArray((nbytes,), np.uint8), lib.DeviceArray((nbytes,), np.uint8)
+        L = lib.load()
+        for _ in range(10):
+            lib.check(L.fg_copy_bytes(dst.ptr, src.ptr, nbytes), "copy")
+        lib.check(L.fg_synchronize(), "sync")
+        print(json.dumps({"copy_calibration_bytes": nbytes}))
+        return
 
     scene = scenes.config_scene(a.config)
     fused = not a.unfused

@@ -23,6 +23,10 @@ class Float4(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
 
 
+class Double4(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("w", C.c_double)]
+
+
 class Int3(C.Structure):
     _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int)]
 
@@ -93,6 +97,14 @@ class CommonRigidBodyParams(C.Structure):
         ("particleRadius", C.c_float),
         ("_pad1", C.c_byte * 4),
     ]
+
+
+class BodyGrid(C.Structure):
+    """fg_body_grid (include/fluidgrid_amd.h)"""
+    _fields_ = [("orderedSpheres", C.c_void_p), ("bodyVelocities", C.c_void_p), ("gridStart", C.c_void_p),
+                ("gridEnd", C.c_void_p), ("bounds", C.c_void_p), ("cellSize", Float3), ("numBuckets", C.c_uint),
+                ("relation", Int3), ("numSpheres", C.c_uint), ("numBodies", C.c_uint), ("fluidRadius", C.c_float),
+                ("sphereRadius", C.c_float), ("kSpring", C.c_float), ("kDamping", C.c_float)]
 
 
 KEY_REF_XOR = 0

@@ -1,0 +1,97 @@
+// fg_contact.hpp -- penalty contact of one sphere against the spheres of a
+// bucket grid (GPU_Kernels/K_DeviceFunctions.cu:32-159), written with the
+// reference's arithmetic and contraction off.
+#pragma once
+#include "fg_internal.hpp"
+
+namespace fg {
+#pragma clang fp contract(off)
+
+// ProcessCellForcesD, K_DeviceFunctions.cu:32-106.  For each sphere j of the
+// bucket within r_i + r_j + eps: F += (ks*l - kd*dot(n, v_j - v_i)) * n with
+// n = (x_j - x_i)/l, l = |x_j - x_i| (the spring term uses the distance).
+// Velocities: per body (indexed by w) when w_index, else per sphere; none
+// when vel is null.
+__device__ __forceinline__ F3 contact_cell(const float4& sp, const float4& sv, float sr, float cr,
+                                           const float4* __restrict__ cpos, const float4* __restrict__ cvel,
+                                           uint32_t key, const uint32_t* __restrict__ start,
+                                           const uint32_t* __restrict__ end, float ks, float kd, bool w_index,
+                                           bool group, int w_group, float eps, uint32_t limit, uint32_t vlimit) {
+    F3 F = f3(0.0f, 0.0f, 0.0f);
+    const float mind = sr + cr + eps;
+    const uint32_t s0 = start[key];
+    if (s0 != kEmpty) {
+        const uint32_t e = min(end[key], limit);
+        for (uint32_t i = s0; i < e; i++) {
+            const float4 pos = cpos[i];
+            float4 vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (group && f2i_sat(pos.w) == w_group) continue;
+            if (cvel) {
+                const uint32_t vi = w_index ? (uint32_t)f2i_sat(pos.w) : i;
+                if (vi < vlimit) vel = cvel[vi];
+            }
+            const F3 rel = sub(xyz(pos), xyz(sp));
+            const float lg = len(rel);
+            if (lg < mind) {
+                const F3 n = divs(rel, lg);
+                const F3 rv = sub(xyz(vel), xyz(sv));
+                F = add(F, mul(ks * lg - kd * dot(n, rv), n));
+            }
+        }
+    }
+    return F;
+}
+
+// ProcessCollisionForcesD, K_DeviceFunctions.cu:113-159: the (2r+1)^3
+// stencil in z, y, x order; REF_XOR grids rehash x + k*cs, LINEAR grids
+// step cell coordinates and skip cells outside the grid.
+__device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv, float sr,
+                                              const float4* __restrict__ cpos, const float4* __restrict__ cvel,
+                                              float cr, float ks, float kd, int3 rel,
+                                              const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                              const GridDesc& g, bool w_index, bool group, int w_group, float eps,
+                                              uint32_t limit, uint32_t vlimit) {
+    F3 F = f3(0.0f, 0.0f, 0.0f);
+    CellXYZ c{0, 0, 0};
+    if (g.mode == FG_KEY_LINEAR) c = grid_cell(sp.x, sp.y, sp.z, g);
+    for (int z = -rel.z; z <= rel.z; z++)
+        for (int y = -rel.y; y <= rel.y; y++)
+            for (int x = -rel.x; x <= rel.x; x++) {
+                uint32_t key;
+                if (g.mode == FG_KEY_REF_XOR) {
+                    const float ox = g.cs.x * (float)x, oy = g.cs.y * (float)y, oz = g.cs.z * (float)z;
+                    key = hash_ref_xor(sp.x + ox, sp.y + oy, sp.z + oz, g.cs.x, g.cs.y, g.cs.z, g.nb);
+                } else {
+                    const int a = c.x + x, b = c.y + y, d = c.z + z;
+                    if (a < 0 || b < 0 || d < 0 || a >= g.dims.x || b >= g.dims.y || d >= g.dims.z) continue;
+                    key = grid_linear_key(a, b, d, g);
+                }
+                F = add(F, contact_cell(sp, sv, sr, cr, cpos, cvel, key, start, end, ks, kd, w_index, group, w_group,
+                                        eps, limit, vlimit));
+            }
+    return F;
+}
+
+// Force on a fluid particle from one body grid (K_FluidSim.cu:372-386);
+// zero outside the expanded sphere bounds, where no contact is possible.
+__device__ __forceinline__ F3 fluid_body_contact(const float4& pos, const float4& vel, const BodyContact& bc) {
+    if (bc.aabb) {
+        const float4 lo = bc.aabb[0], hi = bc.aabb[1];
+        if (pos.x < lo.x || pos.y < lo.y || pos.z < lo.z || pos.x > hi.x || pos.y > hi.y || pos.z > hi.z)
+            return f3(0.0f, 0.0f, 0.0f);
+    }
+    return contact_stencil(pos, vel, bc.fluid_radius, bc.pos, bc.vel, bc.body_radius, bc.ks, bc.kd, bc.relation,
+                           bc.start, bc.end, bc.grid, bc.vel != nullptr, false, 0, bc.eps, bc.count, bc.nvel);
+}
+
+// External force of K_FluidSim.cu:369-386: static grid first, then the
+// dynamic grid, summed from zero.
+__device__ __forceinline__ F3 fluid_external(const float4& pos, const float4& vel, const Contacts& c) {
+    F3 e = f3(0.0f, 0.0f, 0.0f);
+    if (c.st.enabled) e = add(e, fluid_body_contact(pos, vel, c.st));
+    if (c.dyn.enabled) e = add(e, fluid_body_contact(pos, vel, c.dyn));
+    return e;
+}
+
+#pragma clang fp contract(fast)
+}  // namespace fg

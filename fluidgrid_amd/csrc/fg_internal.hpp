@@ -49,6 +49,30 @@ struct GridDesc {
     int mode;          // FG_KEY_REF_XOR or FG_KEY_LINEAR
 };
 
+// Penalty contact of fluid particles against a body-sphere grid
+// (K_DeviceFunctions.cu:32-159 as called from K_FluidSim.cu:380-386).
+struct BodyContact {
+    const float4* pos;      // body spheres in grid order, w = body index
+    const float4* vel;      // per-body velocity, indexed by w
+    const uint32_t* start;  // body grid cell ranges
+    const uint32_t* end;
+    const float4* aabb;     // [2] expanded bounds of all spheres, or null
+    GridDesc grid;          // REF_XOR body grid
+    int3 relation;          // stencil half-width per axis
+    float fluid_radius, body_radius, ks, kd, eps;
+    uint32_t count;         // number of body spheres
+    uint32_t nvel;          // number of entries of vel
+    int enabled;
+};
+
+// Body grids seen by the fluid force pass; `coupled` selects the force
+// expression of K_FluidSim.cu:389-390 (external term added after the mass
+// factor) over that of K_FluidSim.cu:287.
+struct Contacts {
+    BodyContact st, dyn;
+    int coupled;
+};
+
 inline uint32_t num_buckets(const fg_common_fluid_params& p) {
     return (uint32_t)p.bucket_size.x * (uint32_t)p.bucket_size.y * (uint32_t)p.bucket_size.z;
 }

@@ -1,0 +1,463 @@
+// fg_rigid.hip -- two-way fluid/rigid-body coupling (the work of
+// GPU_Kernels/K_RigidBodySim.cu and the coupled branch of K_FluidSim.cu):
+//  * fluid forces with penalty contact against static and dynamic body grids;
+//  * per body sphere: penalty force from the fluid and torque about the body
+//    centre of mass;
+//  * per-body force/torque sums in FP64 (replacing the CUDPP FP32 segmented
+//    scan), one workgroup per body with a fixed reduction order;
+//  * body centre of mass and inertia tensor in FP64 from the sphere set;
+//  * body integration (IntegrateRBK) and the reference's reduction helpers.
+#include "fg_contact.hpp"
+#include "fg_internal.hpp"
+#include "fg_quat.hpp"
+
+namespace fg {
+
+int launch_forces(float4* sp, float4* sv, float2* rho_p, uint32_t* start, uint32_t* end, float4* vout, float4* pout,
+                  uint32_t* id, uint32_t n, float m, float mu, const unsigned long long* masks, const Contacts* bcp);
+GridDesc fluid_grid_desc(const fg_common_fluid_params& cp, fg_float3 cs, uint32_t nb, int mode);
+
+#pragma clang fp contract(off)
+
+// K_RigidBodySim.cu:66-99
+__global__ void __launch_bounds__(256)
+    k_body_forces_from_fluid(const float4* __restrict__ rb_pos, const float4* __restrict__ rb_vel,
+                             const float4* __restrict__ rb_cm, float4* __restrict__ out_f, float4* __restrict__ out_t,
+                             uint32_t m, uint32_t nbodies, float radius, const float4* __restrict__ sph_pos,
+                             const float4* __restrict__ sph_vel, const uint32_t* __restrict__ sph_start,
+                             const uint32_t* __restrict__ sph_end, uint32_t nsph, float sph_radius, GridDesc g,
+                             int3 relation, float ks, float kd) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const float4 pos = rb_pos[i];
+    const int b = f2i_sat(pos.w);
+    const bool ok = b >= 0 && (uint32_t)b < nbodies;
+    const float4 vel = ok ? rb_vel[b] : make_float4(0, 0, 0, 0);
+    const float4 cm = ok ? rb_cm[b] : make_float4(0, 0, 0, 0);
+    const F3 F = contact_stencil(pos, vel, radius, sph_pos, sph_vel, sph_radius, ks, kd, relation, sph_start, sph_end,
+                                 g, false, false, 0, 0.0001f, nsph, nsph);
+    out_f[i] = make_float4(F.x, F.y, F.z, (float)b);
+    const float4 dir = make_float4(pos.x - cm.x, pos.y - cm.y, pos.z - cm.z, pos.w - cm.w);
+    float4 t = vec3_cross(dir, make_float4(F.x, F.y, F.z, 0.0f));
+    t.w = (float)b;
+    out_t[i] = t;
+}
+
+// K_RigidBodySim.cu:102-111
+__global__ void k_copy_rb_data(uint32_t* __restrict__ out_rb, uint32_t* __restrict__ out_idx,
+                               const float4* __restrict__ f, uint64_t m) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    out_rb[i] = (uint32_t)f[i].w;
+    out_idx[i] = (uint32_t)i;
+}
+
+// K_RigidBodySim.cu:114-134
+__global__ void k_reorder_force_torque(float4* __restrict__ of, float4* __restrict__ ot, const float4* __restrict__ f,
+                                       const float4* __restrict__ t, const uint32_t* __restrict__ idx, uint32_t m) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t s = idx[i];
+    if (s >= m) return;
+    of[i] = f[s];
+    ot[i] = t[s];
+}
+
+// K_RigidBodySim.cu:140-155 (one segment per body, so the += is race-free)
+__global__ void k_add_force_torque(const float4* __restrict__ sum_f, const float4* __restrict__ sum_t,
+                                   const float4* __restrict__ ordered_f, float4* __restrict__ to_f,
+                                   float4* __restrict__ to_t, const int* __restrict__ offset, uint64_t nb) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int o = offset[i];
+    const int b = (int)ordered_f[o].w;
+    to_f[b] = quat_add(to_f[b], sum_f[o]);
+    to_t[b] = quat_add(to_t[b], sum_t[o]);
+}
+
+// K_RigidBodySim.cu:160-262
+__global__ void k_integrate_rb(float4* __restrict__ pos, const float* __restrict__ mass, float4* __restrict__ vel,
+                               const float4* __restrict__ forces, uint64_t nb, float dt, float4 ext, float3 vmin,
+                               float3 vmax, float4* __restrict__ L, const float4* __restrict__ torque,
+                               const float4* __restrict__ inv_inertia, float4* __restrict__ orient, float vlimit) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    float4 F = forces[idx];
+    const float spd = sqrtf(F.x * F.x + F.y * F.y + F.z * F.z + F.w * F.w);
+    if (spd > vlimit) F = f4_scale(F, vlimit / spd);
+    float4 p = pos[idx];
+    const float mm = mass[idx];
+    const float4 acc = make_float4(((F.x + ext.x) / mm) * dt, ((F.y + ext.y) / mm) * dt, ((F.z + ext.z) / mm) * dt,
+                                   ((F.w + ext.w) / mm) * dt);
+    float4 v = quat_add(vel[idx], f4_scale(acc, dt));
+    p.x += v.x * dt;
+    p.y += v.y * dt;
+    p.z += v.z * dt;
+    float4 l = quat_add(L[idx], f4_scale(torque[idx], dt));
+    l = f4_scale(l, 0.9f);
+    L[idx] = l;
+    const float4 q = orient[idx];
+    float4 w = angular_velocity(inv_inertia + 3 * idx, l, q);
+    w.w = 0.0f;
+    w = f4_scale(quat_product(w, q), 0.5f);
+    const float4 nq = quat_normalize(quat_add(q, quat_product(f4_scale(f4_scale(w, dt), 0.1f), q)));
+    orient[idx] = nq;
+    if (p.y < vmin.y) { p.y = vmin.y; v.y *= -0.7f; }
+    if (p.x <= vmin.x) { p.x = vmin.x; v.x *= -0.7f; }
+    else if (p.x >= vmax.x) { p.x = vmax.x; v.x *= -0.7f; }
+    if (p.z <= vmin.z) { p.z = vmin.z; v.z *= -0.7f; }
+    else if (p.z >= vmax.z) { p.z = vmax.z; v.z *= -0.7f; }
+    pos[idx] = p;
+    vel[idx] = v;
+}
+
+#pragma clang fp contract(fast)
+
+// Per-body FP64 sums of per-sphere forces and torques.  Spheres of body b
+// occupy [offset[b], offset[b+1]) of the input arrays (body order).  One
+// workgroup per body; each lane sums a strided subset in index order, then a
+// fixed tree combines the lanes, so the result does not depend on timing.
+constexpr int kReduceThreads = 256;
+
+__global__ void __launch_bounds__(kReduceThreads)
+    k_body_reduce_f64(const float4* __restrict__ f, const float4* __restrict__ t, const uint32_t* __restrict__ offset,
+                      uint32_t nbodies, double4* __restrict__ out_f, double4* __restrict__ out_t,
+                      float4* __restrict__ acc_f, float4* __restrict__ acc_t) {
+    __shared__ double s[6][kReduceThreads];
+    const uint32_t b = blockIdx.x;
+    if (b >= nbodies) return;
+    const uint32_t lo = offset[b], hi = offset[b + 1];
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kReduceThreads) {
+        const float4 fi = f[i], ti = t[i];
+        a[0] += fi.x; a[1] += fi.y; a[2] += fi.z;
+        a[3] += ti.x; a[4] += ti.y; a[5] += ti.z;
+    }
+    for (int k = 0; k < 6; k++) s[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 6; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out_f[b] = make_double4(s[0][0], s[1][0], s[2][0], 0.0);
+        out_t[b] = make_double4(s[3][0], s[4][0], s[5][0], 0.0);
+        if (acc_f) acc_f[b] = make_float4((float)s[0][0], (float)s[1][0], (float)s[2][0], 0.0f);
+        if (acc_t) acc_t[b] = make_float4((float)s[3][0], (float)s[4][0], (float)s[5][0], 0.0f);
+    }
+}
+
+// Centre of mass (pass 0) or inertia tensor about `cm` (pass 1) in FP64 of
+// equal point masses M_b / count_b at the sphere centres.
+__global__ void __launch_bounds__(kReduceThreads)
+    k_body_mass_props_f64(const float4* __restrict__ world, const uint32_t* __restrict__ offset, uint32_t nbodies,
+                          const double* __restrict__ mass, double* __restrict__ cm, double* __restrict__ inertia,
+                          int pass) {
+    __shared__ double s[6][kReduceThreads];
+    const uint32_t b = blockIdx.x;
+    if (b >= nbodies) return;
+    const uint32_t lo = offset[b], hi = offset[b + 1];
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    const double c0 = pass ? cm[3 * b] : 0.0, c1 = pass ? cm[3 * b + 1] : 0.0, c2 = pass ? cm[3 * b + 2] : 0.0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kReduceThreads) {
+        const double x = world[i].x, y = world[i].y, z = world[i].z;
+        if (pass == 0) {
+            a[0] += x; a[1] += y; a[2] += z;
+        } else {
+            const double rx = x - c0, ry = y - c1, rz = z - c2;
+            a[0] += ry * ry + rz * rz;  // Ixx
+            a[1] += rx * rx + rz * rz;  // Iyy
+            a[2] += rx * rx + ry * ry;  // Izz
+            a[3] -= rx * ry;            // Ixy
+            a[4] -= rx * rz;            // Ixz
+            a[5] -= ry * rz;            // Iyz
+        }
+    }
+    for (int k = 0; k < 6; k++) s[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 6; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double cnt = (double)(hi - lo);
+        if (pass == 0) {
+            for (int k = 0; k < 3; k++) cm[3 * b + k] = cnt > 0 ? s[k][0] / cnt : 0.0;
+        } else {
+            const double mi = cnt > 0 ? mass[b] / cnt : 0.0;
+            double* I = inertia + 9 * b;
+            I[0] = mi * s[0][0]; I[4] = mi * s[1][0]; I[8] = mi * s[2][0];
+            I[1] = I[3] = mi * s[3][0];
+            I[2] = I[6] = mi * s[4][0];
+            I[5] = I[7] = mi * s[5][0];
+        }
+    }
+}
+
+// Bounds of all sphere centres, expanded by `margin`: out[0] = min, out[1] = max.
+__global__ void __launch_bounds__(1024) k_sphere_bounds(const float4* __restrict__ p, uint32_t m, float margin,
+                                                        float4* __restrict__ out) {
+    __shared__ float s[6][1024];
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (uint32_t i = threadIdx.x; i < m; i += 1024) {
+        const float4 q = p[i];
+        lo[0] = fminf(lo[0], q.x); lo[1] = fminf(lo[1], q.y); lo[2] = fminf(lo[2], q.z);
+        hi[0] = fmaxf(hi[0], q.x); hi[1] = fmaxf(hi[1], q.y); hi[2] = fmaxf(hi[2], q.z);
+    }
+    for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = lo[k];
+        s[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = make_float4(s[0][0] - margin, s[1][0] - margin, s[2][0] - margin, 0.0f);
+        out[1] = make_float4(s[3][0] + margin, s[4][0] + margin, s[5][0] + margin, 0.0f);
+    }
+}
+
+static GridDesc body_grid(fg_float3 cs, uint32_t nb) {
+    GridDesc g{};
+    g.cs = make_float3(cs.x, cs.y, cs.z);
+    g.nb = nb;
+    g.mode = FG_KEY_REF_XOR;
+    return g;
+}
+
+}  // namespace fg
+
+using namespace fg;
+
+extern "C" {
+
+// K_FluidSim.cu:615-680
+void gSPHCalculateForcesWithSRB(fg_float4* oldPositions, fg_float4* oldVelocity, fg_float4* forces, fg_float4* newVel,
+                                fg_float2* density_pressure, unsigned int* particleId, unsigned int* gridStart,
+                                unsigned int* gridEnd, unsigned int numParticles, float mass, float kViscosity,
+                                float _in_sph_particle_radius, fg_int3 _in_cell_size_relation,
+                                float _in_srb_particle_radius, fg_float3 _in_srb_cell_size, float _in_kspring,
+                                float _in_kdamping, fg_float4* _in_srb_positions, unsigned int* _in_srb_start_offset,
+                                unsigned int* _in_srb_end_offset, unsigned int _in_srb_num_buckets,
+                                fg_float4* _in_rb_positions, fg_float4* _in_rb_velocities,
+                                unsigned int* _in_rb_start_offset, unsigned int* _in_rb_end_offset,
+                                unsigned int _in_rb_num_buckets, bool useStaticRB, bool useDynamicRB) {
+    (void)forces;
+    if (numParticles == 0) return;
+    if (!oldPositions || !oldVelocity || !newVel || !density_pressure || !particleId || !gridStart || !gridEnd) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "gSPHCalculateForcesWithSRB: null pointer");
+        return;
+    }
+    if ((useStaticRB && (!_in_srb_positions || !_in_srb_start_offset || !_in_srb_end_offset ||
+                         _in_srb_num_buckets == 0)) ||
+        (useDynamicRB && (!_in_rb_positions || !_in_rb_velocities || !_in_rb_start_offset || !_in_rb_end_offset ||
+                          _in_rb_num_buckets == 0))) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "gSPHCalculateForcesWithSRB: body grid missing");
+        return;
+    }
+    Contacts c{};
+    c.coupled = 1;
+    const int3 rel = make_int3(_in_cell_size_relation.x, _in_cell_size_relation.y, _in_cell_size_relation.z);
+    // The body-sphere arrays carry no length in this signature; the ranges in
+    // the body grids bound every read.
+    c.st.enabled = useStaticRB;
+    c.st.pos = FG_F4(_in_srb_positions);
+    c.st.vel = nullptr;
+    c.st.start = _in_srb_start_offset;
+    c.st.end = _in_srb_end_offset;
+    c.st.grid = body_grid(_in_srb_cell_size, _in_srb_num_buckets);
+    c.st.relation = rel;
+    c.st.fluid_radius = _in_sph_particle_radius;
+    c.st.body_radius = _in_srb_particle_radius;
+    c.st.ks = _in_kspring;
+    c.st.kd = _in_kdamping;
+    c.st.eps = 0.003f;
+    c.st.count = 0xFFFFFFFFu;
+    c.st.nvel = 0;
+    c.dyn = c.st;
+    c.dyn.enabled = useDynamicRB;
+    c.dyn.pos = FG_F4(_in_rb_positions);
+    c.dyn.vel = FG_F4(_in_rb_velocities);
+    c.dyn.start = _in_rb_start_offset;
+    c.dyn.end = _in_rb_end_offset;
+    c.dyn.grid = body_grid(_in_srb_cell_size, _in_rb_num_buckets);
+    c.dyn.nvel = 0xFFFFFFFFu;
+    launch_forces(FG_F4(oldPositions), FG_F4(oldVelocity), FG_F2(density_pressure), gridStart, gridEnd,
+                  FG_F4(newVel), nullptr, particleId, numParticles, mass, kViscosity, nullptr, &c);
+}
+
+// Coupled forces with explicit sphere/body counts, optional neighbour masks
+// (fg_sph_density_nbr), optional fused integration and optional sphere
+// bounds used to skip the body walk far from every body.
+int fg_sph_forces_coupled(fg_float4* orderedPositions, fg_float4* orderedVelocity, fg_float2* density_pressure,
+                          unsigned int* gridStart, unsigned int* gridEnd, const unsigned long long* neighborMasks,
+                          fg_float4* vel, fg_float4* pos, unsigned int* particleId, unsigned int numParticles,
+                          float mass, float viscosity, const fg_body_grid* bodies) {
+    if (numParticles == 0) return FG_OK;
+    if (!orderedPositions || !orderedVelocity || !density_pressure || !gridStart || !gridEnd || !vel ||
+        !particleId || !bodies) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sph_forces_coupled: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    Contacts c{};
+    c.coupled = 1;
+    c.dyn.enabled = bodies->numSpheres > 0;
+    c.dyn.pos = reinterpret_cast<const float4*>(bodies->orderedSpheres);
+    c.dyn.vel = reinterpret_cast<const float4*>(bodies->bodyVelocities);
+    c.dyn.start = bodies->gridStart;
+    c.dyn.end = bodies->gridEnd;
+    c.dyn.aabb = reinterpret_cast<const float4*>(bodies->bounds);
+    c.dyn.grid = body_grid(bodies->cellSize, bodies->numBuckets);
+    c.dyn.relation = make_int3(bodies->relation.x, bodies->relation.y, bodies->relation.z);
+    c.dyn.fluid_radius = bodies->fluidRadius;
+    c.dyn.body_radius = bodies->sphereRadius;
+    c.dyn.ks = bodies->kSpring;
+    c.dyn.kd = bodies->kDamping;
+    c.dyn.eps = 0.003f;
+    c.dyn.count = bodies->numSpheres;
+    c.dyn.nvel = bodies->numBodies;
+    return launch_forces(FG_F4(orderedPositions), FG_F4(orderedVelocity), FG_F2(density_pressure), gridStart, gridEnd,
+                         FG_F4(vel), pos ? FG_F4(pos) : nullptr, particleId, numParticles, mass, viscosity,
+                         neighborMasks, &c);
+}
+
+// K_RigidBodySim.cu:309-342
+void gCalculateRigidBodyForcesFromFluids(fg_float4* _in_RBPositions, fg_float4* _in_RBVelocites,
+                                         fg_float4* _in_RBCMPos, fg_float4* _out_Forces, fg_float4* _out_torque,
+                                         unsigned int* _in_RBGridStart, unsigned int* _in_RBGridEnd,
+                                         unsigned int _in_NumParticles, fg_float3 _in_CellSize, float _in_Radius,
+                                         int _in_RBHashTableSize, fg_float4* _in_sph_particles,
+                                         fg_float4* _in_sph_velocities, unsigned int* _in_sph_grid_start,
+                                         unsigned int* _in_sph_grid_end, float _in_sph_particle_radius,
+                                         fg_float3 _in_sph_cell_size, unsigned int _in_sph_num_buckets,
+                                         fg_int3 _in_cell_size_relation, float _in_sph_kSpring,
+                                         float _in_sph_kDamping) {
+    (void)_in_RBGridStart;
+    (void)_in_RBGridEnd;
+    (void)_in_CellSize;
+    (void)_in_RBHashTableSize;
+    fg_body_forces_from_fluid(_in_RBPositions, _in_RBVelocites, _in_RBCMPos, _out_Forces, _out_torque,
+                              _in_NumParticles, 0xFFFFFFFFu, _in_Radius, _in_sph_particles, _in_sph_velocities,
+                              _in_sph_grid_start, _in_sph_grid_end, 0xFFFFFFFFu, _in_sph_particle_radius,
+                              _in_sph_cell_size, _in_sph_num_buckets, _in_cell_size_relation, _in_sph_kSpring,
+                              _in_sph_kDamping);
+}
+
+int fg_body_forces_from_fluid(fg_float4* spheres, fg_float4* bodyVelocities, fg_float4* bodyCM, fg_float4* outForces,
+                              fg_float4* outTorques, unsigned int numSpheres, unsigned int numBodies, float radius,
+                              fg_float4* fluidOrderedPositions, fg_float4* fluidOrderedVelocities,
+                              unsigned int* fluidGridStart, unsigned int* fluidGridEnd, unsigned int numFluid,
+                              float fluidRadius, fg_float3 fluidCellSize, unsigned int fluidNumBuckets,
+                              fg_int3 relation, float kSpring, float kDamping) {
+    if (numSpheres == 0) return FG_OK;
+    if (!spheres || !bodyVelocities || !bodyCM || !outForces || !outTorques || !fluidOrderedPositions ||
+        !fluidOrderedVelocities || !fluidGridStart || !fluidGridEnd) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_body_forces_from_fluid: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    const State& st = state();
+    // The fluid grid is walked with the key function it was built with.
+    const GridDesc g = fluid_grid_desc(st.common, fluidCellSize, fluidNumBuckets, st.key_mode);
+    hipLaunchKernelGGL(k_body_forces_from_fluid, dim3(grid_for(numSpheres, 256)), dim3(256), 0, st.stream,
+                       FG_F4(spheres), FG_F4(bodyVelocities), FG_F4(bodyCM), FG_F4(outForces), FG_F4(outTorques),
+                       numSpheres, numBodies, radius, FG_F4(fluidOrderedPositions), FG_F4(fluidOrderedVelocities),
+                       fluidGridStart, fluidGridEnd, numFluid, fluidRadius, g,
+                       make_int3(relation.x, relation.y, relation.z), kSpring, kDamping);
+    return check_launch("fg_body_forces_from_fluid") ? FG_OK : fg_peek_error();
+}
+
+// K_RigidBodySim.cu:344-352
+void gCopyRBDataToParticle(unsigned int* _out_force_rb, unsigned int* _out_force_index, fg_float4* _in_rb_particle_forces,
+                           size_t num_particles) {
+    if (num_particles == 0) return;
+    hipLaunchKernelGGL(k_copy_rb_data, dim3(grid_for(num_particles, 256)), dim3(256), 0, state().stream,
+                       _out_force_rb, _out_force_index, FG_F4(_in_rb_particle_forces), (uint64_t)num_particles);
+    check_launch("gCopyRBDataToParticle");
+}
+
+// K_RigidBodySim.cu:354-366
+void gReorderForceTorque(fg_float4* _out_force, fg_float4* _out_torque, fg_float4* _in_force, fg_float4* _in_torque,
+                         unsigned int* _in_ordered_index, unsigned int _num_values) {
+    if (_num_values == 0) return;
+    hipLaunchKernelGGL(k_reorder_force_torque, dim3(grid_for(_num_values, 256)), dim3(256), 0, state().stream,
+                       FG_F4(_out_force), FG_F4(_out_torque), FG_F4(_in_force), FG_F4(_in_torque), _in_ordered_index,
+                       _num_values);
+    check_launch("gReorderForceTorque");
+}
+
+// K_RigidBodySim.cu:368-377
+void gAddForceTorque(fg_float4* _in_sum_force, fg_float4* _in_sum_torque, fg_float4* _in_ordened_force,
+                     fg_float4* _out_to_force, fg_float4* _out_to_torque, int* _in_sum_offset, size_t num_rigid_body) {
+    if (num_rigid_body == 0) return;
+    hipLaunchKernelGGL(k_add_force_torque, dim3(grid_for(num_rigid_body, 256)), dim3(256), 0, state().stream,
+                       FG_F4(_in_sum_force), FG_F4(_in_sum_torque), FG_F4(_in_ordened_force), FG_F4(_out_to_force),
+                       FG_F4(_out_to_torque), _in_sum_offset, (uint64_t)num_rigid_body);
+    check_launch("gAddForceTorque");
+}
+
+// K_RigidBodySim.cu:381-392
+void gIntegrateRB(fg_float4* _in_RBPosition, float* _in_mass, fg_float4* _in_RBVelocities, fg_float4* _in_rbforces,
+                  size_t num_rb, float _elapsed_time_in_seconds, fg_float4 _in_external_force, fg_float3 min_volume,
+                  fg_float3 max_volume, fg_float4* _angular_momentum, fg_float4* _torque, fg_float4* _inertia_tensor,
+                  fg_float4* _orientation, fg_float4* _inv_inertia_tensor, float velocity_limit) {
+    (void)_inertia_tensor;
+    if (num_rb == 0) return;
+    hipLaunchKernelGGL(k_integrate_rb, dim3(grid_for(num_rb, 64)), dim3(64), 0, state().stream, FG_F4(_in_RBPosition),
+                       _in_mass, FG_F4(_in_RBVelocities), FG_F4(_in_rbforces), (uint64_t)num_rb,
+                       _elapsed_time_in_seconds,
+                       make_float4(_in_external_force.x, _in_external_force.y, _in_external_force.z,
+                                   _in_external_force.w),
+                       make_float3(min_volume.x, min_volume.y, min_volume.z),
+                       make_float3(max_volume.x, max_volume.y, max_volume.z), FG_F4(_angular_momentum),
+                       FG_F4(_torque), FG_F4(_inv_inertia_tensor), FG_F4(_orientation), velocity_limit);
+    check_launch("gIntegrateRB");
+}
+
+int fg_body_reduce_f64(const fg_float4* forces, const fg_float4* torques, const unsigned int* bodyOffsets,
+                       unsigned int numBodies, fg_double4* outForces, fg_double4* outTorques, fg_float4* accForces,
+                       fg_float4* accTorques) {
+    if (numBodies == 0) return FG_OK;
+    if (!forces || !torques || !bodyOffsets || !outForces || !outTorques) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_body_reduce_f64: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    hipLaunchKernelGGL(k_body_reduce_f64, dim3(numBodies), dim3(kReduceThreads), 0, state().stream,
+                       reinterpret_cast<const float4*>(forces), reinterpret_cast<const float4*>(torques), bodyOffsets,
+                       numBodies, reinterpret_cast<double4*>(outForces), reinterpret_cast<double4*>(outTorques),
+                       accForces ? FG_F4(accForces) : nullptr, accTorques ? FG_F4(accTorques) : nullptr);
+    return check_launch("fg_body_reduce_f64") ? FG_OK : fg_peek_error();
+}
+
+int fg_body_mass_properties_f64(const fg_float4* worldSpheres, const unsigned int* bodyOffsets, unsigned int numBodies,
+                                const double* masses, double* centreOfMass, double* inertia) {
+    if (numBodies == 0) return FG_OK;
+    if (!worldSpheres || !bodyOffsets || !masses || !centreOfMass || !inertia) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_body_mass_properties_f64: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    for (int pass = 0; pass < 2; pass++)
+        hipLaunchKernelGGL(k_body_mass_props_f64, dim3(numBodies), dim3(kReduceThreads), 0, state().stream,
+                           reinterpret_cast<const float4*>(worldSpheres), bodyOffsets, numBodies, masses,
+                           centreOfMass, inertia, pass);
+    return check_launch("fg_body_mass_properties_f64") ? FG_OK : fg_peek_error();
+}
+
+int fg_sphere_bounds(const fg_float4* spheres, unsigned int numSpheres, float margin, fg_float4* out2) {
+    if (!spheres || !out2) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sphere_bounds: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    hipLaunchKernelGGL(k_sphere_bounds, dim3(1), dim3(1024), 0, state().stream, reinterpret_cast<const float4*>(spheres),
+                       numSpheres, margin, FG_F4(out2));
+    return check_launch("fg_sphere_bounds") ? FG_OK : fg_peek_error();
+}
+
+}  // extern "C"

@@ -51,6 +51,13 @@ __global__ void k_fill_u32(uint32_t* __restrict__ dst, uint32_t value, uint64_t 
     }
 }
 
+// Streaming copy, 16 bytes per lane: the reference kernel for scaling the
+// HBM byte counters (a copy of B bytes reads B and writes B).
+__global__ void k_copy_u4(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t count) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) dst[k] = src[k];
+}
+
 void launch_fill_u32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t s) {
     if (count == 0) return;
     uint64_t work = ((uintptr_t)dst & 15u) == 0 ? (count + 3) / 4 : count;
@@ -194,6 +201,20 @@ int fg_memset_u32(unsigned int* dst, unsigned int value, size_t count) {
     }
     launch_fill_u32(dst, value, count, state().stream);
     return check_launch("fill") ? FG_OK : t_err;
+}
+
+int fg_copy_bytes(void* dst, const void* src, size_t bytes) {
+    if (bytes % 16 || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_copy_bytes: size and pointers must be 16-byte aligned");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    const uint64_t count = bytes / 16;
+    if (count == 0) return FG_OK;
+    unsigned blocks = (unsigned)((count + 255) / 256);
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_copy_u4, dim3(blocks), dim3(256), 0, state().stream, static_cast<uint4*>(dst),
+                       static_cast<const uint4*>(src), count);
+    return check_launch("copy") ? FG_OK : t_err;
 }
 
 int fg_synchronize(void) {

@@ -17,6 +17,7 @@
 //    reference; only rounding differs.
 #include "fg_internal.hpp"
 #include "fg_walls.hpp"
+#include "fg_contact.hpp"
 
 namespace fg {
 
@@ -129,9 +130,12 @@ __device__ __forceinline__ F3 bucket_internal_ref(uint32_t i, uint32_t s0, uint3
 // Acceleration after gravity, walls and clamp (K_FluidSim.cu:287-303), from
 // the pair sum `internal`.
 __device__ __forceinline__ F3 finish_accel(F3 internal, float m, const float4& pos, const float4& vel,
-                                           const fg_common_fluid_params& cp) {
+                                           const fg_common_fluid_params& cp, const Contacts& bc) {
     const F3 g = f3(cp.global_external_force.x, cp.global_external_force.y, cp.global_external_force.z);
     F3 total = add(mul(internal, m), g);
+    // K_FluidSim.cu:389-390: body contact forces are added after the mass
+    // factor (the uncoupled kernel adds nothing here).
+    if (bc.coupled) total = add(total, fluid_external(pos, vel, bc));
     F3 wall = walls_no_penetration(xyz(pos), xyz(vel), cp);
     wall = add(wall, walls_no_slip(xyz(pos), xyz(vel), cp));
     total = add(total, wall);
@@ -164,7 +168,7 @@ __global__ void __launch_bounds__(kSphThreads)
     k_forces_ref(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                  const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, float4* __restrict__ vout,
                  float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
-                 fg_common_fluid_params cp, fg_kernels_cache kc) {
+                 fg_common_fluid_params cp, fg_kernels_cache kc, Contacts bc) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pos = sp[i];
@@ -187,7 +191,7 @@ __global__ void __launch_bounds__(kSphThreads)
                                                                  cp.simulation_scale, cp.kernel_radius,
                                                                  kc.kSpikyGradient, kc.kViscosityLaplacian, mu));
                 }
-        a = finish_accel(internal, m, pos, vel, cp);
+        a = finish_accel(internal, m, pos, vel, cp, bc);
     }
     write_back(id[i], n, pos, vel, a, cp, vout, pout);
 }
@@ -287,6 +291,24 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
 // Rows longer than 64 particles are not encoded and are re-tested.
 constexpr uint32_t kRowMaskBits = 64;
 
+// Poly6 sum over four candidates; bit u of the returned nibble is set when
+// candidate u lies within h.
+__device__ __forceinline__ uint32_t density4(const float4& p, const float4& q0, const float4& q1, const float4& q2,
+                                             const float4& q3, float h2u, float hhu, float& acc) {
+    const float4 q[4] = {q0, q1, q2, q3};
+    uint32_t nib = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const float dx = p.x - q[u].x, dy = p.y - q[u].y, dz = p.z - q[u].z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float t = h2u - r2;
+        const bool hit = r2 < hhu;
+        acc = hit ? acc + t * t * t : acc;
+        nib |= hit ? (1u << u) : 0u;
+    }
+    return nib;
+}
+
 template <bool MASKS>
 __global__ void __launch_bounds__(kSphThreads)
     k_density_fast(const float4* __restrict__ sp, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
@@ -296,8 +318,11 @@ __global__ void __launch_bounds__(kSphThreads)
     if (i >= n) return;
     const float4 p = sp[i];
     if (!(p.w > 0.0f)) return;
-    const float h = cp.kernel_radius, hh = h * h, h2 = cp.kernel_radius_squared;
-    const float s2 = cp.simulation_scale * cp.simulation_scale;
+    // Distances are compared unscaled: r_u = r / s, so r < h <=> r_u^2 < (h/s)^2
+    // and h^2 - r^2 = s^2 (h^2/s^2 - r_u^2).
+    const float s = cp.simulation_scale, s2 = s * s;
+    const float h = cp.kernel_radius, hh = h * h;
+    const float hu = h / s, hhu = hu * hu, h2u = cp.kernel_radius_squared / s2;
     const float cut = hh * 1.0001f;
     const RowWalker w = make_rows(p, cp);
     float acc = 0.0f;
@@ -307,28 +332,39 @@ __global__ void __launch_bounds__(kSphThreads)
         unsigned long long msk = 0ull;
         uint32_t rb, re;
         if (row_range(w, dy, dz, cut, cp, start, end, n, rb, re)) {
-            const uint32_t len = re - rb, last = re - 1;
-            for (uint32_t kk = 0; kk < len; kk += 4) {
-                // Four independent loads in flight per lane.
-                const float4 q0 = sp[rb + kk];
-                const float4 q1 = sp[min(rb + kk + 1, last)];
-                const float4 q2 = sp[min(rb + kk + 2, last)];
-                const float4 q3 = sp[min(rb + kk + 3, last)];
-                const float4 qs[4] = {q0, q1, q2, q3};
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const float dx = p.x - qs[u].x, dyv = p.y - qs[u].y, dzv = p.z - qs[u].z;
-                    const float r2 = (dx * dx + dyv * dyv + dzv * dzv) * s2;
-                    const bool hit = (r2 < hh) && (kk + u < len);
-                    const float t = h2 - r2;
-                    acc += hit ? t * t * t : 0.0f;
-                    if (MASKS && hit && kk + u < kRowMaskBits) msk |= 1ull << (kk + u);
+            const uint32_t len = re - rb, n4 = len & ~3u;
+            const float4* pp = sp + rb;
+            uint32_t kk = 0;
+            if (n4) {
+                // Software-pipelined groups of four: the next group's loads are
+                // in flight while the current group is evaluated.
+                float4 a0 = pp[0], a1 = pp[1], a2 = pp[2], a3 = pp[3];
+                for (; kk + 4 < n4; kk += 4) {
+                    pp += 4;
+                    const float4 b0 = pp[0], b1 = pp[1], b2 = pp[2], b3 = pp[3];
+                    const uint32_t nib = density4(p, a0, a1, a2, a3, h2u, hhu, acc);
+                    if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
+                    a0 = b0; a1 = b1; a2 = b2; a3 = b3;
                 }
+                const uint32_t nib = density4(p, a0, a1, a2, a3, h2u, hhu, acc);
+                if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
+                kk += 4;
+                pp += 4;
+            }
+            if (kk < len) {  // 1-3 remaining candidates; the others are masked out
+                const uint32_t rest = len - kk;
+                const float4 far = make_float4(3.0e38f, 3.0e38f, 3.0e38f, 0.0f);
+                const float4 t0 = pp[0];
+                const float4 t1 = rest > 1 ? pp[1] : far;
+                const float4 t2 = rest > 2 ? pp[2] : far;
+                const uint32_t nib = density4(p, t0, t1, t2, far, h2u, hhu, acc);
+                if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
             }
         }
+        // Rows longer than 64 particles are never read through their mask.
         if (MASKS) masks[(size_t)row * n + i] = msk;
     }
-    const float rho = m * kc.kPoly6 * acc;
+    const float rho = m * kc.kPoly6 * (s2 * s2 * s2) * acc;
     rho_p[i] = make_float2(fmaxf(1.0f, rho), k * (rho - rho0));
 }
 
@@ -339,15 +375,14 @@ struct PairAcc {
 };
 
 __device__ __forceinline__ void pair_term(PairAcc& A, uint32_t i, uint32_t j, const float4& pos, const float4& vel,
-                                          float pri, float4 q, float4 vj, float2 dpj, float h, float hh, float s2) {
+                                          float pri, float4 q, float4 vj, float2 dpj, float h, float s, float hhu) {
     const float dx = pos.x - q.x, dy = pos.y - q.y, dz = pos.z - q.z;
-    const float r2 = (dx * dx + dy * dy + dz * dz) * s2;
-    if (r2 < hh && j != i) {
-        const float ir = __builtin_amdgcn_rsqf(r2);
-        const float r = r2 * ir;
-        const float hr = h - r;
+    const float r2u = dx * dx + dy * dy + dz * dz;  // unscaled
+    if (r2u < hhu && j != i) {
+        const float iru = __builtin_amdgcn_rsqf(r2u);
+        const float hr = h - r2u * iru * s;
         const float irj = __builtin_amdgcn_rcpf(dpj.x);
-        const float cp_ = hr * hr * (pri + dpj.y) * irj * ir;
+        const float cp_ = hr * hr * (pri + dpj.y) * irj * iru;
         A.px += dx * cp_;
         A.py += dy * cp_;
         A.pz += dz * cp_;
@@ -363,7 +398,8 @@ __global__ void __launch_bounds__(kSphThreads)
     k_forces_fast(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, float4* __restrict__ vout,
                   float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
-                  fg_common_fluid_params cp, fg_kernels_cache kc, const unsigned long long* __restrict__ masks) {
+                  fg_common_fluid_params cp, fg_kernels_cache kc, const unsigned long long* __restrict__ masks,
+                  Contacts bc) {
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pos = sp[i];
@@ -371,7 +407,8 @@ __global__ void __launch_bounds__(kSphThreads)
     F3 a = f3(0.0f, 0.0f, 0.0f);
     if (pos.w > 0.0f) {
         const float h = cp.kernel_radius, hh = h * h;
-        const float s = cp.simulation_scale, s2 = s * s;
+        const float s = cp.simulation_scale;
+        const float hu = h / s, hhu = hu * hu;
         const float cut = hh * 1.0001f;
         const float2 dpi = rho_p[i];
         const RowWalker w = make_rows(pos, cp);
@@ -392,18 +429,19 @@ __global__ void __launch_bounds__(kSphThreads)
                     const float4 q1 = sp[j1], q2 = sp[j2];
                     const float4 v1 = sv[j1], v2 = sv[j2];
                     const float2 d1 = rho_p[j1], d2 = rho_p[j2];
-                    pair_term(A, i, j1, pos, vel, dpi.y, q1, v1, d1, h, hh, s2);
-                    if (two) pair_term(A, i, j2, pos, vel, dpi.y, q2, v2, d2, h, hh, s2);
+                    pair_term(A, i, j1, pos, vel, dpi.y, q1, v1, d1, h, s, hhu);
+                    if (two) pair_term(A, i, j2, pos, vel, dpi.y, q2, v2, d2, h, s, hhu);
                 }
             } else {
-                for (uint32_t j = rb; j < re; j++) pair_term(A, i, j, pos, vel, dpi.y, sp[j], sv[j], rho_p[j], h, hh, s2);
+                for (uint32_t j = rb; j < re; j++) pair_term(A, i, j, pos, vel, dpi.y, sp[j], sv[j], rho_p[j], h, s, hhu);
             }
         }
+        // pressure: (s d_u / r) = d_u / r_u, so no scale factor remains
         const float iri = 1.0f / dpi.x;
-        const float cP = -s * kc.kSpikyGradient * iri;
+        const float cP = -kc.kSpikyGradient * iri;
         const float cV = kc.kViscosityLaplacian * mu * iri;
         const F3 internal = f3(cP * A.px + cV * A.vx, cP * A.py + cV * A.vy, cP * A.pz + cV * A.vz);
-        a = finish_accel(internal, m, pos, vel, cp);
+        a = finish_accel(internal, m, pos, vel, cp, bc);
     }
     write_back(id[i], n, pos, vel, a, cp, vout, pout);
 }
@@ -442,21 +480,24 @@ int launch_density(float4* sp, float2* rho_p, uint32_t* start, uint32_t* end, ui
 }
 
 int launch_forces(float4* sp, float4* sv, float2* rho_p, uint32_t* start, uint32_t* end, float4* vout,
-                  float4* pout, uint32_t* id, uint32_t n, float m, float mu, const unsigned long long* masks = nullptr) {
+                  float4* pout, uint32_t* id, uint32_t n, float m, float mu, const unsigned long long* masks = nullptr,
+                  const Contacts* bcp = nullptr) {
+    Contacts bc{};
+    if (bcp) bc = *bcp;
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
     if (use_fast() && masks)
         hipLaunchKernelGGL(k_forces_fast<true>, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout, pout, id,
-                           n, m, mu, st.common, st.kernels, masks);
+                           n, m, mu, st.common, st.kernels, masks, bc);
     else if (use_fast())
         hipLaunchKernelGGL(k_forces_fast<false>, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout, pout, id,
-                           n, m, mu, st.common, st.kernels, (const unsigned long long*)nullptr);
+                           n, m, mu, st.common, st.kernels, (const unsigned long long*)nullptr, bc);
     else if (st.key_mode == FG_KEY_LINEAR)
         hipLaunchKernelGGL(k_forces_ref<FG_KEY_LINEAR>, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout,
-                           pout, id, n, m, mu, st.common, st.kernels);
+                           pout, id, n, m, mu, st.common, st.kernels, bc);
     else
         hipLaunchKernelGGL(k_forces_ref<FG_KEY_REF_XOR>, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout,
-                           pout, id, n, m, mu, st.common, st.kernels);
+                           pout, id, n, m, mu, st.common, st.kernels, bc);
     return check_launch("forces") ? FG_OK : fg_peek_error();
 }
 

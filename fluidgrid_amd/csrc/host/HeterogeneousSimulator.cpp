@@ -147,9 +147,28 @@ void HeterogeneousSimulator::calculateFluidDensity(Fluid::GPUSPHFluid* fluid) {
                      fluid->getRestDensity(), fluid->getMass());
 }
 
-// HeterogeneousSimulator.cpp:161-205 (the coupled branch is wired by the
-// rigid-body simulator through gSPHCalculateForcesWithSRB).
-void HeterogeneousSimulator::calculateFluidForces(Fluid::GPUSPHFluid* fluid, bool, bool) {
+bool HeterogeneousSimulator::coupled() const {
+    RigidBody::RigidBodySimulator* rb = RigidBody::RigidBodySimulator::instance();
+    return rb->getNumDynamicBodies() > 0;
+}
+
+// HeterogeneousSimulator.cpp:161-205.  The coupled branch (commented out in
+// the reference, :174-186) is the dynamic-body grid of the rigid-body
+// simulator with the fluid contact radius 2*pointRadius (:179).
+void HeterogeneousSimulator::calculateFluidForces(Fluid::GPUSPHFluid* fluid, bool, bool useDynamicRB) {
+    if (useDynamicRB) {
+        const Application::CommonFluidParams& cp = Application::ApplicationData::instance()->getCommonFluidParams();
+        const fg_body_grid g = RigidBody::RigidBodySimulator::instance()->bodyGrid(2.0f * cp.pointRadius);
+        fg_sph_forces_coupled(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
+                              reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()),
+                              reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashStart(),
+                              fluid->getHashEnd(), mOptions.fused ? fluid->getNeighborMasks() : nullptr,
+                              reinterpret_cast<fg_float4*>(fluid->getVelocity()),
+                              mOptions.fused ? reinterpret_cast<fg_float4*>(fluid->getPosition()) : nullptr,
+                              fluid->getParticleID(), (unsigned)fluid->getNumParticles(), fluid->getMass(),
+                              fluid->getViscosity(), &g);
+        return;
+    }
     if (mOptions.fused) {
         fg_sph_forces_integrate_nbr(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
                                     reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()),
@@ -177,8 +196,16 @@ void HeterogeneousSimulator::integrateFluidForces(Fluid::GPUSPHFluid* fluid) {
                   fluid->getHashStart(), fluid->getHashEnd());
 }
 
+// Events: 0 start, 1 hash, 2 sort, 3 cells, 4 density, 5 bodies (collision
+// data + forces from the fluid), 6 fluid forces, 7 fluid integrate, 8 body
+// sums and integration.
 void HeterogeneousSimulator::enqueueStep() {
     const bool t = mOptions.stage_timing;
+    RigidBody::RigidBodySimulator* rb = RigidBody::RigidBodySimulator::instance();
+    const bool withBodies = coupled();
+    if (withBodies && mFluids.size() != 1)
+        throw std::runtime_error("HeterogeneousSimulator: rigid-body coupling supports exactly one fluid");
+    if (withBodies) rb->startInteraction();
     for (Fluid::GPUSPHFluid* f : mFluids) {
         f->beginUpdate();
         if (t) (void)hipEventRecord(mEv[0], mStream);
@@ -186,24 +213,42 @@ void HeterogeneousSimulator::enqueueStep() {
         if (t) (void)hipEventRecord(mEv[3], mStream);
         calculateFluidDensity(f);
         if (t) (void)hipEventRecord(mEv[4], mStream);
-        calculateFluidForces(f, false, false);
+        if (withBodies) {
+            rb->generateCollisionData();
+            rb->calculateRigidBodyForcesFromFluid(f->getOrderedPosition(), f->getOrderedVelocities(),
+                                                  f->getHashStart(), f->getHashEnd(), (unsigned)f->getNumParticles());
+        }
         if (t) (void)hipEventRecord(mEv[5], mStream);
-        if (!mOptions.fused) integrateFluidForces(f);
+        calculateFluidForces(f, false, withBodies);
         if (t) (void)hipEventRecord(mEv[6], mStream);
+        if (!mOptions.fused) integrateFluidForces(f);
+        if (t) (void)hipEventRecord(mEv[7], mStream);
         f->endUpdate();
     }
+    if (withBodies) {
+        rb->applyForces();
+        rb->integrate();
+        rb->endInteraction();
+    }
+    if (t) (void)hipEventRecord(mEv[8], mStream);
 }
 
 // HeterogeneousSimulator.cpp:54-127
 void HeterogeneousSimulator::doTimeStep() {
     StreamScope scope(mStream);
+    RigidBody::RigidBodySimulator::instance()->setStream(mStream);
     enqueueStep();
     fg_check("doTimeStep");
     if (mOptions.stage_timing && !mFluids.empty()) {
-        hip_check(hipEventSynchronize(mEv[6]), "stage timing");
-        float* out[6] = {&mTimes.hash_ms, &mTimes.sort_ms, &mTimes.cells_ms, &mTimes.density_ms, &mTimes.forces_ms,
-                         &mTimes.integrate_ms};
-        for (int k = 0; k < 6; k++) (void)hipEventElapsedTime(out[k], mEv[k], mEv[k + 1]);
+        hip_check(hipEventSynchronize(mEv[8]), "stage timing");
+        float body_a = 0, body_b = 0;
+        float* out[4] = {&mTimes.hash_ms, &mTimes.sort_ms, &mTimes.cells_ms, &mTimes.density_ms};
+        for (int k = 0; k < 4; k++) (void)hipEventElapsedTime(out[k], mEv[k], mEv[k + 1]);
+        (void)hipEventElapsedTime(&body_a, mEv[4], mEv[5]);
+        (void)hipEventElapsedTime(&mTimes.forces_ms, mEv[5], mEv[6]);
+        (void)hipEventElapsedTime(&mTimes.integrate_ms, mEv[6], mEv[7]);
+        (void)hipEventElapsedTime(&body_b, mEv[7], mEv[8]);
+        mTimes.bodies_ms = body_a + body_b;
     }
 }
 
@@ -214,6 +259,7 @@ void HeterogeneousSimulator::doTimeSteps(int steps) {
         return;
     }
     StreamScope scope(mStream);
+    RigidBody::RigidBodySimulator::instance()->setStream(mStream);
     if (!mGraphExec) {
         hip_check(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         enqueueStep();

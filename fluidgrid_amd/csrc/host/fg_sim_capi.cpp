@@ -14,6 +14,7 @@ namespace {
 struct SimState {
     std::vector<std::unique_ptr<Emitter::ArrayEmitter>> emitters;
     std::vector<std::unique_ptr<Fluid::GPUSPHFluid>> fluids;
+    std::vector<std::unique_ptr<RigidBody::RigidBody>> bodies;
 };
 
 SimState*& sim() {
@@ -37,6 +38,7 @@ extern "C" {
 int fg_sim_shutdown(void) {
     try {
         Simulators::HeterogeneousSimulator::destroy();
+        RigidBody::RigidBodySimulator::destroy();
         delete sim();
         sim() = nullptr;
         return FG_OK;
@@ -156,6 +158,62 @@ int fg_sim_fluid_buffers(int idx, fg_fluid_buffers* out) {
     return FG_OK;
 }
 
+int fg_sim_add_bodies(const fg_float4* worldSpheres, const unsigned int* bodyOffsets, unsigned int numBodies,
+                      const double* masses, float sphereRadius, const fg_common_rigid_body_params* rigidParams,
+                      float springForce, float dampingForce) {
+    if (!sim() || !worldSpheres || !bodyOffsets || !masses || !rigidParams) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        Application::ApplicationData* ad = Application::ApplicationData::instance();
+        ad->setCommonRigidBodyParams(Application::abi_cast<Application::CommonRigidBodyParams>(*rigidParams));
+        Application::SimulationParams sp{dampingForce, springForce};
+        ad->setSimulationParams(sp);
+        RigidBody::RigidBodySimulator* rb = RigidBody::RigidBodySimulator::instance();
+        for (unsigned b = 0; b < numBodies; b++) {
+            std::vector<float4> s;
+            for (unsigned i = bodyOffsets[b]; i < bodyOffsets[b + 1]; i++)
+                s.push_back(make_float4(worldSpheres[i].x, worldSpheres[i].y, worldSpheres[i].z, 0.0f));
+            sim()->bodies.push_back(
+                std::make_unique<RigidBody::RigidBody>(false, (float)masses[b], sphereRadius, std::move(s)));
+            rb->addElement(sim()->bodies.back().get());
+        }
+        rb->setStream(Simulators::HeterogeneousSimulator::instance()->stream());
+        rb->reset();
+        Simulators::HeterogeneousSimulator::instance()->invalidateGraph();
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_body_state(fg_float4* positions, fg_float4* velocities, fg_float4* orientations,
+                      fg_float4* angularMomenta) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        RigidBody::RigidBodySimulator::instance()->downloadBodies(
+            reinterpret_cast<float4*>(positions), reinterpret_cast<float4*>(velocities),
+            reinterpret_cast<float4*>(orientations), reinterpret_cast<float4*>(angularMomenta));
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_body_sums(double* forces4, double* torques4) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        RigidBody::RigidBodySimulator::instance()->downloadSums(forces4, torques4);
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_body_mass_properties(double* centreOfMass3, double* inertia9) {
+    if (!sim() || !centreOfMass3 || !inertia9) return FG_ERR_INVALID_ARGUMENT;
+    RigidBody::RigidBodySimulator::instance()->downloadMassProperties(centreOfMass3, inertia9);
+    return FG_OK;
+}
+
 int fg_sim_stage_times(float* out6) {
     if (!sim() || !out6) return FG_ERR_INVALID_ARGUMENT;
     const auto& t = Simulators::HeterogeneousSimulator::instance()->lastStageTimes();
@@ -165,6 +223,7 @@ int fg_sim_stage_times(float* out6) {
     out6[3] = t.density_ms;
     out6[4] = t.forces_ms;
     out6[5] = t.integrate_ms;
+    out6[6] = t.bodies_ms;
     return FG_OK;
 }
 

@@ -65,6 +65,7 @@ SIGNATURES = {
     "fg_memcpy_dtoh": (i32, [vp, vp, sz]),
     "fg_memcpy_dtod": (i32, [vp, vp, sz]),
     "fg_memset_u32": (i32, [vp, u32, sz]),
+    "fg_copy_bytes": (i32, [vp, vp, sz]),
     "fg_synchronize": (i32, []),
     "fg_event_create": (i32, [C.POINTER(C.c_void_p)]),
     "fg_event_destroy": (i32, [vp]),
@@ -82,6 +83,26 @@ SIGNATURES = {
     "fg_sim_fluid_buffers": (i32, [i32, vp]),
     "fg_sim_stage_times": (i32, [vp]),
     "fg_sim_shutdown": (i32, []),
+    "fg_sim_add_bodies": (i32, [vp, vp, u32, vp, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
+    "fg_sim_body_state": (i32, [vp, vp, vp, vp]),
+    "fg_sim_body_sums": (i32, [vp, vp]),
+    "fg_sim_body_mass_properties": (i32, [vp, vp]),
+    # coupling
+    "gSPHCalculateForcesWithSRB": (None, [vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32, f32, abi.Int3, f32,
+                                          abi.Float3, f32, f32, vp, vp, vp, u32, vp, vp, vp, vp, u32, C.c_bool,
+                                          C.c_bool]),
+    "gCalculateRigidBodyForcesFromFluids": (None, [vp, vp, vp, vp, vp, vp, vp, u32, abi.Float3, f32, i32, vp, vp,
+                                                   vp, vp, f32, abi.Float3, u32, abi.Int3, f32, f32]),
+    "gCopyRBDataToParticle": (None, [vp, vp, vp, sz]),
+    "gReorderForceTorque": (None, [vp, vp, vp, vp, vp, u32]),
+    "gAddForceTorque": (None, [vp, vp, vp, vp, vp, vp, sz]),
+    "gIntegrateRB": (None, [vp, vp, vp, vp, sz, f32, abi.Float4, abi.Float3, abi.Float3, vp, vp, vp, vp, vp, f32]),
+    "fg_sph_forces_coupled": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32, vp]),
+    "fg_body_forces_from_fluid": (i32, [vp, vp, vp, vp, vp, u32, u32, f32, vp, vp, vp, vp, u32, f32, abi.Float3,
+                                        u32, abi.Int3, f32, f32]),
+    "fg_body_reduce_f64": (i32, [vp, vp, vp, u32, vp, vp, vp, vp]),
+    "fg_body_mass_properties_f64": (i32, [vp, vp, u32, vp, vp, vp]),
+    "fg_sphere_bounds": (i32, [vp, u32, f32, vp]),
 }
 
 
@@ -247,9 +268,37 @@ class Simulator:
         )
 
     def stage_times(self):
-        out = (C.c_float * 6)()
+        out = (C.c_float * 7)()
         check(load().fg_sim_stage_times(out), "stage times")
-        return dict(zip(("hash", "sort", "cells", "density", "forces", "integrate"), list(out)))
+        return dict(zip(("hash", "sort", "cells", "density", "forces", "integrate", "bodies"), list(out)))
+
+    def add_bodies(self, bodies):
+        """Couple a scenes.BodySet to the fluid."""
+        world = np.ascontiguousarray(bodies.world, np.float32)
+        offs = np.ascontiguousarray(bodies.offsets, np.uint32)
+        mass = np.ascontiguousarray(bodies.mass, np.float64)
+        check(load().fg_sim_add_bodies(world.ctypes.data, offs.ctypes.data, bodies.b, mass.ctypes.data,
+                                       bodies.sphere_radius, C.byref(bodies.rb), bodies.spring, bodies.damping),
+              "fg_sim_add_bodies")
+        self.nbodies = bodies.b
+
+    def body_state(self):
+        b = self.nbodies
+        out = [np.empty((b, 4), np.float32) for _ in range(4)]
+        check(load().fg_sim_body_state(*[o.ctypes.data for o in out]), "body state")
+        return dict(zip(("pos", "vel", "orient", "L"), out))
+
+    def body_sums(self):
+        f = np.empty((self.nbodies, 4), np.float64)
+        t = np.empty((self.nbodies, 4), np.float64)
+        check(load().fg_sim_body_sums(f.ctypes.data, t.ctypes.data), "body sums")
+        return f, t
+
+    def body_mass_properties(self):
+        cm = np.empty((self.nbodies, 3), np.float64)
+        inertia = np.empty((self.nbodies, 3, 3), np.float64)
+        check(load().fg_sim_body_mass_properties(cm.ctypes.data, inertia.ctypes.data), "mass properties")
+        return cm, inertia
 
     def stream(self):
         return load().fg_sim_stream()

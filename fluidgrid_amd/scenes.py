@@ -216,3 +216,138 @@ def block_emitter(bmin, bmax, spacing, count):
                     used += 1
         pos_y = f(pos_y + spacing)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Rigid bodies
+# ---------------------------------------------------------------------------
+
+def _surface_voxels(sdf, lo, hi, d):
+    """Centres of the voxels (spacing d) whose cube can meet the surface
+    sdf = 0, i.e. |sdf(centre)| <= (sqrt(3)/2) d."""
+    axes = [np.arange(lo[k], hi[k] + 0.5 * d, d) for k in range(3)]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    sel = np.abs(sdf(X, Y, Z)) <= 0.5 * np.sqrt(3.0) * d
+    return np.stack([X[sel], Y[sel], Z[sel]], axis=1)
+
+
+def torus_spheres(R, r, d):
+    """Surface-voxel sample of a torus (axis +y, centred at the origin)."""
+    ext = R + r + d
+
+    def sdf(x, y, z):
+        return np.sqrt((np.sqrt(x * x + z * z) - R) ** 2 + y * y) - r
+
+    return _surface_voxels(sdf, (-ext, -(r + d), -ext), (ext, r + d, ext), d)
+
+
+def box_spheres(edge, d):
+    """Surface-voxel sample of a cube of the given edge centred at the origin."""
+    a = 0.5 * edge
+    ext = a + d
+
+    def sdf(x, y, z):
+        q = np.stack([np.abs(x) - a, np.abs(y) - a, np.abs(z) - a])
+        outside = np.sqrt((np.maximum(q, 0.0) ** 2).sum(0))
+        inside = np.minimum(q.max(0), 0.0)
+        return outside + inside
+
+    return _surface_voxels(sdf, (-ext,) * 3, (ext,) * 3, d)
+
+
+def quat_rotate(q, v):
+    """Rotate vectors v (N,3) by unit quaternion q = (x, y, z, w)."""
+    u = np.asarray(q[:3], np.float64)
+    w = float(q[3])
+    v = np.asarray(v, np.float64)
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+@dataclass
+class BodySet:
+    """Dynamic rigid bodies as world-space sphere sets (body-major order)."""
+    world: np.ndarray          # (M,4) float32, w = body index
+    offsets: np.ndarray        # (B+1,) uint32: spheres of body b are [offsets[b], offsets[b+1])
+    mass: np.ndarray           # (B,) float64
+    sphere_radius: float
+    rb: abi.CommonRigidBodyParams
+    spring: float              # SimulationParams.springForce (kSpring = -spring)
+    damping: float             # SimulationParams.dampingForce (kDamping = -damping)
+    fluid_radius: float        # contact radius of a fluid particle
+    names: list = field(default_factory=list)
+
+    @property
+    def m(self):
+        return int(self.world.shape[0])
+
+    @property
+    def b(self):
+        return int(self.mass.shape[0])
+
+
+def make_bodies(sets, masses, h, common, spring=2000.0, damping=50.0, names=None, buckets=(64, 64, 64)):
+    """Pack sphere sets (list of (K,3) world centres) into a BodySet with the
+    coupling parameters used by the coupled configurations.
+
+    The coupled passes take the fluid contact radius as 2 * pointRadius
+    (HeterogeneousSimulator.cpp:179, RigidBodySimulator.cpp:517), so this sets
+    common.pointRadius = fluid_radius / 2 on the fluid parameters passed in."""
+    world = np.concatenate([np.concatenate([s, np.full((len(s), 1), b)], 1) for b, s in enumerate(sets)]).astype(
+        np.float32)
+    offsets = np.concatenate([[0], np.cumsum([len(s) for s in sets])]).astype(np.uint32)
+    sphere_r = 0.5 * h
+    fluid_r = 0.25 * h
+    common.pointRadius = float(np.float32(0.5 * fluid_r))
+    fluid_r = 2.0 * common.pointRadius
+    contact = fluid_r + sphere_r + 0.003
+    rb = abi.CommonRigidBodyParams()
+    rb.rb_to_fluid_relation = abi.i3(1, 1, 1)
+    rb.st_bucket_size = abi.i3(1, 1, 1)
+    rb.dn_bucket_size = abi.i3(*buckets)
+    rb.st_keyBits = key_bits(buckets[0] * buckets[1] * buckets[2])
+    rb.velocity_limit = common.velocity_limit
+    rb.volume_min = abi.Float3(*common.volume_min)
+    rb.volume_max = abi.Float3(*common.volume_max)
+    g = common.global_external_force.y
+    m0 = float(masses[0])
+    rb.global_external_force.x, rb.global_external_force.y = 0.0, float(np.float32(g * m0))
+    rb.global_external_force.z, rb.global_external_force.w = 0.0, 0.0
+    rb.simulation_scale = common.simulation_scale
+    rb.deltaTimeIntegration = common.deltaTimeIntegration
+    # The body grid cell (getCellSize) spans the whole contact distance, so
+    # the 27-cell stencil finds every contact.
+    rb.particleRadius = float(np.float32(contact * 1.001))
+    return BodySet(world, offsets, np.asarray(masses, np.float64), float(np.float32(sphere_r)), rb, spring, damping,
+                   float(np.float32(fluid_r)), names or [])
+
+
+def coupled_config_scene(cfg_id=1, seed=None):
+    """Config 1 (1M dam break + one rigid torus, R=0.15, r=0.05, density 500,
+    0.3 m above the fluid surface) or config 2 (8M + four 0.1 m boxes,
+    density 700, seeded poses above the fluid)."""
+    s = config_scene(cfg_id)
+    h = s.info["h"]
+    nx, ny, nz = s.info["lattice"]
+    top = 0.5 * h * ny
+    cx, cz = 0.5 * h * (nx + 1) / 2.0, 0.5 * h * (nz + 1) / 2.0
+    if cfg_id == 1:
+        R, r = 0.15, 0.05
+        pts = torus_spheres(R, r, 0.5 * h) + np.array([cx, top + 0.3 + r, cz])
+        mass = 500.0 * 2.0 * np.pi ** 2 * R * r * r
+        bodies = make_bodies([pts], [mass], h, s.common, names=["torus"])
+    elif cfg_id == 2:
+        rng = np.random.default_rng(SEED_BASE + 2)
+        sets, masses = [], []
+        for b in range(4):
+            q = rng.normal(size=4)
+            q /= np.linalg.norm(q)
+            c = np.array([cx + (b % 2 - 0.5) * 0.25, top + 0.15 + 0.12 * b, cz + (b // 2 - 0.5) * 0.25])
+            sets.append(quat_rotate(q, box_spheres(0.1, 0.5 * h)) + c)
+            masses.append(700.0 * 0.1 ** 3)
+        bodies = make_bodies(sets, masses, h, s.common, names=[f"box{b}" for b in range(4)])
+    else:
+        raise ValueError(cfg_id)
+    s.name = f"coupled_cfg{cfg_id}"
+    s.info["bodies"] = bodies
+    return s

@@ -16,6 +16,7 @@
 
 #include "ApplicationData.h"
 #include "GPUSPHFluid.h"
+#include "RigidBodySimulator.h"
 
 namespace Simulation {
 namespace Simulators {
@@ -27,7 +28,7 @@ struct StepOptions {
 };
 
 struct StageTimes {
-    float hash_ms = 0, sort_ms = 0, cells_ms = 0, density_ms = 0, forces_ms = 0, integrate_ms = 0;
+    float hash_ms = 0, sort_ms = 0, cells_ms = 0, density_ms = 0, forces_ms = 0, integrate_ms = 0, bodies_ms = 0;
 };
 
 class HeterogeneousSimulator {
@@ -64,7 +65,8 @@ private:
     bool mOwnStream = false;
     hipGraph_t mGraph = nullptr;
     hipGraphExec_t mGraphExec = nullptr;
-    hipEvent_t mEv[7] = {};
+    hipEvent_t mEv[9] = {};
+    bool coupled() const;
     StageTimes mTimes;
     static HeterogeneousSimulator* mInstance;
 };

@@ -1,0 +1,114 @@
+// RigidBodySimulator.h -- dynamic rigid bodies coupled to the fluid, after
+// the reference's Simulators/RigidBodySimulator.h:130-165 and RigidBody.h.
+//
+// The reference wires this path only in commented code
+// (HeterogeneousSimulator.cpp:57, 82-102, 113-126); here it is live:
+//   generateCollisionData()              body spheres to world space, keyed
+//                                        and sorted into the body grid
+//   calculateRigidBodyForcesFromFluid()  penalty force and torque per sphere
+//   applyForces()                        per-body sums in FP64
+//   integrate()                          IntegrateRBK per body
+// Centre of mass and inertia tensor are computed in FP64 on the GPU from the
+// sphere set (equal point masses); the reference hard-codes
+// diag(200 m / 12) and its inverse 0.00125 (RigidBody.cpp:116-122).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "ApplicationData.h"
+
+namespace Simulation {
+namespace RigidBody {
+
+// RigidBody.h:28-33
+struct sInertiaTensor {
+    float4 Ix, Iy, Iz;
+};
+
+// One body as a set of spheres (RigidBody.h:45-88 without the scene node).
+class RigidBody {
+public:
+    // Spheres in world space; w is ignored.
+    RigidBody(bool isStatic, float mass, float particleRadius, std::vector<float4> worldSpheres);
+
+    // RigidBody.cpp:55-108 text format: "static" or "dynamic <mass>", then the
+    // sphere radius, a mesh name, the sphere count and one "x y z" line per
+    // sphere (positions relative to the body origin, placed at `origin`).
+    static RigidBody* loadFromFile(const std::string& filename, float3 origin);
+
+    const std::vector<float4>& getWorldSpheres() const { return mSpheres; }
+    int getNumParticles() const { return (int)mSpheres.size(); }
+    float getParticleRadius() const { return mParticleRadius; }
+    float getMass() const { return mMass; }
+    bool isStatic() const { return mIsStatic; }
+
+private:
+    bool mIsStatic;
+    float mMass;
+    float mParticleRadius;
+    std::vector<float4> mSpheres;
+};
+
+class RigidBodySimulator {
+public:
+    static RigidBodySimulator* instance();
+    static void destroy();
+
+    void addElement(RigidBody* rbElement);
+    // Uploads the bodies, computes centre of mass and inertia (FP64) and
+    // allocates the per-step buffers (RigidBodySimulator.cpp:216-311).
+    void reset();
+    void release();
+
+    void startInteraction();
+    void endInteraction();
+    void generateCollisionData();
+    void calculateRigidBodyForcesFromFluid(float4* dfluidParticlePos, float4* dfluidParticleVel,
+                                           unsigned int* dfluidHashStart, unsigned int* dfluidHashEnd,
+                                           unsigned int numFluid);
+    void applyForces();
+    void integrate();
+
+    int getNumDynamicBodies() const { return (int)mMass.size(); }
+    int getNumDynamicSpheres() const { return (int)mRel.size(); }
+    float4* getDynamicVelocities() { return d_vel; }
+    // The dynamic-body grid for the coupled fluid force pass.
+    fg_body_grid bodyGrid(float fluidRadius) const;
+
+    // Extensions: state readback (positions, velocities, orientations,
+    // angular momenta: float4 per body) and the FP64 sums of the last step.
+    void downloadBodies(float4* pos, float4* vel, float4* orient, float4* L) const;
+    void downloadSums(double* forces4, double* torques4) const;
+    void downloadMassProperties(double* cm3, double* inertia9) const;
+    void setStream(hipStream_t s) { mStream = s; }
+
+private:
+    RigidBodySimulator() = default;
+    ~RigidBodySimulator();
+
+    std::vector<RigidBody*> mDynamic;
+    std::vector<float4> mRel;          // spheres relative to the centre of mass, w = body
+    std::vector<unsigned int> mOffsets;
+    std::vector<float> mMass;
+    std::vector<double> mCM, mInertia;  // FP64, 3 and 9 per body
+    float mSphereRadius = 0.0f;
+    bool mInteraction = false;
+    hipStream_t mStream = nullptr;
+
+    // device
+    float4 *d_rel = nullptr, *d_world = nullptr, *d_sorted = nullptr, *d_cm = nullptr, *d_vel = nullptr;
+    float4 *d_orient = nullptr, *d_L = nullptr, *d_force = nullptr, *d_torque = nullptr, *d_inv = nullptr;
+    float4 *d_inertia = nullptr, *d_pf = nullptr, *d_pt = nullptr, *d_bounds = nullptr;
+    float* d_mass = nullptr;
+    double *d_sumF = nullptr, *d_sumT = nullptr, *d_massd = nullptr, *d_cmd = nullptr, *d_inertiad = nullptr;
+    unsigned int *d_offsets = nullptr, *d_key = nullptr, *d_id = nullptr, *d_start = nullptr, *d_end = nullptr;
+    void* d_sortWs = nullptr;
+    size_t mSortWsBytes = 0;
+    unsigned int mBuckets = 0;
+
+    static RigidBodySimulator* mInstance;
+};
+
+}  // namespace RigidBody
+}  // namespace Simulation

@@ -163,6 +163,61 @@ void gSPHIntegrate(fg_float4* _in_sph_positions, fg_float4* _in_sph_velocity,
                    fg_float2* _in_density_press, unsigned int* _in_grid_start,
                    unsigned int* _in_grid_end);
 
+/* K_FluidSim.cuh:46-78 -- gSPHCalculateForces plus penalty contact against a
+ * static body grid and a dynamic body grid (velocities indexed by body id). */
+void gSPHCalculateForcesWithSRB(fg_float4* oldPositions, fg_float4* oldVelocity,
+                                fg_float4* forces, fg_float4* newVel,
+                                fg_float2* density_pressure, unsigned int* particleId,
+                                unsigned int* gridStart, unsigned int* gridEnd,
+                                unsigned int numParticles, float mass, float kViscosity,
+                                float _in_sph_particle_radius, fg_int3 _in_cell_size_relation,
+                                float _in_srb_particle_radius, fg_float3 _in_srb_cell_size,
+                                float _in_kspring, float _in_kdamping,
+                                fg_float4* _in_srb_positions, unsigned int* _in_srb_start_offset,
+                                unsigned int* _in_srb_end_offset, unsigned int _in_srb_num_buckets,
+                                fg_float4* _in_rb_positions, fg_float4* _in_rb_velocities,
+                                unsigned int* _in_rb_start_offset, unsigned int* _in_rb_end_offset,
+                                unsigned int _in_rb_num_buckets,
+                                bool useStaticRB, bool useDynamicRB);
+
+/* K_RigidBodySim.cuh:27-50 -- per body sphere: penalty force from the fluid
+ * grid and torque (x - cm[w]) x F; F.w = torque.w = body id. */
+void gCalculateRigidBodyForcesFromFluids(fg_float4* _in_RBPositions, fg_float4* _in_RBVelocites,
+                                         fg_float4* _in_RBCMPos, fg_float4* _out_Forces,
+                                         fg_float4* _out_torque, unsigned int* _in_RBGridStart,
+                                         unsigned int* _in_RBGridEnd, unsigned int _in_NumParticles,
+                                         fg_float3 _in_CellSize, float _in_Radius,
+                                         int _in_RBHashTableSize, fg_float4* _in_sph_particles,
+                                         fg_float4* _in_sph_velocities,
+                                         unsigned int* _in_sph_grid_start,
+                                         unsigned int* _in_sph_grid_end,
+                                         float _in_sph_particle_radius, fg_float3 _in_sph_cell_size,
+                                         unsigned int _in_sph_num_buckets,
+                                         fg_int3 _in_cell_size_relation,
+                                         float _in_sph_kSpring, float _in_sph_kDamping);
+
+/* K_RigidBodySim.cuh:52-55 -- key = (uint)F.w (body id), value = index. */
+void gCopyRBDataToParticle(unsigned int* _out_force_rb, unsigned int* _out_force_index,
+                           fg_float4* _in_rb_particle_forces, size_t num_particles);
+
+/* K_RigidBodySim.cuh:57-59 -- gather F and torque through sorted indices. */
+void gReorderForceTorque(fg_float4* _out_force, fg_float4* _out_torque, fg_float4* _in_force,
+                         fg_float4* _in_torque, unsigned int* _in_ordered_index,
+                         unsigned int _num_values);
+
+/* K_RigidBodySim.cuh:61-62 -- body[F.w] += scan value at each segment end. */
+void gAddForceTorque(fg_float4* _in_sum_force, fg_float4* _in_sum_torque,
+                     fg_float4* _in_ordened_force, fg_float4* _out_to_force,
+                     fg_float4* _out_to_torque, int* _in_sum_offset, size_t num_rigid_body);
+
+/* K_RigidBodySim.cuh:63-65 -- body linear/angular integration with floor and
+ * side-wall bounce. */
+void gIntegrateRB(fg_float4* _in_RBPosition, float* _in_mass, fg_float4* _in_RBVelocities,
+                  fg_float4* _in_rbforces, size_t num_rb, float _elapsed_time_in_seconds,
+                  fg_float4 _in_external_force, fg_float3 min_volume, fg_float3 max_volume,
+                  fg_float4* _angular_momentum, fg_float4* _torque, fg_float4* _inertia_tensor,
+                  fg_float4* _orientation, fg_float4* _inv_inertia_tensor, float velocity_limit);
+
 /* ------------------------------------------------------------------------ */
 /* 2. Extensions (no counterpart in the reference)                           */
 /* ------------------------------------------------------------------------ */
@@ -248,6 +303,59 @@ int fg_sph_forces_integrate_nbr(fg_float4* orderedPositions, fg_float4* orderedV
                                 const unsigned long long* neighborMasks, fg_float4* vel, fg_float4* pos,
                                 unsigned int* particleId, unsigned int numParticles, float mass, float viscosity);
 
+/* Body-sphere grid seen by the coupled fluid force pass (the dynamic-body
+ * arguments of gSPHCalculateForcesWithSRB, with explicit counts). */
+typedef struct fg_body_grid {
+    const fg_float4* orderedSpheres;   /* world sphere centres in grid order, w = body index */
+    const fg_float4* bodyVelocities;   /* [numBodies] */
+    const unsigned int* gridStart;     /* [numBuckets] */
+    const unsigned int* gridEnd;       /* [numBuckets] */
+    const fg_float4* bounds;           /* [2] expanded sphere bounds (fg_sphere_bounds) or NULL */
+    fg_float3 cellSize;                /* body grid cell size (REF_XOR keys) */
+    unsigned int numBuckets;
+    fg_int3 relation;                  /* stencil half-width per axis */
+    unsigned int numSpheres;
+    unsigned int numBodies;
+    float fluidRadius, sphereRadius;   /* contact at fluidRadius + sphereRadius + 0.003 */
+    float kSpring, kDamping;           /* F += (kSpring*l - kDamping*dot(n, dv)) n */
+} fg_body_grid;
+
+/* Coupled fluid forces: gSPHCalculateForcesWithSRB for the dynamic grid,
+ * plus the optional neighbour masks of fg_sph_density_nbr, optional fused
+ * integration (pos may be NULL) and sphere bounds that skip the body walk
+ * for particles that cannot touch a body. */
+int fg_sph_forces_coupled(fg_float4* orderedPositions, fg_float4* orderedVelocity, fg_float2* density_pressure,
+                          unsigned int* gridStart, unsigned int* gridEnd, const unsigned long long* neighborMasks,
+                          fg_float4* vel, fg_float4* pos, unsigned int* particleId, unsigned int numParticles,
+                          float mass, float viscosity, const fg_body_grid* bodies);
+
+/* gCalculateRigidBodyForcesFromFluids with explicit sphere/body/fluid counts
+ * (reads are bounded by them).  The fluid grid is walked with the current
+ * key mode. */
+int fg_body_forces_from_fluid(fg_float4* spheres, fg_float4* bodyVelocities, fg_float4* bodyCM, fg_float4* outForces,
+                              fg_float4* outTorques, unsigned int numSpheres, unsigned int numBodies, float radius,
+                              fg_float4* fluidOrderedPositions, fg_float4* fluidOrderedVelocities,
+                              unsigned int* fluidGridStart, unsigned int* fluidGridEnd, unsigned int numFluid,
+                              float fluidRadius, fg_float3 fluidCellSize, unsigned int fluidNumBuckets,
+                              fg_int3 relation, float kSpring, float kDamping);
+
+/* Per-body force and torque sums in FP64 (replaces the CUDPP segmented scan
+ * of RigidBodySimulator.cpp:551-553): spheres of body b occupy
+ * [bodyOffsets[b], bodyOffsets[b+1]).  accForces/accTorques (optional)
+ * receive the sums rounded to float for gIntegrateRB. */
+int fg_body_reduce_f64(const fg_float4* forces, const fg_float4* torques, const unsigned int* bodyOffsets,
+                       unsigned int numBodies, fg_double4* outForces, fg_double4* outTorques, fg_float4* accForces,
+                       fg_float4* accTorques);
+
+/* Centre of mass (3 doubles per body) and inertia tensor about it (9 doubles,
+ * row-major) in FP64 of equal point masses masses[b]/count at the sphere
+ * centres of each body. */
+int fg_body_mass_properties_f64(const fg_float4* worldSpheres, const unsigned int* bodyOffsets, unsigned int numBodies,
+                                const double* masses, double* centreOfMass, double* inertia);
+
+/* Bounds of all sphere centres expanded by margin: out2[0] min, out2[1] max. */
+int fg_sphere_bounds(const fg_float4* spheres, unsigned int numSpheres, float margin, fg_float4* out2);
+
 /* Device memory and timing helpers (thin wrappers over the HIP runtime). */
 int fg_device_count(int* count);
 int fg_device_name(int device, char* buf, size_t len);
@@ -257,6 +365,7 @@ int fg_memcpy_htod(void* dst, const void* src, size_t bytes);
 int fg_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int fg_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int fg_memset_u32(unsigned int* dst, unsigned int value, size_t count); /* fill kernel */
+int fg_copy_bytes(void* dst, const void* src, size_t bytes);          /* streaming copy kernel, 16-B aligned */
 int fg_synchronize(void);                  /* the selected stream */
 int fg_event_create(void** ev);
 int fg_event_destroy(void* ev);

@@ -43,9 +43,26 @@ void* fg_sim_stream(void);                        /* hipStream_t of the simulato
 int fg_sim_download(int fluid, fg_float4* positions, fg_float4* velocities);
 int fg_sim_upload(int fluid, const fg_float4* positions, const fg_float4* velocities);
 int fg_sim_fluid_buffers(int fluid, fg_fluid_buffers* out);
-/* hash, sort, cells, density, forces, integrate (ms) of the last step run
- * with stage_timing on. */
-int fg_sim_stage_times(float* out6);
+/* hash, sort, cells, density, forces, integrate, bodies (ms) of the last step
+ * run with stage_timing on (7 floats). */
+int fg_sim_stage_times(float* out7);
+
+/* Dynamic rigid bodies coupled to the (single) fluid: sphere centres in
+ * world space, body b owning [bodyOffsets[b], bodyOffsets[b+1]); masses in
+ * kg; the coupling parameters (ApplicationData::setCommonRigidBodyParams /
+ * setSimulationParams).  Centre of mass and inertia are computed on the GPU
+ * in FP64 from the spheres. */
+int fg_sim_add_bodies(const fg_float4* worldSpheres, const unsigned int* bodyOffsets, unsigned int numBodies,
+                      const double* masses, float sphereRadius, const fg_common_rigid_body_params* rigidParams,
+                      float springForce, float dampingForce);
+/* Per-body centre-of-mass position, velocity, orientation (x,y,z,w) and
+ * angular momentum; any pointer may be NULL. */
+int fg_sim_body_state(fg_float4* positions, fg_float4* velocities, fg_float4* orientations,
+                      fg_float4* angularMomenta);
+/* FP64 force and torque sums of the last step (4 doubles per body, w = 0). */
+int fg_sim_body_sums(double* forces4, double* torques4);
+/* FP64 centre of mass (3 per body) and inertia tensor (9 per body). */
+int fg_sim_body_mass_properties(double* centreOfMass3, double* inertia9);
 int fg_sim_shutdown(void);
 
 #ifdef __cplusplus

@@ -493,6 +493,332 @@ void orc_step(fg_float4* pos, fg_float4* vel, uint32_t n, const fg_fluid_params*
     orc_integrate(pos, vel, n, cp);
 }
 
+/* ======================================================================
+ * Rigid-body coupling
+ * ====================================================================== */
+
+/* A bucket grid walked by the penalty-contact stencil: the fluid grid (its
+ * key mode), or a body-sphere grid (REF_XOR). */
+struct OGrid {
+    int mode;
+    fg_float3 cs;
+    uint32_t nb;
+    fg_float3 vmin;
+    fg_int3 dims;
+};
+
+} // extern "C"
+
+namespace {
+
+inline uint32_t ogrid_key_of_cell(const OGrid& g, int cx, int cy, int cz) {
+    return (uint32_t)cx + (uint32_t)g.dims.x * ((uint32_t)cy + (uint32_t)g.dims.y * (uint32_t)cz);
+}
+
+/* K_DeviceFunctions.cu:32-106 (ProcessCellForcesD) */
+template <class R>
+V3<R> process_cell(const fg_float4& sp, const fg_float4& sv, R sr, R cr, const fg_float4* cpos, const fg_float4* cvel,
+                   uint32_t key, const uint32_t* start, const uint32_t* end, R ks, R kd, bool w_index, bool group,
+                   int w_group, R eps) {
+    V3<R> F = v3<R>(0, 0, 0);
+    const R mind = sr + cr + eps;
+    if (start[key] != kEmpty) {
+        for (uint32_t i = start[key]; i < end[key]; i++) {
+            const fg_float4 pos = cpos[i];
+            fg_float4 vel{0.0f, 0.0f, 0.0f, 0.0f};
+            if (group && f2i_sat(pos.w) == w_group) continue;
+            if (cvel) vel = w_index ? cvel[f2i_sat(pos.w)] : cvel[i];
+            const V3<R> rel = xyz<R>(pos) - xyz<R>(sp);
+            const R lg = len(rel);
+            if (lg < mind) {
+                const V3<R> n = rel / lg;
+                const V3<R> rv = xyz<R>(vel) - xyz<R>(sv);
+                F += repulsion(rv, n, lg, kd, ks);
+            }
+        }
+    }
+    return F;
+}
+
+/* K_DeviceFunctions.cu:113-159 (ProcessCollisionForcesD); LINEAR grids use
+ * the cell-offset stencil with out-of-range cells skipped. */
+template <class R>
+V3<R> process_collision(const fg_float4& sp, const fg_float4& sv, R sr, const fg_float4* cpos, const fg_float4* cvel,
+                        R cr, R ks, R kd, fg_int3 rel, const uint32_t* start, const uint32_t* end, const OGrid& g,
+                        bool w_index, bool group, int w_group, R eps) {
+    V3<R> F = v3<R>(0, 0, 0);
+    int cx = 0, cy = 0, cz = 0;
+    if (g.mode == FG_KEY_LINEAR) {
+        cx = cell_coord(sp.x, g.vmin.x, g.cs.x, g.dims.x);
+        cy = cell_coord(sp.y, g.vmin.y, g.cs.y, g.dims.y);
+        cz = cell_coord(sp.z, g.vmin.z, g.cs.z, g.dims.z);
+    }
+    for (int z = -rel.z; z <= rel.z; z++)
+        for (int y = -rel.y; y <= rel.y; y++)
+            for (int x = -rel.x; x <= rel.x; x++) {
+                uint32_t key;
+                if (g.mode == FG_KEY_REF_XOR) {
+                    fg_float3 off = g.cs;
+                    off.x *= (float)x; off.y *= (float)y; off.z *= (float)z;
+                    key = hash_ref_xor(sp.x + off.x, sp.y + off.y, sp.z + off.z, g.cs, g.nb);
+                } else {
+                    const int a = cx + x, b = cy + y, c = cz + z;
+                    if (a < 0 || b < 0 || c < 0 || a >= g.dims.x || b >= g.dims.y || c >= g.dims.z) continue;
+                    key = ogrid_key_of_cell(g, a, b, c);
+                }
+                F += process_cell<R>(sp, sv, sr, cr, cpos, cvel, key, start, end, ks, kd, w_index, group, w_group,
+                                     eps);
+            }
+    return F;
+}
+
+/* Coupled fluid forces, K_FluidSim.cu:315-413 (dynamic bodies only; the
+ * static-body branch is the same call against a static grid with no
+ * velocities). */
+template <class R, class In>
+void forces_coupled_t(const fg_float4* sp, const fg_float4* sv, const In* rho_p, const uint32_t* start,
+                      const uint32_t* end, R* v_out, const uint32_t* id, uint32_t n, float m, float mu,
+                      const fg_common_fluid_params& cp, const fg_kernels_cache& kc, int mode, float fluid_radius,
+                      fg_int3 relation, float body_radius, float ks, float kd, const fg_float4* rb_pos,
+                      const fg_float4* rb_vel, const uint32_t* rb_start, const uint32_t* rb_end, const OGrid& rbg,
+                      R* acc_out) {
+    const R scale = (R)cp.simulation_scale, h = (R)cp.kernel_radius;
+    const R kSpiky = (R)kc.kSpikyGradient, kVisc = (R)kc.kViscosityLaplacian;
+    const R vlim = (R)cp.velocity_limit, dt = (R)cp.deltaTimeIntegration;
+    const V3<R> g = v3<R>(cp.global_external_force.x, cp.global_external_force.y, cp.global_external_force.z);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+        const uint32_t i = (uint32_t)ii;
+        const fg_float4 pos = sp[i], vel = sv[i];
+        V3<R> total = v3<R>(0, 0, 0);
+        if (pos.w > 0.0f) {
+            uint32_t b[27];
+            int nb = stencil(pos, cp, mode, b);
+            const R r1 = (R)rho_p[2 * i], pr1 = (R)rho_p[2 * i + 1];
+            const V3<R> v1 = xyz<R>(vel);
+            V3<R> internal = v3<R>(0, 0, 0);
+            for (int c = 0; c < nb; c++) {
+                V3<R> part = v3<R>(0, 0, 0);
+                uint32_t s = start[b[c]];
+                if (s != kEmpty) {
+                    V3<R> p1 = xyz<R>(pos) * scale;
+                    uint32_t e = end[b[c]];
+                    for (uint32_t j = s; j < e; j++) {
+                        if (j == i) continue;
+                        V3<R> p2 = xyz<R>(sp[j]) * scale;
+                        const R r2 = (R)rho_p[2 * j], pr2 = (R)rho_p[2 * j + 1];
+                        const V3<R> v2 = xyz<R>(sv[j]);
+                        V3<R> diff = p1 - p2;
+                        R l = len(diff);
+                        if (!(l > h)) {
+                            R pp = (pr1 + pr2) / (r1 * r2);
+                            part += (diff / l) * (kSpiky * ((h - l) * (h - l))) * pp * (R)-1.0f;
+                            R lap = kVisc * (h - l);
+                            part += v3<R>(((v2.x - v1.x) / (r1 * r2)) * lap * (R)mu,
+                                          ((v2.y - v1.y) / (r1 * r2)) * lap * (R)mu,
+                                          ((v2.z - v1.z) / (r1 * r2)) * lap * (R)mu);
+                        }
+                    }
+                }
+                internal += part;
+            }
+            V3<R> external = process_collision<R>(pos, vel, (R)fluid_radius, rb_pos, rb_vel, (R)body_radius, (R)ks,
+                                                  (R)kd, relation, rb_start, rb_end, rbg, true, false, 0,
+                                                  (R)0.003f);
+            total = internal * (R)m + g + external;
+            V3<R> w = wall_terms<R>(pos, vel, total, cp);
+            total += w;
+            R speed = len(total);
+            if (speed > vlim) total = total * (vlim / speed);
+        }
+        const uint32_t o = id[i];
+        V3<R> dv = total * dt;
+        v_out[4 * o + 0] = (R)vel.x + dv.x;
+        v_out[4 * o + 1] = (R)vel.y + dv.y;
+        v_out[4 * o + 2] = (R)vel.z + dv.z;
+        v_out[4 * o + 3] = (R)vel.w + (R)0;
+        if (acc_out) { acc_out[3 * i] = total.x; acc_out[3 * i + 1] = total.y; acc_out[3 * i + 2] = total.z; }
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+/* K_HashGrid.cu:43-63: world = conj(q) (p,0) q + cm[w]; w kept; key; id. */
+void orc_body_transform(const fg_float4* rel, const fg_float4* cm, const fg_float4* q, uint32_t m, const OGrid* g,
+                        fg_float4* world, uint32_t* key, uint32_t* id) {
+    for (uint32_t i = 0; i < m; i++) {
+        const fg_float4 p = rel[i];
+        const int b = f2i_sat(p.w);
+        const fg_float4 o = q[b];
+        /* Quaternion_ProductD (K_MathUtils.cu:160-169) twice, conj(q) first. */
+        auto prod = [](fg_float4 a, fg_float4 c) {
+            float w = a.w * c.w - (a.x * c.x + a.y * c.y + a.z * c.z);
+            float cx = (a.y * c.z) - (a.z * c.y), cy = (a.z * c.x) - (a.x * c.z), cz = (a.x * c.y) - (a.y * c.x);
+            return fg_float4{(a.w * c.x) + (c.w * a.x) + cx, (a.w * c.y) + (c.w * a.y) + cy,
+                             (a.w * c.z) + (c.w * a.z) + cz, w};
+        };
+        fg_float4 pp{p.x, p.y, p.z, 0.0f};
+        fg_float4 t = prod(prod(fg_float4{-o.x, -o.y, -o.z, o.w}, pp), o);
+        fg_float4 a{t.x + cm[b].x, t.y + cm[b].y, t.z + cm[b].z, t.w + cm[b].w};
+        a.w = p.w;
+        world[i] = a;
+        id[i] = i;
+        key[i] = (g->mode == FG_KEY_REF_XOR)
+                     ? hash_ref_xor(a.x, a.y, a.z, g->cs, g->nb)
+                     : ogrid_key_of_cell(*g, cell_coord(a.x, g->vmin.x, g->cs.x, g->dims.x),
+                                         cell_coord(a.y, g->vmin.y, g->cs.y, g->dims.y),
+                                         cell_coord(a.z, g->vmin.z, g->cs.z, g->dims.z));
+    }
+}
+
+void orc_forces_coupled_f32(const fg_float4* sp, const fg_float4* sv, const float* rho_p, const uint32_t* start,
+                            const uint32_t* end, float* v_out, const uint32_t* id, uint32_t n, float m, float mu,
+                            const fg_common_fluid_params* cp, const fg_kernels_cache* kc, int mode,
+                            float fluid_radius, fg_int3 relation, float body_radius, float ks, float kd,
+                            const fg_float4* rb_pos, const fg_float4* rb_vel, const uint32_t* rb_start,
+                            const uint32_t* rb_end, const OGrid* rbg, float* acc_out) {
+    forces_coupled_t<float, float>(sp, sv, rho_p, start, end, v_out, id, n, m, mu, *cp, *kc, mode, fluid_radius,
+                                   relation, body_radius, ks, kd, rb_pos, rb_vel, rb_start, rb_end, *rbg, acc_out);
+}
+
+void orc_forces_coupled_f64(const fg_float4* sp, const fg_float4* sv, const double* rho_p, const uint32_t* start,
+                            const uint32_t* end, double* v_out, const uint32_t* id, uint32_t n, float m, float mu,
+                            const fg_common_fluid_params* cp, const fg_kernels_cache* kc, int mode,
+                            float fluid_radius, fg_int3 relation, float body_radius, float ks, float kd,
+                            const fg_float4* rb_pos, const fg_float4* rb_vel, const uint32_t* rb_start,
+                            const uint32_t* rb_end, const OGrid* rbg, double* acc_out) {
+    forces_coupled_t<double, double>(sp, sv, rho_p, start, end, v_out, id, n, m, mu, *cp, *kc, mode, fluid_radius,
+                                     relation, body_radius, ks, kd, rb_pos, rb_vel, rb_start, rb_end, *rbg,
+                                     acc_out);
+}
+
+/* K_RigidBodySim.cu:66-99: per body sphere, penalty force from the fluid and
+ * torque (x - cm[w]) x F; F.w = torque.w = (float)(int)w. */
+void orc_body_forces_from_fluid(const fg_float4* rb_pos, const fg_float4* rb_vel, const fg_float4* rb_cm,
+                                fg_float4* out_f, fg_float4* out_t, uint32_t m, float radius,
+                                const fg_float4* sph_pos, const fg_float4* sph_vel, const uint32_t* sph_start,
+                                const uint32_t* sph_end, float sph_radius, const OGrid* fg, fg_int3 relation,
+                                float ks, float kd) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < (int64_t)m; ii++) {
+        const uint32_t i = (uint32_t)ii;
+        const fg_float4 pos = rb_pos[i];
+        const int b = f2i_sat(pos.w);
+        const fg_float4 vel = rb_vel[b], cm = rb_cm[b];
+        V3<float> F = process_collision<float>(pos, vel, radius, sph_pos, sph_vel, sph_radius, ks, kd, relation,
+                                               sph_start, sph_end, *fg, false, false, 0, 0.0001f);
+        out_f[i] = fg_float4{F.x, F.y, F.z, (float)b};
+        const fg_float4 dir{pos.x - cm.x, pos.y - cm.y, pos.z - cm.z, pos.w - cm.w};
+        out_t[i] = fg_float4{(dir.y * F.z) - (dir.z * F.y), (dir.z * F.x) - (dir.x * F.z),
+                             (dir.x * F.y) - (dir.y * F.x), (float)b};
+    }
+}
+
+/* Per-body sums in FP64 of per-sphere forces and torques (the reduction of
+ * K_RigidBodySim.cu:102-155 and RigidBodySimulator.cpp:522-566, whose CUDPP
+ * scan sums in FP32); body index from F.w, sums in sphere order. */
+void orc_body_reduce_f64(const fg_float4* f, const fg_float4* t, uint32_t m, uint32_t nbodies, double* out_f,
+                         double* out_t) {
+    std::fill(out_f, out_f + 3 * nbodies, 0.0);
+    std::fill(out_t, out_t + 3 * nbodies, 0.0);
+    for (uint32_t i = 0; i < m; i++) {
+        const int b = f2i_sat(f[i].w);
+        if (b < 0 || (uint32_t)b >= nbodies) continue;
+        out_f[3 * b] += f[i].x; out_f[3 * b + 1] += f[i].y; out_f[3 * b + 2] += f[i].z;
+        out_t[3 * b] += t[i].x; out_t[3 * b + 1] += t[i].y; out_t[3 * b + 2] += t[i].z;
+    }
+}
+
+/* K_RigidBodySim.cu:160-262 (IntegrateRBK) with K_MathUtils.cu:72-236. */
+void orc_integrate_rb(fg_float4* pos, const float* mass, fg_float4* vel, const fg_float4* forces, uint32_t nb,
+                      float dt, fg_float4 ext, fg_float3 vmin, fg_float3 vmax, fg_float4* L, const fg_float4* torque,
+                      const fg_float4* inv_inertia, fg_float4* orient, float vlimit) {
+    for (uint32_t idx = 0; idx < nb; idx++) {
+        fg_float4 F = forces[idx];
+        const float spd = std::sqrt(F.x * F.x + F.y * F.y + F.z * F.z + F.w * F.w);
+        if (spd > vlimit) {
+            const float t = vlimit / spd;
+            F = fg_float4{F.x * t, F.y * t, F.z * t, F.w * t};
+        }
+        fg_float4 p = pos[idx];
+        const float mm = mass[idx];
+        const fg_float4 acc{((F.x + ext.x) / mm) * dt, ((F.y + ext.y) / mm) * dt, ((F.z + ext.z) / mm) * dt,
+                            ((F.w + ext.w) / mm) * dt};
+        fg_float4 v{vel[idx].x + (acc.x * dt), vel[idx].y + (acc.y * dt), vel[idx].z + (acc.z * dt),
+                    vel[idx].w + (acc.w * dt)};
+        p.x += v.x * dt;
+        p.y += v.y * dt;
+        p.z += v.z * dt;
+        fg_float4 l = L[idx];
+        const fg_float4 tq = torque[idx];
+        l = fg_float4{l.x + tq.x * dt, l.y + tq.y * dt, l.z + tq.z * dt, l.w + tq.w * dt};
+        l = fg_float4{l.x * 0.9f, l.y * 0.9f, l.z * 0.9f, l.w * 0.9f};
+        L[idx] = l;
+        /* AngularVelocityD: R I^-1 R^T L */
+        const fg_float4 q = orient[idx];
+        float R[3][3];
+        R[0][0] = 1 - (2 * q.y * q.y) - (2 * q.z * q.z);
+        R[0][1] = (2 * q.x * q.y) + (2 * q.w * q.z);
+        R[0][2] = (2 * q.x * q.z) - (2 * q.w * q.y);
+        R[1][0] = (2 * q.x * q.y) - (2 * q.w * q.z);
+        R[1][1] = 1 - (2 * q.x * q.x) - (2 * q.z * q.z);
+        R[1][2] = (2 * q.y * q.z) + (2 * q.w * q.x);
+        R[2][0] = (2 * q.x * q.z) + (2 * q.w * q.y);
+        R[2][1] = (2 * q.y * q.z) - (2 * q.w * q.x);
+        R[2][2] = 1 - (2 * q.x * q.x) - (2 * q.y * q.y);
+        const fg_float4* Ii = inv_inertia + 3 * idx;
+        float I[3][3] = {{Ii[0].x, Ii[0].y, Ii[0].z}, {Ii[1].x, Ii[1].y, Ii[1].z}, {Ii[2].x, Ii[2].y, Ii[2].z}};
+        float T[3][3], G[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) T[a][c] = R[a][0] * I[0][c] + R[a][1] * I[1][c] + R[a][2] * I[2][c];
+        for (int a = 0; a < 3; a++)  /* T * R^T */
+            for (int c = 0; c < 3; c++) G[a][c] = T[a][0] * R[c][0] + T[a][1] * R[c][1] + T[a][2] * R[c][2];
+        fg_float4 w{G[0][0] * l.x + G[0][1] * l.y + G[0][2] * l.z, G[1][0] * l.x + G[1][1] * l.y + G[1][2] * l.z,
+                    G[2][0] * l.x + G[2][1] * l.y + G[2][2] * l.z, 0.0f};
+        auto prod = [](fg_float4 a, fg_float4 c) {
+            float ww = a.w * c.w - (a.x * c.x + a.y * c.y + a.z * c.z);
+            float cx = (a.y * c.z) - (a.z * c.y), cy = (a.z * c.x) - (a.x * c.z), cz = (a.x * c.y) - (a.y * c.x);
+            return fg_float4{(a.w * c.x) + (c.w * a.x) + cx, (a.w * c.y) + (c.w * a.y) + cy,
+                             (a.w * c.z) + (c.w * a.z) + cz, ww};
+        };
+        fg_float4 wq = prod(w, q);
+        wq = fg_float4{wq.x * 0.5f, wq.y * 0.5f, wq.z * 0.5f, wq.w * 0.5f};
+        const fg_float4 scaled{wq.x * dt * 0.1f, wq.y * dt * 0.1f, wq.z * dt * 0.1f, wq.w * dt * 0.1f};
+        fg_float4 dq = prod(scaled, q);
+        fg_float4 nq{q.x + dq.x, q.y + dq.y, q.z + dq.z, q.w + dq.w};
+        const float norm = std::sqrt(nq.w * nq.w + nq.x * nq.x + nq.y * nq.y + nq.z * nq.z);
+        nq = fg_float4{nq.x / norm, nq.y / norm, nq.z / norm, nq.w / norm};
+        orient[idx] = nq;
+        if (p.y < vmin.y) { p.y = vmin.y; v.y *= -0.7f; }
+        if (p.x <= vmin.x) { p.x = vmin.x; v.x *= -0.7f; }
+        else if (p.x >= vmax.x) { p.x = vmax.x; v.x *= -0.7f; }
+        if (p.z <= vmin.z) { p.z = vmin.z; v.z *= -0.7f; }
+        else if (p.z >= vmax.z) { p.z = vmax.z; v.z *= -0.7f; }
+        pos[idx] = p;
+        vel[idx] = v;
+    }
+}
+
+/* Centre of mass and inertia tensor about it (FP64) of equal point masses
+ * M/count at the sphere centres of one body. */
+void orc_body_mass_props_f64(const fg_float4* world, uint32_t count, double mass, double* cm3, double* inertia9) {
+    double c[3] = {0, 0, 0};
+    for (uint32_t i = 0; i < count; i++) { c[0] += world[i].x; c[1] += world[i].y; c[2] += world[i].z; }
+    for (int a = 0; a < 3; a++) c[a] /= (double)count;
+    double I[9] = {0};
+    const double mi = mass / (double)count;
+    for (uint32_t i = 0; i < count; i++) {
+        const double r[3] = {world[i].x - c[0], world[i].y - c[1], world[i].z - c[2]};
+        const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) I[3 * a + b] += mi * ((a == b ? r2 : 0.0) - r[a] * r[b]);
+    }
+    for (int a = 0; a < 3; a++) cm3[a] = c[a];
+    for (int a = 0; a < 9; a++) inertia9[a] = I[a];
+}
+
 void orc_set_reverse_order(int r) { g_reverse = r; }
 
 int orc_num_threads(void) {

@@ -185,3 +185,123 @@ def run_steps(scene, steps, reverse=False, mode=None):
     finally:
         set_reverse_order(False)
     return pos, vel, rho_p
+
+
+# ---------------------------------------------------------------------------
+# Rigid-body coupling
+# ---------------------------------------------------------------------------
+
+class OGrid(C.Structure):
+    _fields_ = [("mode", C.c_int), ("cs", abi.Float3), ("nb", C.c_uint32), ("vmin", abi.Float3), ("dims", abi.Int3)]
+
+
+def grid_of_fluid(common, mode):
+    g = OGrid()
+    g.mode = mode
+    g.cs = abi.Float3(*common.cell_size)
+    g.nb = common.num_buckets()
+    g.vmin = abi.Float3(*common.volume_min)
+    g.dims = abi.Int3(*common.bucket_size)
+    return g
+
+
+def grid_of_bodies(cell, nb):
+    g = OGrid()
+    g.mode = abi.KEY_REF_XOR
+    g.cs = abi.Float3(cell, cell, cell)
+    g.nb = nb
+    return g
+
+
+def _rigid_sigs(L):
+    cpp = C.POINTER(abi.CommonFluidParams)
+    kcp = C.POINTER(abi.KernelsCache)
+    og = C.POINTER(OGrid)
+    L.orc_body_transform.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, og, _f32p, _u32p, _u32p]
+    L.orc_forces_coupled_f32.argtypes = [_f32p, _f32p, _f32p, _u32p, _u32p, _f32p, _u32p, C.c_uint32, C.c_float,
+                                         C.c_float, cpp, kcp, C.c_int, C.c_float, abi.Int3, C.c_float, C.c_float,
+                                         C.c_float, _f32p, _f32p, _u32p, _u32p, og, C.c_void_p]
+    L.orc_forces_coupled_f64.argtypes = [_f32p, _f32p, _f64p, _u32p, _u32p, _f64p, _u32p, C.c_uint32, C.c_float,
+                                         C.c_float, cpp, kcp, C.c_int, C.c_float, abi.Int3, C.c_float, C.c_float,
+                                         C.c_float, _f32p, _f32p, _u32p, _u32p, og, C.c_void_p]
+    L.orc_body_forces_from_fluid.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, C.c_uint32, C.c_float, _f32p, _f32p,
+                                             _u32p, _u32p, C.c_float, og, abi.Int3, C.c_float, C.c_float]
+    L.orc_body_reduce_f64.argtypes = [_f32p, _f32p, C.c_uint32, C.c_uint32, _f64p, _f64p]
+    L.orc_integrate_rb.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_uint32, C.c_float, abi.Float4, abi.Float3,
+                                   abi.Float3, _f32p, _f32p, _f32p, _f32p, C.c_float]
+    L.orc_body_mass_props_f64.argtypes = [_f32p, C.c_uint32, C.c_double, _f64p, _f64p]
+
+
+_rigid_ready = False
+
+
+def _rl():
+    global _rigid_ready
+    L = lib()
+    if not _rigid_ready:
+        _rigid_sigs(L)
+        _rigid_ready = True
+    return L
+
+
+def body_transform(rel, cm, q, grid):
+    rel = _c(rel, np.float32)
+    m = rel.shape[0]
+    world = np.zeros((m, 4), np.float32)
+    key = np.zeros(m, np.uint32)
+    pid = np.zeros(m, np.uint32)
+    _rl().orc_body_transform(rel, _c(cm, np.float32), _c(q, np.float32), m, C.byref(grid), world, key, pid)
+    return world, key, pid
+
+
+def forces_coupled(spos, svel, rho_p, start, end, pid, fluid, common, kernels, mode, fluid_radius, relation,
+                   body_radius, ks, kd, rb_pos, rb_vel, rb_start, rb_end, rb_grid, f64=False, v_out_init=None,
+                   want_acc=False):
+    spos, svel = _c(spos, np.float32), _c(svel, np.float32)
+    n = spos.shape[0]
+    dt = np.float64 if f64 else np.float32
+    v_out = np.zeros((n, 4), dt) if v_out_init is None else _c(v_out_init, dt).copy()
+    acc = np.zeros((n, 3), dt) if want_acc else None
+    fn = _rl().orc_forces_coupled_f64 if f64 else _rl().orc_forces_coupled_f32
+    fn(spos, svel, _c(rho_p, dt), _c(start, np.uint32), _c(end, np.uint32), v_out, _c(pid, np.uint32), n,
+       fluid.mass, fluid.viscosity, C.byref(common), C.byref(kernels), mode, fluid_radius, abi.Int3(*relation),
+       body_radius, ks, kd, _c(rb_pos, np.float32), _c(rb_vel, np.float32), _c(rb_start, np.uint32),
+       _c(rb_end, np.uint32), C.byref(rb_grid), acc.ctypes.data if want_acc else None)
+    return v_out, acc
+
+
+def body_forces_from_fluid(rb_pos, rb_vel, rb_cm, radius, spos, svel, start, end, fluid_radius, fluid_grid,
+                           relation, ks, kd):
+    rb_pos = _c(rb_pos, np.float32)
+    m = rb_pos.shape[0]
+    f = np.zeros((m, 4), np.float32)
+    t = np.zeros((m, 4), np.float32)
+    _rl().orc_body_forces_from_fluid(rb_pos, _c(rb_vel, np.float32), _c(rb_cm, np.float32), f, t, m, radius,
+                                     _c(spos, np.float32), _c(svel, np.float32), _c(start, np.uint32),
+                                     _c(end, np.uint32), fluid_radius, C.byref(fluid_grid), abi.Int3(*relation), ks,
+                                     kd)
+    return f, t
+
+
+def body_reduce_f64(f, t, nbodies):
+    f, t = _c(f, np.float32), _c(t, np.float32)
+    of = np.zeros((nbodies, 3), np.float64)
+    ot = np.zeros((nbodies, 3), np.float64)
+    _rl().orc_body_reduce_f64(f, t, f.shape[0], nbodies, of, ot)
+    return of, ot
+
+
+def integrate_rb(pos, mass, vel, forces, dt, ext, vmin, vmax, L, torque, inv_inertia, orient, vlimit):
+    pos, vel, L, orient = (_c(a, np.float32).copy() for a in (pos, vel, L, orient))
+    _rl().orc_integrate_rb(pos, _c(mass, np.float32), vel, _c(forces, np.float32), pos.shape[0], dt,
+                           abi.Float4(*ext), abi.Float3(*vmin), abi.Float3(*vmax), L, _c(torque, np.float32),
+                           _c(inv_inertia, np.float32), orient, vlimit)
+    return pos, vel, L, orient
+
+
+def body_mass_props(world, mass):
+    world = _c(world, np.float32)
+    cm = np.zeros(3, np.float64)
+    inertia = np.zeros(9, np.float64)
+    _rl().orc_body_mass_props_f64(world, world.shape[0], float(mass), cm, inertia)
+    return cm, inertia.reshape(3, 3)

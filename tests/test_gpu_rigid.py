@@ -1,0 +1,279 @@
+"""GPU parity of the rigid-body coupling through the C-ABI against the oracle.
+
+Tolerances:
+  body transform, keys                 bit-exact
+  body forces/torques from the fluid   <= 1 ulp (same IEEE operations in the
+                                        same order, contraction off)
+  coupled fluid forces                 |dv| <= dt*1e-5*(|a| + S + |F_ext|)
+  FP64 per-body sums                   rtol 1e-12 against numpy FP64
+  body integration                     bit-exact
+  mass properties (FP64)               rtol 1e-12
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from fluidgrid_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(lib, a):
+    return lib.DeviceArray.from_numpy(np.ascontiguousarray(a))
+
+
+def _ulp_diff(a, b):
+    a = np.ascontiguousarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.ascontiguousarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+def test_body_transform_bit_exact(fglib, orc):
+    L = fglib.load()
+    rng = np.random.default_rng(21)
+    m, nb, cell = 5000, 4096, 0.0105
+    rel = np.zeros((m, 4), np.float32)
+    rel[:, :3] = rng.normal(0, 0.1, (m, 3))
+    rel[:, 3] = rng.integers(0, 4, m)
+    q = rng.normal(size=(4, 4))
+    q = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+    cm = (rng.normal(size=(4, 4)) + 0.5).astype(np.float32)
+    d = {k: _dev(fglib, v) for k, v in dict(rel=rel, q=q, cm=cm).items()}
+    world = fglib.DeviceArray((m, 4), np.float32)
+    key = fglib.DeviceArray((m,), np.uint32)
+    pid = fglib.DeviceArray((m,), np.uint32)
+    L.gCalcHashKeyWithOffset(key.ptr, pid.ptr, d["cm"].ptr, d["q"].ptr, d["rel"].ptr, world.ptr, m,
+                             abi.Float3(cell, cell, cell), nb)
+    fglib.check(None, "gCalcHashKeyWithOffset")
+    ow, ok, op = orc.body_transform(rel, cm, q, orc.grid_of_bodies(cell, nb))
+    assert np.array_equal(world.numpy(), ow)
+    assert np.array_equal(key.numpy(), ok)
+    assert np.array_equal(pid.numpy(), op)
+
+
+def _fluid_and_torus(h=0.02, inside=True):
+    s = scenes.dam_break((40, 20, 40), h, (1.0, 1.0, 1.0), 1e-4)
+    rng = np.random.default_rng(5)
+    s.vel[:, :3] = rng.normal(0, 0.2, (s.n, 3)).astype(np.float32)
+    top = s.pos[:, 1].max()
+    cy = top - 0.05 if inside else top + 0.3
+    pts = scenes.torus_spheres(0.1, 0.03, 0.5 * h) + np.array([0.2, cy, 0.2])
+    bodies = scenes.make_bodies([pts], [2.0], h, s.common, spring=2000.0, damping=50.0)
+    return s, bodies
+
+
+def _fluid_grid(orc, s):
+    nb = s.num_buckets
+    key, pid = orc.hash_keys(s.pos, nb, tuple(s.common.cell_size), s.common, s.key_mode)
+    sk, sp = orc.sort_pairs(key, pid)
+    start, end, spos, svel = orc.cell_offsets(sk, sp, s.pos, s.vel, nb)
+    return sk, sp, start, end, spos, svel
+
+
+def test_body_forces_from_fluid_match_oracle(fglib, orc):
+    L = fglib.load()
+    s, b = _fluid_and_torus()
+    fglib.init_params(s.common, s.kernels, s.key_mode)
+    sk, sp, start, end, spos, svel = _fluid_grid(orc, s)
+    bvel = np.array([[0.05, -0.3, 0.02, 0.0]], np.float32)
+    cm = np.array([[*b.world[:, :3].mean(0), 0.0]], np.float32)
+    ks, kd = -b.spring, -b.damping
+    d = {k: _dev(fglib, v) for k, v in dict(w=b.world, bv=bvel, cm=cm, spos=spos, svel=svel, start=start,
+                                           end=end).items()}
+    f = fglib.DeviceArray((b.m, 4), np.float32)
+    t = fglib.DeviceArray((b.m, 4), np.float32)
+    fglib.check(L.fg_body_forces_from_fluid(d["w"].ptr, d["bv"].ptr, d["cm"].ptr, f.ptr, t.ptr, b.m, 1,
+                                            b.sphere_radius, d["spos"].ptr, d["svel"].ptr, d["start"].ptr,
+                                            d["end"].ptr, s.n, b.fluid_radius, abi.Float3(*s.common.cell_size),
+                                            s.num_buckets, abi.Int3(1, 1, 1), ks, kd), "forces from fluid")
+    of, ot = orc.body_forces_from_fluid(b.world, bvel, cm, b.sphere_radius, spos, svel, start, end, b.fluid_radius,
+                                        orc.grid_of_fluid(s.common, s.key_mode), (1, 1, 1), ks, kd)
+    gf, gt = f.numpy(), t.numpy()
+    assert np.count_nonzero(np.linalg.norm(of[:, :3], axis=1)) > 100  # the torus is in contact
+    assert _ulp_diff(gf, of).max() <= 1
+    assert _ulp_diff(gt, ot).max() <= 1
+    print("bit-exact fraction (forces):", np.mean(gf == of))
+
+
+def test_coupled_fluid_forces_match_oracle(fglib, orc):
+    """fg_sph_density_nbr + fg_sph_forces_coupled (the simulator's coupled
+    path) against orc_forces_coupled on the same density; the tolerance adds
+    the contact acceleration to the force scale."""
+    L = fglib.load()
+    s, b = _fluid_and_torus()
+    fglib.init_params(s.common, s.kernels, s.key_mode, abi.MATH_FAST)
+    sk, sp, start, end, spos, svel = _fluid_grid(orc, s)
+    cell = b.rb.particleRadius
+    nbb = b.rb.dn_bucket_size.x * b.rb.dn_bucket_size.y * b.rb.dn_bucket_size.z
+    bgrid = orc.grid_of_bodies(cell, nbb)
+    bkey, bpid = orc.hash_keys(b.world, nbb, (cell,) * 3, s.common, abi.KEY_REF_XOR)
+    bsk, bsp = orc.sort_pairs(bkey, bpid)
+    bstart, bend, bpos, _ = orc.cell_offsets(bsk, bsp, b.world, None, nbb)
+    bvel = np.array([[0.05, -0.3, 0.02, 0.0]], np.float32)
+    ks, kd = -b.spring, -b.damping
+    d = {k: _dev(fglib, v) for k, v in dict(spos=spos, svel=svel, start=start, end=end, sp=sp, vel=s.vel,
+                                           rp=np.zeros((s.n, 2), np.float32), bpos=bpos, bvel=bvel, bstart=bstart,
+                                           bend=bend).items()}
+    masks = fglib.DeviceArray((L.fg_neighbor_mask_bytes(s.n),), np.uint8)
+    fglib.check(L.fg_sph_density_nbr(d["spos"].ptr, d["rp"].ptr, d["start"].ptr, d["end"].ptr, masks.ptr, s.n,
+                                     s.fluid.gasConstant, s.fluid.restDensity, s.fluid.mass), "density")
+    rp = d["rp"].numpy()
+    args = (start, end, sp, s.fluid, s.common, s.kernels, s.key_mode, b.fluid_radius, (1, 1, 1), b.sphere_radius,
+            ks, kd, bpos, bvel, bstart, bend, bgrid)
+    v_ref, acc = orc.forces_coupled(spos, svel, rp, *args, v_out_init=s.vel, want_acc=True)
+    _, acc64 = orc.forces_coupled(spos, svel, rp.astype(np.float64), *args, f64=True, want_acc=True)
+    _, acc_un, mag = orc.forces(spos, svel, rp.astype(np.float64), start, end, sp, s.fluid, s.common, s.kernels,
+                                s.key_mode, f64=True, want_acc=True, want_mag=True)
+    ext = np.linalg.norm(acc64 - acc_un, axis=1)
+    assert np.count_nonzero(ext > 0) > 50  # particles in contact with the torus
+    for bounds in (False, True):
+        if bounds:
+            bb = fglib.DeviceArray((2, 4), np.float32)
+            fglib.check(L.fg_sphere_bounds(d["bpos"].ptr, b.m, b.fluid_radius + b.sphere_radius + 0.003 + 1e-4,
+                                           bb.ptr), "bounds")
+        vel = _dev(fglib, s.vel)
+        g = abi.BodyGrid()
+        g.orderedSpheres, g.bodyVelocities = d["bpos"].ptr, d["bvel"].ptr
+        g.gridStart, g.gridEnd, g.bounds = d["bstart"].ptr, d["bend"].ptr, (bb.ptr if bounds else None)
+        g.cellSize = abi.Float3(cell, cell, cell)
+        g.numBuckets, g.relation = nbb, abi.Int3(1, 1, 1)
+        g.numSpheres, g.numBodies = b.m, 1
+        g.fluidRadius, g.sphereRadius, g.kSpring, g.kDamping = b.fluid_radius, b.sphere_radius, ks, kd
+        fglib.check(L.fg_sph_forces_coupled(d["spos"].ptr, d["svel"].ptr, d["rp"].ptr, d["start"].ptr,
+                                            d["end"].ptr, masks.ptr, vel.ptr, None, d["sp"].ptr, s.n, s.fluid.mass,
+                                            s.fluid.viscosity, C.byref(g)), "coupled forces")
+        v_gpu = vel.numpy()
+        scale = np.zeros(s.n)
+        scale[sp] = np.linalg.norm(acc64, axis=1) + mag + ext
+        tol = s.common.deltaTimeIntegration * 1e-5 * scale[:, None] + 2 * np.spacing(np.abs(v_ref[:, :3]))
+        err = np.abs(v_gpu[:, :3].astype(np.float64) - v_ref[:, :3])
+        assert np.all(err <= tol), f"bounds={bounds}: {(err > tol).sum()} out of tolerance"
+
+
+def test_body_reduce_f64_matches_numpy(fglib):
+    L = fglib.load()
+    rng = np.random.default_rng(4)
+    counts = [1, 0, 700, 20000, 3]
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    m = int(offs[-1])
+    f = rng.normal(size=(m, 4)).astype(np.float32)
+    t = rng.normal(size=(m, 4)).astype(np.float32)
+    d = {k: _dev(fglib, v) for k, v in dict(f=f, t=t, o=offs).items()}
+    of = fglib.DeviceArray((len(counts), 4), np.float64)
+    ot = fglib.DeviceArray((len(counts), 4), np.float64)
+    af = fglib.DeviceArray((len(counts), 4), np.float32)
+    at = fglib.DeviceArray((len(counts), 4), np.float32)
+    fglib.check(L.fg_body_reduce_f64(d["f"].ptr, d["t"].ptr, d["o"].ptr, len(counts), of.ptr, ot.ptr, af.ptr, at.ptr),
+                "reduce")
+    gf, gt = of.numpy(), ot.numpy()
+    for b in range(len(counts)):
+        sl = slice(offs[b], offs[b + 1])
+        assert np.allclose(gf[b, :3], f[sl, :3].astype(np.float64).sum(0), rtol=1e-12, atol=1e-12)
+        assert np.allclose(gt[b, :3], t[sl, :3].astype(np.float64).sum(0), rtol=1e-12, atol=1e-12)
+    assert np.array_equal(af.numpy()[:, :3], gf[:, :3].astype(np.float32))
+    # deterministic: a second launch gives identical bits
+    fglib.check(L.fg_body_reduce_f64(d["f"].ptr, d["t"].ptr, d["o"].ptr, len(counts), of.ptr, ot.ptr, None, None),
+                "reduce")
+    assert np.array_equal(of.numpy(), gf)
+
+
+def test_integrate_rb_bit_exact(fglib, orc):
+    L = fglib.load()
+    rng = np.random.default_rng(6)
+    nb = 5
+    pos = rng.uniform(0.2, 0.8, (nb, 4)).astype(np.float32)
+    pos[0, 1] = 0.0001  # floor bounce
+    vel = rng.normal(0, 0.5, (nb, 4)).astype(np.float32)
+    vel[0, 1] = -3.0
+    mass = rng.uniform(0.5, 3, nb).astype(np.float32)
+    forces = rng.normal(0, 200, (nb, 4)).astype(np.float32)
+    forces[2, :3] *= 10  # clamped to velocity_limit
+    Lm = rng.normal(size=(nb, 4)).astype(np.float32)
+    tq = rng.normal(size=(nb, 4)).astype(np.float32)
+    inv = np.zeros((nb * 3, 4), np.float32)
+    for b in range(nb):
+        inv[3 * b:3 * b + 3, :3] = np.diag(rng.uniform(1, 5, 3))
+    q = rng.normal(size=(nb, 4))
+    q = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+    dt, ext = 0.01, (0.0, -9.81, 0.0, 0.0)
+    vmin, vmax = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    d = {k: _dev(fglib, v) for k, v in dict(pos=pos, mass=mass, vel=vel, f=forces, L=Lm, t=tq, inv=inv, q=q,
+                                           inert=inv).items()}
+    L.gIntegrateRB(d["pos"].ptr, d["mass"].ptr, d["vel"].ptr, d["f"].ptr, nb, dt, abi.Float4(*ext),
+                   abi.Float3(*vmin), abi.Float3(*vmax), d["L"].ptr, d["t"].ptr, d["inert"].ptr, d["q"].ptr,
+                   d["inv"].ptr, 600.0)
+    fglib.check(None, "gIntegrateRB")
+    rp, rv, rL, rq = orc.integrate_rb(pos, mass, vel, forces, dt, ext, vmin, vmax, Lm, tq, inv, q, 600.0)
+    assert np.array_equal(d["pos"].numpy(), rp)
+    assert np.array_equal(d["vel"].numpy(), rv)
+    assert np.array_equal(d["L"].numpy(), rL)
+    assert _ulp_diff(d["q"].numpy(), rq).max() <= 1
+
+
+def test_mass_properties_match_oracle(fglib, orc):
+    L = fglib.load()
+    sets = [scenes.torus_spheres(0.15, 0.05, 0.005) + [0.3, 0.6, 0.3], scenes.box_spheres(0.1, 0.0025) + [0.5, 0.5, 0.5]]
+    b = scenes.make_bodies(sets, [3.7, 0.7], 0.01, scenes.config_scene(0).common)
+    d = {k: _dev(fglib, v) for k, v in dict(w=b.world, o=b.offsets, m=b.mass).items()}
+    cm = fglib.DeviceArray((2, 3), np.float64)
+    inertia = fglib.DeviceArray((2, 9), np.float64)
+    fglib.check(L.fg_body_mass_properties_f64(d["w"].ptr, d["o"].ptr, 2, d["m"].ptr, cm.ptr, inertia.ptr), "props")
+    gcm, gI = cm.numpy(), inertia.numpy()
+    for k in range(2):
+        ocm, oI = orc.body_mass_props(b.world[b.offsets[k]:b.offsets[k + 1]], b.mass[k])
+        assert np.allclose(gcm[k], ocm, rtol=1e-12)
+        assert np.allclose(gI[k].reshape(3, 3), oI, rtol=1e-10, atol=1e-12 * np.abs(np.diag(oI)).max())
+
+
+def test_reference_reduction_helpers(fglib):
+    """gCopyRBDataToParticle, gReorderForceTorque and gAddForceTorque restate
+    K_RigidBodySim.cu:102-155."""
+    L = fglib.load()
+    f = np.array([[1, 2, 3, 1], [4, 5, 6, 0], [7, 8, 9, 1]], np.float32)
+    t = f * 10
+    d = {k: _dev(fglib, v) for k, v in dict(f=f, t=t).items()}
+    rb = fglib.DeviceArray((3,), np.uint32)
+    ix = fglib.DeviceArray((3,), np.uint32)
+    L.gCopyRBDataToParticle(rb.ptr, ix.ptr, d["f"].ptr, 3)
+    assert rb.numpy().tolist() == [1, 0, 1] and ix.numpy().tolist() == [0, 1, 2]
+    order = _dev(fglib, np.array([1, 0, 2], np.uint32))
+    of = fglib.DeviceArray((3, 4), np.float32)
+    ot = fglib.DeviceArray((3, 4), np.float32)
+    L.gReorderForceTorque(of.ptr, ot.ptr, d["f"].ptr, d["t"].ptr, order.ptr, 3)
+    assert np.array_equal(of.numpy(), f[[1, 0, 2]])
+    # segment sums already scanned: body 0 = row 0, body 1 = rows 1..2 (end at 2)
+    scanned_f = _dev(fglib, np.array([[4, 5, 6, 0], [1, 2, 3, 1], [8, 10, 12, 2]], np.float32))
+    scanned_t = _dev(fglib, np.array([[40, 50, 60, 0], [10, 20, 30, 10], [80, 100, 120, 20]], np.float32))
+    ordered = _dev(fglib, np.array([[0, 0, 0, 0], [0, 0, 0, 1], [0, 0, 0, 1]], np.float32))
+    acc_f = _dev(fglib, np.ones((2, 4), np.float32))
+    acc_t = _dev(fglib, np.zeros((2, 4), np.float32))
+    offs = _dev(fglib, np.array([0, 2], np.int32))
+    L.gAddForceTorque(scanned_f.ptr, scanned_t.ptr, ordered.ptr, acc_f.ptr, acc_t.ptr, offs.ptr, 2)
+    fglib.check(None, "reduction helpers")
+    assert np.array_equal(acc_f.numpy(), [[5, 6, 7, 1], [9, 11, 13, 3]])
+
+
+def test_coupled_simulator_runs_and_is_graph_stable(fglib):
+    """The config-1 torus scene coupled to a small fluid block: graph replay
+    equals eager stepping bit for bit, the contact sums are non-zero once the
+    torus reaches the fluid, and the state stays finite."""
+    s, b = _fluid_and_torus(inside=True)
+    res = []
+    for graph in (False, True):
+        sim = fglib.Simulator(s, fused=True, use_graph=graph)
+        try:
+            sim.add_bodies(b)
+            sim.step(10)
+            pos, vel = sim.state()
+            bs = sim.body_state()
+            sums = sim.body_sums()
+            res.append((pos, vel, bs, sums))
+        finally:
+            sim.close()
+    (p0, v0, b0, s0), (p1, v1, b1, s1) = res
+    assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
+    for k in b0:
+        assert np.array_equal(b0[k], b1[k]), k
+    assert np.all(np.isfinite(p0)) and np.all(np.isfinite(b0["pos"]))
+    assert np.linalg.norm(s0[0][0, :3]) > 0

@@ -1,0 +1,204 @@
+"""Pins of the oracle's rigid-body coupling functions.
+
+  penalty contact (K_DeviceFunctions.cu:32-159)  single fluid-particle/sphere
+        contact in closed form from both sides, and no force beyond the
+        contact distance
+  coupled fluid forces (K_FluidSim.cu:315-413)   external term added after
+        the mass factor: acc = g + F for an isolated particle
+  body torque (K_RigidBodySim.cu:94-97)         tau = (x - cm) x F exactly
+  per-body sums                                  numpy FP64 sums
+  body integration (K_RigidBodySim.cu:160-262)   free fall with the dt^2
+        factor in closed form; angular momentum recursion L <- 0.9 (L + tau dt);
+        unit quaternion and unchanged orientation without torque
+  body transform (K_HashGrid.cu:43-63)           conj(q) p q + cm equals a
+        rotation by q^-1 (numpy)
+  mass properties (FP64)                         numpy FP64 sums; thin-shell
+        torus I_axis = m(R^2 + 3r^2/2), I_diam = m(R^2/2 + 5r^2/4) and hollow
+        cube I = 5 m a^2 / 18 within the surface-voxel sampling error
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fluidgrid_amd import abi, scenes
+
+
+def _fluid_scene(pos, vel=None, h=0.02, g=0.0):
+    s = scenes.dam_break((1, 1, 1), h, (1.0, 1.0, 1.0), 1e-4)
+    s.pos = np.asarray(pos, np.float32)
+    s.vel = np.zeros_like(s.pos) if vel is None else np.asarray(vel, np.float32)
+    s.fluid.max_particles = len(s.pos)
+    s.common.global_external_force = abi.f3(0.0, g, 0.0)
+    return s
+
+
+def _grid(orc, s):
+    nb = s.num_buckets
+    key, pid = orc.hash_keys(s.pos, nb, tuple(s.common.cell_size), s.common, s.key_mode)
+    sk, sp = orc.sort_pairs(key, pid)
+    start, end, spos, svel = orc.cell_offsets(sk, sp, s.pos, s.vel, nb)
+    return sk, sp, start, end, spos, svel
+
+
+def _body_grid(orc, spheres, cell, nb=4096):
+    g = orc.grid_of_bodies(cell, nb)
+    key = np.array([orc.hash_keys(np.array([p], np.float32), nb, (cell,) * 3, abi.CommonFluidParams(),
+                                  abi.KEY_REF_XOR)[0][0] for p in spheres], np.uint32)
+    pid = np.arange(len(spheres), dtype=np.uint32)
+    sk, sp = orc.sort_pairs(key, pid)
+    start, end, spos, _ = orc.cell_offsets(sk, sp, spheres, None, nb)
+    return g, start, end, spos
+
+
+@pytest.mark.parametrize("dist", [0.004, 0.009, 0.02])
+def test_single_contact_fluid_side(orc, dist):
+    fr, br, eps = 0.002, 0.004, 0.003
+    ks, kd = -500.0, -20.0
+    vf = np.array([0.1, -0.3, 0.2])
+    vb = np.array([0.0, 0.5, 0.0])
+    s = _fluid_scene([[0.5, 0.5, 0.5, 1.0]], [[*vf, 0.0]])
+    sk, sp, start, end, spos, svel = _grid(orc, s)
+    rp = orc.density(spos, start, end, s.fluid, s.common, s.kernels, s.key_mode, f64=True)
+    sphere = np.array([[0.5 + dist, 0.5, 0.5, 0.0]], np.float32)
+    g, bstart, bend, bpos = _body_grid(orc, sphere, 0.012)
+    _, acc = orc.forces_coupled(spos, svel, rp, start, end, sp, s.fluid, s.common, s.kernels, s.key_mode, fr,
+                                (1, 1, 1), br, ks, kd, bpos, np.array([[*vb, 0.0]], np.float32), bstart, bend, g,
+                                f64=True, want_acc=True)
+    l = float(np.float32(0.5 + dist)) - float(np.float32(0.5))
+    if l < fr + br + eps:
+        n = np.array([1.0, 0.0, 0.0])
+        expect = (ks * l - kd * np.dot(n, vb - vf)) * n
+    else:
+        expect = np.zeros(3)
+    assert np.allclose(acc[0], expect, rtol=1e-6, atol=1e-12)
+
+
+def test_single_contact_body_side_and_torque(orc):
+    s = _fluid_scene([[0.5, 0.5, 0.5, 1.0]], [[0.2, 0.1, -0.1, 0.0]])
+    sk, sp, start, end, spos, svel = _grid(orc, s)
+    fg = orc.grid_of_fluid(s.common, s.key_mode)
+    sphere = np.array([[0.503, 0.502, 0.499, 0.0]], np.float32)
+    vb = np.array([[0.0, -0.4, 0.0, 0.0]], np.float32)
+    cm = np.array([[0.45, 0.55, 0.5, 0.0]], np.float32)
+    ks, kd = -300.0, -10.0
+    f, t = orc.body_forces_from_fluid(sphere, vb, cm, 0.004, spos, svel, start, end, 0.002, fg, (1, 1, 1), ks, kd)
+    d = spos[0, :3].astype(np.float64) - sphere[0, :3]
+    l = np.linalg.norm(d)
+    n = d / l
+    rv = svel[0, :3] - vb[0, :3]
+    expect = (ks * l - kd * np.dot(n, rv)) * n
+    assert np.allclose(f[0, :3], expect, rtol=1e-5)
+    assert f[0, 3] == 0.0 and t[0, 3] == 0.0
+    r = (sphere[0, :3] - cm[0, :3]).astype(np.float32)
+    F = f[0, :3]
+    exact = np.array([r[1] * F[2] - r[2] * F[1], r[2] * F[0] - r[0] * F[2], r[0] * F[1] - r[1] * F[0]], np.float32)
+    assert np.array_equal(t[0, :3], exact)
+
+
+def test_body_reduce_matches_numpy(orc):
+    rng = np.random.default_rng(3)
+    m, nb = 5000, 7
+    f = rng.normal(size=(m, 4)).astype(np.float32)
+    t = rng.normal(size=(m, 4)).astype(np.float32)
+    b = rng.integers(0, nb, m)
+    f[:, 3] = b
+    t[:, 3] = b
+    of, ot = orc.body_reduce_f64(f, t, nb)
+    for k in range(nb):
+        sel = b == k
+        assert np.allclose(of[k], f[sel, :3].astype(np.float64).sum(0), rtol=1e-12, atol=1e-12)
+        assert np.allclose(ot[k], t[sel, :3].astype(np.float64).sum(0), rtol=1e-12, atol=1e-12)
+
+
+def test_body_free_fall_with_dt_squared(orc):
+    m = 2.0
+    dt = np.float32(0.01)
+    ext = (0.0, -9.81 * m, 0.0, 0.0)
+    pos = np.array([[0.0, 10.0, 0.0, 0.0]], np.float32)
+    vel = np.zeros((1, 4), np.float32)
+    L = np.zeros((1, 4), np.float32)
+    q = np.array([[0, 0, 0, 1]], np.float32)
+    inv = np.tile(np.eye(3, 4, dtype=np.float32), (1, 1))
+    zero = np.zeros((1, 4), np.float32)
+    p0 = pos.copy()
+    steps = 50
+    for _ in range(steps):
+        pos, vel, L, q = orc.integrate_rb(pos, np.array([m], np.float32), vel, zero, float(dt), ext,
+                                          (-100, -100, -100), (100, 100, 100), L, zero, inv, q, 600.0)
+    a = -9.81 * float(dt) * float(dt)  # K_RigidBodySim.cu:185-186 applies dt twice
+    v = a * np.arange(1, steps + 1)
+    assert vel[0, 1] == pytest.approx(v[-1], rel=1e-4)
+    assert pos[0, 1] == pytest.approx(p0[0, 1] + float(dt) * v.sum(), rel=1e-6)
+    assert np.array_equal(q[0], [0, 0, 0, 1])
+    assert np.all(L == 0)
+
+
+def test_angular_momentum_recursion_and_unit_quaternion(orc):
+    dt = 0.02
+    tau = np.array([[0.3, -0.2, 0.5, 0.0]], np.float32)
+    inv = np.eye(3, 4, dtype=np.float32) * 0.5
+    q = np.array([[0.1, 0.2, 0.3, 0.9]], np.float32)
+    q /= np.linalg.norm(q)
+    L = np.zeros((1, 4), np.float32)
+    Lref = np.zeros(3)
+    pos = np.zeros((1, 4), np.float32)
+    vel = np.zeros((1, 4), np.float32)
+    for _ in range(30):
+        pos, vel, L, q = orc.integrate_rb(pos, np.array([1.0], np.float32), vel, np.zeros((1, 4), np.float32), dt,
+                                          (0, 0, 0, 0), (-1, -1, -1), (1, 1, 1), L, tau, inv, q, 600.0)
+        Lref = 0.9 * (Lref + tau[0, :3] * dt)
+        assert abs(np.linalg.norm(q[0]) - 1.0) < 1e-6
+    assert np.allclose(L[0, :3], Lref, rtol=1e-5)
+
+
+def test_body_transform_is_inverse_rotation(orc):
+    rng = np.random.default_rng(9)
+    rel = np.zeros((200, 4), np.float32)
+    rel[:, :3] = rng.normal(size=(200, 3))
+    rel[:, 3] = rng.integers(0, 3, 200)
+    q = rng.normal(size=(3, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q = q.astype(np.float32)
+    cm = rng.normal(size=(3, 4)).astype(np.float32)
+    world, key, pid = orc.body_transform(rel, cm, q, orc.grid_of_bodies(0.1, 1000))
+    for i in range(200):
+        b = int(rel[i, 3])
+        qi = q[b].astype(np.float64) * np.array([-1, -1, -1, 1])  # q^-1
+        expect = scenes.quat_rotate(qi, rel[i:i + 1, :3])[0] + cm[b, :3]
+        assert np.allclose(world[i, :3], expect, atol=1e-5)
+        assert world[i, 3] == rel[i, 3]
+    assert key.max() < 1000 and np.array_equal(pid, np.arange(200))
+
+
+def test_mass_properties_torus_and_box(orc):
+    R, r, d = 0.15, 0.05, 0.0025
+    pts = scenes.torus_spheres(R, r, d) + np.array([0.3, 0.4, 0.5])
+    world = np.concatenate([pts, np.zeros((len(pts), 1))], 1).astype(np.float32)
+    m = 3.7
+    cm, inertia = orc.body_mass_props(world, m)
+    # exact FP64 sums of the same point set
+    w = world[:, :3].astype(np.float64)
+    c = w.mean(0)
+    rr = w - c
+    I = (m / len(w)) * ((rr * rr).sum() * np.eye(3) - rr.T @ rr)
+    assert np.allclose(cm, c, rtol=1e-12) and np.allclose(inertia, I, rtol=1e-10, atol=1e-14)
+    assert np.allclose(cm, [0.3, 0.4, 0.5], atol=1e-5)
+    # thin-shell torus, axis y
+    assert inertia[1, 1] == pytest.approx(m * (R * R + 1.5 * r * r), rel=0.03)
+    assert inertia[0, 0] == pytest.approx(m * (0.5 * R * R + 1.25 * r * r), rel=0.03)
+    assert abs(inertia[0, 1]) < 1e-3 * inertia[0, 0]
+    a = 0.1
+    pts = scenes.box_spheres(a, a / 40)
+    world = np.concatenate([pts, np.zeros((len(pts), 1))], 1).astype(np.float32)
+    cm, inertia = orc.body_mass_props(world, 0.7)
+    assert np.allclose(np.diag(inertia), 5.0 * 0.7 * a * a / 18.0, rtol=0.05)
+
+
+def test_torus_scene_sphere_count_and_mass():
+    s = scenes.coupled_config_scene(1)
+    b = s.info["bodies"]
+    assert 18000 < b.m < 22000  # "about 20K boundary spheres"
+    assert b.mass[0] == pytest.approx(500 * 2 * math.pi ** 2 * 0.15 * 0.05 ** 2, rel=1e-12)
+    top = s.pos[:, 1].max()
+    assert b.world[:, 1].min() == pytest.approx(top + 0.3, abs=0.01)
