@@ -1,19 +1,26 @@
 """Throughput of the SPH step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--fluid-only] [--no-cpu-baseline]
 
-A step is one pass of the per-timestep pipeline (hash -> stable radix sort
--> cell ranges + SoA reorder -> density -> forces + walls + integrate) over
-every particle of a synthetic dam-break scene, driven through the C++
-HeterogeneousSimulator and replayed as a captured HIP graph.  Inputs are
-resident in HBM before the timed region.  `value` is particle-steps/s.
+A step is one pass of the per-timestep pipeline over every particle of a
+synthetic dam-break scene, driven through the C++ HeterogeneousSimulator and
+replayed as a captured HIP graph: hash -> stable radix sort -> cell ranges +
+SoA reorder -> density (+ neighbour masks) -> [bodies: world transform, body
+grid, forces from the fluid] -> fluid forces + contact + walls + integrate ->
+[bodies: FP64 per-body sums, body integration].  Inputs are resident in HBM
+before the timed region.  `value` is particle-steps/s.
+
+The default workload is the 1,048,576-particle dam break coupled two-way to a
+rigid torus (R 0.15 m, r 0.05 m, ~20K boundary spheres, density 500) dropped
+from 0.3 m above the fluid; the second line is the plain FP32 8,388,608-
+particle fluid step.
 
 Roofline: algorithmic bytes per particle (hash 24, one sort pass 16, cells +
 reorder 72, density 24, forces 60, integrate 48) plus 4 B per bucket for the
-cellStart fill and 24 B per occupied cell (start/end write, two reads).  The
-dominant kernel's bytes are divided by its average duration, measured with
-HIP events on the simulator's stream around each stage over a second, eager
-pass of the same steps.
+cellStart fill, 24 B per occupied cell (start/end write, two reads) and 176 B
+per body sphere.  The dominant kernel's bytes are divided by its average
+duration, measured with HIP events on the simulator's stream around each
+stage over a second, eager pass of the same steps.
 
 cpu_baseline: the oracle (a C++/OpenMP restatement of the reference step;
 the reference itself has no CPU SPH path) timed on this host's cores on a
@@ -48,7 +55,9 @@ def occupied_cells(sim):
     return int(np.count_nonzero(start != 0xFFFFFFFF))
 
 
-def stage_bytes(stage, n, c_occ, fused):
+def stage_bytes(stage, n, c_occ, fused, m=0):
+    if stage == "bodies":
+        return 176 * m
     by = STAGE_BYTES[stage] * n + OCC_BYTES.get(stage, 0) * c_occ
     if stage == "forces" and fused:
         by += STAGE_BYTES["integrate"] * n
@@ -57,7 +66,10 @@ def stage_bytes(stage, n, c_occ, fused):
 
 def measure(scene, steps, warmup, fused=True):
     sim = lib.Simulator(scene, fused=fused, use_graph=True)
+    bodies = scene.info.get("bodies")
     try:
+        if bodies is not None:
+            sim.add_bodies(bodies)
         sim.step(warmup)
         sim.synchronize()
         t0 = time.perf_counter()
@@ -76,26 +88,47 @@ def measure(scene, steps, warmup, fused=True):
         stage_ms = {k: v / nprof for k, v in acc.items()}
         pos, _ = sim.state()
         finite = bool(np.all(np.isfinite(pos)))
+        body = None
+        if bodies is not None:
+            bs = sim.body_state()
+            finite = finite and bool(np.all(np.isfinite(bs["pos"])))
+            body = {"centre_y_m": [round(float(v), 6) for v in bs["pos"][:, 1]]}
     finally:
         sim.close()
-    return dict(seconds=dt, ms_per_step=1e3 * dt / steps, c_occ=c_occ, stage_ms=stage_ms, finite=finite)
+    return dict(seconds=dt, ms_per_step=1e3 * dt / steps, c_occ=c_occ, stage_ms=stage_ms, finite=finite, body=body)
+
+
+def step_bytes(scene, c_occ):
+    bodies = scene.info.get("bodies")
+    m = bodies.m if bodies is not None else 0
+    return 244 * scene.n + 4 * scene.num_buckets + 24 * c_occ + 176 * m
 
 
 def cpu_baseline(scene, seconds_budget=15.0):
     from oracle import oracle
     threads = oracle.num_threads()
-    pos, vel = scene.pos.copy(), scene.vel.copy()
-    rho_p = np.zeros((scene.n, 2), np.float32)
+    bodies = scene.info.get("bodies")
     steps, t_total = 0, 0.0
-    while t_total < seconds_budget and steps < 50:
-        t0 = time.perf_counter()
-        st = oracle.step(pos, vel, scene.fluid, scene.common, scene.kernels, scene.key_mode, rho_p_prev=rho_p)
-        t_total += time.perf_counter() - t0
-        pos, vel, rho_p = st.pos, st.vel, st.rho_p
-        steps += 1
+    if bodies is not None:
+        st = oracle.CoupledState(scene, bodies)
+        while t_total < seconds_budget and steps < 50:
+            t0 = time.perf_counter()
+            st = oracle.coupled_step(st)
+            t_total += time.perf_counter() - t0
+            steps += 1
+        what = f"coupled oracle steps of the {scene.n:,}-particle scene with {bodies.m:,} body spheres"
+    else:
+        pos, vel = scene.pos.copy(), scene.vel.copy()
+        rho_p = np.zeros((scene.n, 2), np.float32)
+        while t_total < seconds_budget and steps < 50:
+            t0 = time.perf_counter()
+            st = oracle.step(pos, vel, scene.fluid, scene.common, scene.kernels, scene.key_mode, rho_p_prev=rho_p)
+            t_total += time.perf_counter() - t0
+            pos, vel, rho_p = st.pos, st.vel, st.rho_p
+            steps += 1
+        what = f"oracle steps of the {scene.n:,}-particle scene"
     return dict(value=scene.n * steps / t_total, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"{steps} oracle steps of the {scene.n:,}-particle scene ({t_total:.1f} s, "
-                       f"{threads} OpenMP threads)")
+                sample=f"{steps} {what} ({t_total:.1f} s, {threads} OpenMP threads)")
 
 
 def main():
@@ -107,6 +140,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-8m", action="store_true")
     ap.add_argument("--unfused", action="store_true")
+    ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB ten times (reference for HBM byte counters)")
     a = ap.parse_args()
@@ -129,24 +163,26 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nbytes}))
         return
 
-    scene = scenes.config_scene(a.config)
+    coupled = a.config in (1, 2) and not a.fluid_only
+    scene = scenes.coupled_config_scene(a.config) if coupled else scenes.config_scene(a.config)
+    bodies = scene.info.get("bodies")
     fused = not a.unfused
-    log(f"scene {scene.name}: n={scene.n:,} buckets={scene.num_buckets:,}")
+    log(f"scene {scene.name}: n={scene.n:,} buckets={scene.num_buckets:,} "
+        f"body spheres={bodies.m if bodies is not None else 0:,}")
     r = measure(scene, a.steps, a.warmup, fused)
     n = scene.n
     value = n * a.steps / r["seconds"]
     stage_ms = r["stage_ms"]
     dom = max((k for k in stage_ms if stage_ms[k] > 0), key=lambda k: stage_ms[k])
-    dom_bytes = stage_bytes(dom, n, r["c_occ"], fused)
+    dom_bytes = stage_bytes(dom, n, r["c_occ"], fused, bodies.m if bodies is not None else 0)
     achieved = dom_bytes / (stage_ms[dom] * 1e-3)
-    step_bytes = 244 * n + 4 * scene.num_buckets + 24 * r["c_occ"]
-    step_gbs = step_bytes / (r["ms_per_step"] * 1e-3)
+    step_gbs = step_bytes(scene, r["c_occ"]) / (r["ms_per_step"] * 1e-3)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
             t = json.load(open(tpath))
-            traffic = t.get(f"cfg{a.config}", {}).get(dom)
+            traffic = t.get(scene.name, {}).get(dom)
         except Exception:
             traffic = None
     out = {
@@ -163,9 +199,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"plain FP32 dam break, config {a.config}: {n:,} particles, "
-                        f"{scene.num_buckets:,} LINEAR buckets, h={scene.info['h']}, dt={scene.info['dt']}, "
-                        "fluid only (no rigid body)",
+            "workload": f"plain FP32 dam break: {n:,} particles, {scene.num_buckets:,} LINEAR buckets, "
+                        f"h={scene.info['h']}, dt={scene.info['dt']}, "
+                        + (f"coupled two-way with {bodies.b} rigid bod{'y' if bodies.b == 1 else 'ies'} "
+                           f"({', '.join(bodies.names)}; {bodies.m:,} boundary spheres of radius 0.5h)"
+                           if bodies is not None else "fluid only (no rigid body)"),
             "particles": n,
             "buckets": scene.num_buckets,
             "occupied_cells": r["c_occ"],
@@ -174,10 +212,12 @@ def main():
             "step_algorithmic_GBps": step_gbs / 1e9,
             "step_hbm_frac": step_gbs / HBM_PEAK,
             "finite": r["finite"],
+            "bodies": r["body"],
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dom if not (dom == "forces" and fused) else "forces+integrate (fused)",
+            "kernel": (dom if not (dom == "forces" and fused) else
+                       ("coupled " if bodies is not None else "") + "forces+integrate (fused)"),
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
@@ -192,9 +232,9 @@ def main():
         k8 = max(20, a.steps // 4)
         r8 = measure(s8, k8, max(5, a.warmup // 2), fused)
         out["config"]["also_8M"] = {
+            "workload": f"plain FP32 dam break, {s8.n:,} particles, fluid only",
             "particles": s8.n, "steps": k8, "value": s8.n * k8 / r8["seconds"], "ms_per_step": r8["ms_per_step"],
-            "step_hbm_frac": (244 * s8.n + 4 * s8.num_buckets + 24 * r8["c_occ"]) / (r8["ms_per_step"] * 1e-3)
-            / HBM_PEAK,
+            "step_hbm_frac": step_bytes(s8, r8["c_occ"]) / (r8["ms_per_step"] * 1e-3) / HBM_PEAK,
             "stage_ms": {k: round(v, 5) for k, v in r8["stage_ms"].items()},
         }
     if not a.no_cpu_baseline:

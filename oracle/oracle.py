@@ -305,3 +305,85 @@ def body_mass_props(world, mass):
     inertia = np.zeros(9, np.float64)
     _rl().orc_body_mass_props_f64(world, world.shape[0], float(mass), cm, inertia)
     return cm, inertia.reshape(3, 3)
+
+
+class CoupledState:
+    """Fluid plus dynamic bodies, advanced by coupled_step.
+
+    Body set-up restates RigidBodySimulator::reset of the HIP library: FP64
+    centre of mass and inertia of equal point masses at the sphere centres,
+    sphere offsets from the centre of mass rounded to float, FP32 inverse
+    inertia from the FP64 inverse, identity orientation, zero velocity and
+    angular momentum."""
+
+    def __init__(self, scene, bodies):
+        self.scene = scene
+        self.bodies = bodies
+        self.pos, self.vel = scene.pos.copy(), scene.vel.copy()
+        self.rho_p = np.zeros((scene.n, 2), np.float32)
+        nb = bodies.b
+        cm = np.zeros((nb, 4), np.float32)
+        inv = np.zeros((3 * nb, 4), np.float32)
+        rel = np.zeros_like(bodies.world)
+        for b in range(nb):
+            lo, hi = int(bodies.offsets[b]), int(bodies.offsets[b + 1])
+            c, inertia = body_mass_props(bodies.world[lo:hi], bodies.mass[b])
+            cm[b, :3] = c.astype(np.float32)
+            inv[3 * b:3 * b + 3, :3] = np.linalg.inv(inertia).astype(np.float32)
+            rel[lo:hi, :3] = (bodies.world[lo:hi, :3].astype(np.float64) - c).astype(np.float32)
+            rel[lo:hi, 3] = b
+        self.rel, self.cm, self.inv = rel, cm, inv
+        self.bvel = np.zeros((nb, 4), np.float32)
+        self.orient = np.tile(np.array([0, 0, 0, 1], np.float32), (nb, 1))
+        self.L = np.zeros((nb, 4), np.float32)
+        self.mass32 = bodies.mass.astype(np.float32)
+        self.sum_f = np.zeros((nb, 3))
+        self.sum_t = np.zeros((nb, 3))
+
+
+def coupled_step(st):
+    """One FP32 coupled step in the stage order of the HIP simulator
+    (HeterogeneousSimulator.cpp:54-127 with the coupled branch of :174-186
+    and RigidBodySimulator.cpp:451-611): fluid grid and density; body world
+    transform, grid and forces from the fluid; coupled fluid forces and
+    integration; per-body FP64 sums and body integration."""
+    s, b = st.scene, st.bodies
+    common, fluid, kern, mode = s.common, s.fluid, s.kernels, s.key_mode
+    nb = common.num_buckets()
+    key, pid = hash_keys(st.pos, nb, tuple(common.cell_size), common, mode)
+    skey, spid = sort_pairs(key, pid)
+    start, end, spos, svel = cell_offsets(skey, spid, st.pos, st.vel, nb)
+    rho_p = density(spos, start, end, fluid, common, kern, mode, rho_p_init=st.rho_p)
+    cell = b.rb.particleRadius
+    nbb = b.rb.dn_bucket_size.x * b.rb.dn_bucket_size.y * b.rb.dn_bucket_size.z
+    bgrid = grid_of_bodies(cell, nbb)
+    world, bkey, bpid = body_transform(st.rel, st.cm, st.orient, bgrid)
+    bskey, bspid = sort_pairs(bkey, bpid)
+    bstart, bend, bpos, _ = cell_offsets(bskey, bspid, world, None, nbb)
+    fluid_r = float(np.float32(2.0 * np.float32(common.pointRadius)))
+    ks, kd = float(np.float32(-b.spring)), float(np.float32(-b.damping))
+    f, t = body_forces_from_fluid(world, st.bvel, st.cm, b.sphere_radius, spos, svel, start, end, fluid_r,
+                                  grid_of_fluid(common, mode), tuple(b.rb.rb_to_fluid_relation), ks, kd)
+    v_new, _ = forces_coupled(spos, svel, rho_p, start, end, spid, fluid, common, kern, mode, fluid_r,
+                              tuple(common.fluid_to_srb_relation), b.sphere_radius, ks, kd, bpos, st.bvel, bstart,
+                              bend, bgrid, v_out_init=st.vel)
+    st.pos = integrate(st.pos, v_new, common)
+    st.vel, st.rho_p = v_new, rho_p
+    sum_f = np.zeros((b.b, 3))
+    sum_t = np.zeros((b.b, 3))
+    for k in range(b.b):
+        lo, hi = int(b.offsets[k]), int(b.offsets[k + 1])
+        sum_f[k] = f[lo:hi, :3].astype(np.float64).sum(0)
+        sum_t[k] = t[lo:hi, :3].astype(np.float64).sum(0)
+    st.sum_f, st.sum_t = sum_f, sum_t
+    F = np.zeros((b.b, 4), np.float32)
+    T = np.zeros((b.b, 4), np.float32)
+    F[:, :3] = sum_f.astype(np.float32)
+    T[:, :3] = sum_t.astype(np.float32)
+    g = b.rb.global_external_force
+    rb = b.rb
+    st.cm, st.bvel, st.L, st.orient = integrate_rb(st.cm, st.mass32, st.bvel, F, rb.deltaTimeIntegration,
+                                                   (g.x, g.y, g.z, g.w), tuple(rb.volume_min),
+                                                   tuple(rb.volume_max), st.L, T, st.inv, st.orient,
+                                                   rb.velocity_limit)
+    return st

@@ -12,6 +12,11 @@
         unit quaternion and unchanged orientation without torque
   body transform (K_HashGrid.cu:43-63)           conj(q) p q + cm equals a
         rotation by q^-1 (numpy)
+  both coupling directions                       O(N*M) FP64 brute force over
+        all fluid/sphere pairs (margins 0.003 fluid side, 0.0001 body side),
+        and equal-and-opposite pair forces on the common pairs
+  coupled step                                   dt^2 free fall of a body
+        above the fluid; upward contact force once it touches
   mass properties (FP64)                         numpy FP64 sums; thin-shell
         torus I_axis = m(R^2 + 3r^2/2), I_diam = m(R^2/2 + 5r^2/4) and hollow
         cube I = 5 m a^2 / 18 within the surface-voxel sampling error
@@ -202,3 +207,85 @@ def test_torus_scene_sphere_count_and_mass():
     assert b.mass[0] == pytest.approx(500 * 2 * math.pi ** 2 * 0.15 * 0.05 ** 2, rel=1e-12)
     top = s.pos[:, 1].max()
     assert b.world[:, 1].min() == pytest.approx(top + 0.3, abs=0.01)
+
+
+def _submerged_torus(h=0.02):
+    s = scenes.dam_break((24, 14, 24), h, (1.0, 1.0, 1.0), 1e-4)
+    s.vel[:, :3] = np.random.default_rng(12).normal(0, 0.2, (s.n, 3)).astype(np.float32)
+    top = s.pos[:, 1].max()
+    pts = scenes.torus_spheres(0.06, 0.02, 0.5 * h) + np.array([0.12, top - 0.03, 0.12])
+    b = scenes.make_bodies([pts], [1.0], h, s.common)
+    return s, b
+
+
+def test_contact_forces_brute_force_and_antisymmetry(orc):
+    """Both coupling directions against an O(N*M) FP64 brute force over all
+    fluid/sphere pairs: the fluid term (K_FluidSim.cu:379-386, contact within
+    r_f + r_b + 0.003) and the sphere force (K_RigidBodySim.cu:84-87, within
+    r_f + r_b + 0.0001; the reference uses the two different margins).  On
+    the common pairs the two terms are equal and opposite, so the body sum is
+    minus the fluid pair sum restricted to the body-side distance."""
+    s, b = _submerged_torus()
+    sk, sp, start, end, spos, svel = _grid(orc, s)
+    rp = orc.density(spos, start, end, s.fluid, s.common, s.kernels, s.key_mode, f64=True)
+    cell = b.rb.particleRadius
+    nbb = 64 ** 3
+    g = orc.grid_of_bodies(cell, nbb)
+    key, pid = orc.hash_keys(b.world, nbb, (cell,) * 3, s.common, abi.KEY_REF_XOR)
+    bsk, bsp = orc.sort_pairs(key, pid)
+    bstart, bend, bpos, _ = orc.cell_offsets(bsk, bsp, b.world, None, nbb)
+    bvel = np.array([[0.1, -0.2, 0.05, 0.0]], np.float32)
+    ks, kd = -b.spring, -b.damping
+    common = (start, end, sp, s.fluid, s.common, s.kernels, s.key_mode)
+    _, acc_c = orc.forces_coupled(spos, svel, rp, *common, b.fluid_radius, (1, 1, 1), b.sphere_radius, ks, kd, bpos,
+                                  bvel, bstart, bend, g, f64=True, want_acc=True)
+    _, acc_u, _ = orc.forces(spos, svel, rp, *common, f64=True, want_acc=True)
+    ext = acc_c - acc_u
+    cm = np.array([[0.12, 0.2, 0.12, 0.0]], np.float32)
+    f, _ = orc.body_forces_from_fluid(b.world, bvel, cm, b.sphere_radius, spos, svel, start, end, b.fluid_radius,
+                                      orc.grid_of_fluid(s.common, s.key_mode), (1, 1, 1), ks, kd)
+    X = spos[:, :3].astype(np.float64)
+    Y = b.world[:, :3].astype(np.float64)
+    d = np.linalg.norm(X[:, None, :] - Y[None, :, :], axis=2)
+    r0 = np.float32(b.fluid_radius) + np.float32(b.sphere_radius)
+
+    def pair_forces(eps):
+        i, j = np.nonzero(d < float(r0 + np.float32(eps)))
+        r = Y[j] - X[i]
+        n = r / np.linalg.norm(r, axis=1)[:, None]
+        dv = bvel[0, :3].astype(np.float64) - svel[i, :3]
+        return i, j, (ks * np.linalg.norm(r, axis=1) - kd * np.einsum("ij,ij->i", n, dv))[:, None] * n
+
+    i, j, ff = pair_forces(0.003)
+    want = np.zeros_like(X)
+    np.add.at(want, i, ff)
+    assert len(i) > 1000
+    assert np.allclose(ext, want, rtol=1e-5, atol=1e-5 * np.abs(ff).max())
+    i, j, ff = pair_forces(0.0001)
+    want_b = np.zeros_like(Y)
+    np.add.at(want_b, j, -ff)
+    assert 0 < len(i) < len(pair_forces(0.003)[0])
+    assert np.allclose(f[:, :3], want_b, rtol=1e-4, atol=1e-4 * np.abs(ff).max())
+    body_sum = f[:, :3].astype(np.float64).sum(0)
+    assert np.allclose(body_sum, -ff.sum(0), rtol=0, atol=1e-5 * np.abs(ff).sum())
+
+
+def test_coupled_step_free_fall_then_contact(orc):
+    """Bodies above the fluid fall with the dt^2 law until they touch it; the
+    contact then pushes them up (the summed vertical force is positive)."""
+    s = scenes.dam_break((20, 10, 20), 0.02, (1.0, 1.0, 1.0), 2e-4)
+    top = s.pos[:, 1].max()
+    pts = scenes.torus_spheres(0.05, 0.02, 0.01) + np.array([0.1, top + 0.05, 0.1])
+    b = scenes.make_bodies([pts], [1.0], 0.02, s.common)
+    st = orc.CoupledState(s, b)
+    y0 = st.cm[0, 1]
+    dt = float(np.float32(2e-4))
+    st = orc.coupled_step(st)
+    assert np.all(st.sum_f == 0)
+    assert st.bvel[0, 1] == pytest.approx(-9.81 * dt * dt, rel=1e-5)
+    assert st.cm[0, 1] == pytest.approx(y0 + dt * st.bvel[0, 1], abs=1e-7)
+    # push the torus into the surface and step once
+    st.cm[0, 1] = top + 0.015
+    st = orc.coupled_step(st)
+    assert st.sum_f[0, 1] > 0
+    assert np.all(np.isfinite(st.pos)) and np.all(np.isfinite(st.cm))
