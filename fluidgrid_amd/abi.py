@@ -107,6 +107,14 @@ class BodyGrid(C.Structure):
                 ("sphereRadius", C.c_float), ("kSpring", C.c_float), ("kDamping", C.c_float)]
 
 
+class SceneDesc(C.Structure):
+    """fg_scene_desc (include/fluidgrid_amd.h): one scene of a batch."""
+    _fields_ = [("common", CommonFluidParams), ("kernels", KernelsCache), ("mass", C.c_float),
+                ("gasConstant", C.c_float), ("restDensity", C.c_float), ("viscosity", C.c_float),
+                ("keyBase", C.c_uint), ("particleBegin", C.c_uint), ("particleCount", C.c_uint),
+                ("reserved", C.c_uint * 4)]
+
+
 KEY_REF_XOR = 0
 KEY_LINEAR = 1
 MATH_FAST = 0
@@ -116,6 +124,7 @@ assert C.sizeof(CommonFluidParams) == 104
 assert C.sizeof(FluidParams) == 28
 assert C.sizeof(KernelsCache) == 12
 assert C.sizeof(CommonRigidBodyParams) == 112
+assert C.sizeof(SceneDesc) == 160
 
 
 def f3(x, y, z):

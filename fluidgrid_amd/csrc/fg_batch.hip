@@ -1,0 +1,161 @@
+// fg_batch.hip -- many independent fluids advanced with one launch per stage.
+//
+// Particles are scene-major and every scene owns a LINEAR grid whose buckets
+// sit at [keyBase, keyBase + buckets) of one shared key space, so the single
+// stable sort and the single cell-range pass of the one-fluid path serve all
+// scenes.  Each lane handles one particle; a wavefront whose particles belong
+// to several scenes runs once per distinct scene (readfirstlane loop), so the
+// scene's parameters are wave-uniform and read through the scalar cache.
+#include "fg_internal.hpp"
+#include "fg_sph_fast.hpp"
+
+namespace fg {
+
+// Calls f(scene) with a wave-uniform scene index for every lane whose scene
+// is below nscenes.
+template <class F>
+__device__ __forceinline__ void for_each_scene(uint32_t sc, uint32_t nscenes, F&& f) {
+    if (sc >= nscenes) return;
+    for (;;) {
+        const uint32_t s0 = __builtin_amdgcn_readfirstlane(sc);
+        if (sc == s0) {
+            f(s0);
+            return;
+        }
+    }
+}
+
+__device__ __forceinline__ GridDesc scene_grid(const fg_common_fluid_params& cp) {
+    GridDesc g;
+    g.vmin = make_float3(cp.volume_min.x, cp.volume_min.y, cp.volume_min.z);
+    g.cs = make_float3(cp.cell_size.x, cp.cell_size.y, cp.cell_size.z);
+    g.dims = make_int3(cp.bucket_size.x, cp.bucket_size.y, cp.bucket_size.z);
+    g.nb = dev_num_buckets(cp);
+    g.mode = FG_KEY_LINEAR;
+    return g;
+}
+
+__global__ void __launch_bounds__(256) k_batch_hash(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes,
+                                                    const uint32_t* __restrict__ scene_of,
+                                                    const float4* __restrict__ pos, uint32_t* __restrict__ key,
+                                                    uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t sc = scene_of[i];
+    if (sc >= nscenes) {
+        key[i] = kEmpty;  // sorted last; skipped by the cell-range pass
+        return;
+    }
+    const float4 p = pos[i];
+    for_each_scene(sc, nscenes, [&](uint32_t s) {
+        const GridDesc g = scene_grid(scenes[s].common);
+        const CellXYZ c = grid_cell(p.x, p.y, p.z, g);
+        key[i] = scenes[s].keyBase + grid_linear_key(c.x, c.y, c.z, g);
+    });
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS)
+    k_batch_density(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes, const uint32_t* __restrict__ scene_of,
+                    const float4* __restrict__ sp, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
+                    const uint32_t* __restrict__ end, unsigned long long* __restrict__ masks, uint32_t n) {
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (i >= n) return;
+    for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
+        const fg_scene_desc& d = scenes[s];
+        density_particle<true>(i, sp, rho_p, start, end, n, d.gasConstant, d.restDensity, d.mass, d.common,
+                               d.kernels, d.keyBase, masks);
+    });
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS)
+    k_batch_forces(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes, const uint32_t* __restrict__ scene_of,
+                   const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
+                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                   const unsigned long long* __restrict__ masks, float4* __restrict__ vout,
+                   float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n) {
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (i >= n) return;
+    const Contacts none{};
+    for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
+        const fg_scene_desc& d = scenes[s];
+        forces_particle<true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, d.mass, d.viscosity, d.common,
+                              d.kernels, d.keyBase, masks, none);
+    });
+}
+
+#define FG_BATCH_LAUNCH(KERN, n, ...)                                                                       \
+    do {                                                                                                    \
+        switch (state().sph_block) {                                                                        \
+            case 64:                                                                                        \
+                hipLaunchKernelGGL(KERN<64>, dim3(grid_for(n, 64)), dim3(64), 0, state().stream, __VA_ARGS__); \
+                break;                                                                                      \
+            case 128:                                                                                       \
+                hipLaunchKernelGGL(KERN<128>, dim3(grid_for(n, 128)), dim3(128), 0, state().stream,         \
+                                   __VA_ARGS__);                                                            \
+                break;                                                                                      \
+            case 512:                                                                                       \
+                hipLaunchKernelGGL(KERN<512>, dim3(grid_for(n, 512)), dim3(512), 0, state().stream,         \
+                                   __VA_ARGS__);                                                            \
+                break;                                                                                      \
+            default:                                                                                        \
+                hipLaunchKernelGGL(KERN<256>, dim3(grid_for(n, 256)), dim3(256), 0, state().stream,         \
+                                   __VA_ARGS__);                                                            \
+        }                                                                                                   \
+    } while (0)
+
+}  // namespace fg
+
+using namespace fg;
+
+extern "C" {
+
+int fg_batch_hash(const fg_scene_desc* scenes, unsigned int numScenes, const unsigned int* sceneOf,
+                  const fg_float4* positions, unsigned int* hashKey, unsigned int numParticles) {
+    if (numParticles == 0) return FG_OK;
+    if (!scenes || !sceneOf || !positions || !hashKey || numScenes == 0) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_batch_hash: null pointer or no scenes");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    hipLaunchKernelGGL(k_batch_hash, dim3(grid_for(numParticles, 256)), dim3(256), 0, state().stream, scenes,
+                       numScenes, sceneOf, reinterpret_cast<const float4*>(positions), hashKey, numParticles);
+    return check_launch("fg_batch_hash") ? FG_OK : fg_peek_error();
+}
+
+int fg_batch_density_nbr(const fg_scene_desc* scenes, unsigned int numScenes, const unsigned int* sceneOf,
+                         fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,
+                         unsigned int* gridEnd, unsigned long long* neighborMasks, unsigned int numParticles) {
+    if (numParticles == 0) return FG_OK;
+    if (!scenes || !sceneOf || !orderedPositions || !mass_pressure || !gridStart || !gridEnd || !neighborMasks ||
+        numScenes == 0) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_batch_density_nbr: null pointer or no scenes");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    FG_BATCH_LAUNCH(k_batch_density, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
+                    FG_F2(mass_pressure), gridStart, gridEnd, neighborMasks, numParticles);
+    return check_launch("fg_batch_density_nbr") ? FG_OK : fg_peek_error();
+}
+
+int fg_batch_forces_integrate_nbr(const fg_scene_desc* scenes, unsigned int numScenes, const unsigned int* sceneOf,
+                                  fg_float4* orderedPositions, fg_float4* orderedVelocity,
+                                  fg_float2* density_pressure, unsigned int* gridStart, unsigned int* gridEnd,
+                                  const unsigned long long* neighborMasks, fg_float4* vel, fg_float4* pos,
+                                  unsigned int* particleId, unsigned int numParticles) {
+    if (numParticles == 0) return FG_OK;
+    if (!scenes || !sceneOf || !orderedPositions || !orderedVelocity || !density_pressure || !gridStart ||
+        !gridEnd || !neighborMasks || !vel || !particleId || numScenes == 0) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_batch_forces_integrate_nbr: null pointer or no scenes");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    if ((void*)vel == (void*)orderedVelocity) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_batch_forces_integrate_nbr: vel must not alias orderedVelocity");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    FG_BATCH_LAUNCH(k_batch_forces, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
+                    FG_F4(orderedVelocity), FG_F2(density_pressure), gridStart, gridEnd, neighborMasks, FG_F4(vel),
+                    pos ? FG_F4(pos) : nullptr, particleId, numParticles);
+    return check_launch("fg_batch_forces_integrate_nbr") ? FG_OK : fg_peek_error();
+}
+
+}  // extern "C"

@@ -27,6 +27,10 @@ struct State {
     int key_mode = FG_KEY_REF_XOR;
     int math_mode = FG_MATH_FAST;
     hipStream_t stream = nullptr;
+    // Launch shape chosen by the tuner (fg_set_tuning): workgroup size of the
+    // fast density/force kernels, and digit bits per radix-sort pass.
+    int sph_block = 256;
+    int sort_digit_bits = 8;
 };
 
 State& state();
@@ -173,6 +177,7 @@ static_assert(sizeof(fg_common_fluid_params) == 104, "CommonFluidParams layout")
 static_assert(sizeof(fg_common_rigid_body_params) == 112, "CommonRigidBodyParams layout");
 static_assert(sizeof(fg_kernels_cache) == 12, "KernelsCache layout");
 static_assert(sizeof(fg_fluid_params) == 28, "FluidParams layout");
+static_assert(sizeof(fg_scene_desc) == 160, "scene descriptor layout");
 
 #define FG_F4(p) reinterpret_cast<float4*>(p)
 #define FG_F2(p) reinterpret_cast<float2*>(p)

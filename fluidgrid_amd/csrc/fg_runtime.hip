@@ -116,6 +116,26 @@ int fg_set_math_mode(int mode) {
 
 int fg_get_math_mode(void) { return state().math_mode; }
 
+int fg_set_tuning(int sphBlock, int sortDigitBits) {
+    if (sphBlock != 64 && sphBlock != 128 && sphBlock != 256 && sphBlock != 512) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_set_tuning: SPH block %d not in {64,128,256,512}", sphBlock);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    if (sortDigitBits < 6 || sortDigitBits > 8) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_set_tuning: sort digit bits %d outside 6..8", sortDigitBits);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    state().sph_block = sphBlock;
+    state().sort_digit_bits = sortDigitBits;
+    return FG_OK;
+}
+
+int fg_get_tuning(int* sphBlock, int* sortDigitBits) {
+    if (sphBlock) *sphBlock = state().sph_block;
+    if (sortDigitBits) *sortDigitBits = state().sort_digit_bits;
+    return FG_OK;
+}
+
 // K_FluidSim.cu:591-597
 void gInitializeCommonFluidParams(fg_common_fluid_params p) {
     state().common = p;

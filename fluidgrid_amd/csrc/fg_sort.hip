@@ -38,11 +38,25 @@ struct SortPlan {
     int bits[kMaxPasses];
 };
 
-static SortPlan make_plan(int keyBits) {
+constexpr int kMinDigitBits = 6;
+
+// Passes for keyBits-bit keys with digits of at most digitBits bits (6..8),
+// never more than kMaxPasses; bits are spread evenly over the passes.
+static int plan_passes(int keyBits, int digitBits) {
+    if (digitBits < kMinDigitBits) digitBits = kMinDigitBits;
+    if (digitBits > 8) digitBits = 8;
+    int p = (keyBits + digitBits - 1) / digitBits;
+    const int pmin = (keyBits + 7) / 8;
+    if (p > kMaxPasses) p = kMaxPasses;
+    if (p < pmin) p = pmin;
+    return p;
+}
+
+static SortPlan make_plan(int keyBits, int digitBits = 8) {
     SortPlan p{};
     if (keyBits < 1) keyBits = 1;
     if (keyBits > 32) keyBits = 32;
-    p.passes = (keyBits + 7) / 8;
+    p.passes = plan_passes(keyBits, digitBits);
     const int base = keyBits / p.passes, rem = keyBits % p.passes;
     int s = 0;
     for (int i = 0; i < p.passes; i++) {
@@ -59,17 +73,22 @@ struct SortLayout {
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-static SortLayout make_layout(uint32_t n, const SortPlan& plan) {
+// Workspace layout for n keys of keyBits bits, large enough for every digit
+// width the tuner may select.
+static SortLayout make_layout(uint32_t n, int keyBits) {
     SortLayout L{};
+    if (keyBits < 1) keyBits = 1;
+    if (keyBits > 32) keyBits = 32;
+    const int max_passes = plan_passes(keyBits, kMinDigitBits);
     const size_t tiles = (n + kSortTile - 1) / kSortTile;
     size_t o = 0;
     L.hist = o; o = align256(o + sizeof(uint32_t) * kMaxPasses * kMaxRadix);
     L.offs = o; o = align256(o + sizeof(uint32_t) * kMaxPasses * kMaxRadix);
     L.counters = o; o = align256(o + sizeof(uint32_t) * kMaxPasses + sizeof(uint32_t));
-    L.status = o; o = align256(o + sizeof(uint32_t) * plan.passes * tiles * kMaxRadix);
+    L.status = o; o = align256(o + sizeof(uint32_t) * (size_t)max_passes * tiles * kMaxRadix);
     L.t1k = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
     L.t1v = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
-    if (plan.passes >= 3) {
+    if (max_passes >= 3) {
         L.t2k = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
         L.t2v = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
     } else {
@@ -319,8 +338,8 @@ static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, voi
         set_error(FG_ERR_INVALID_ARGUMENT, "sort: n = %u exceeds 2^30 - 1", n);
         return FG_ERR_INVALID_ARGUMENT;
     }
-    const SortPlan plan = make_plan(keyBits);
-    const SortLayout L = make_layout(n, plan);
+    const SortPlan plan = make_plan(keyBits, state().sort_digit_bits);
+    const SortLayout L = make_layout(n, keyBits);
     if (ws_bytes < L.total) {
         set_error(FG_ERR_INVALID_ARGUMENT, "sort: workspace %zu bytes < %zu needed", ws_bytes, L.total);
         return FG_ERR_INVALID_ARGUMENT;
@@ -375,7 +394,7 @@ using namespace fg;
 
 extern "C" {
 
-size_t fg_sort_workspace_size(unsigned int n, int keyBits) { return make_layout(n, make_plan(keyBits)).total; }
+size_t fg_sort_workspace_size(unsigned int n, int keyBits) { return make_layout(n, keyBits).total; }
 
 int fg_sort_pairs_ws(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits, void* workspace,
                      size_t workspace_bytes, int values_are_iota) {
@@ -426,7 +445,7 @@ void gSort(unsigned int* _key, unsigned int* _values, int _numParticles) {
     hipStream_t st = state().stream;
     void* ws = ensure_scratch(fg_sort_workspace_size(n, 32));
     if (!ws) return;
-    uint32_t* kmax = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + make_layout(n, make_plan(32)).counters) +
+    uint32_t* kmax = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + make_layout(n, 32).counters) +
                      kMaxPasses;
     launch_fill_u32(kmax, 0u, 1, st);
     unsigned blocks = grid_for(n, 256 * 16);

@@ -1,0 +1,294 @@
+// fg_sph_fast.hpp -- per-particle SPH work shared by the single-fluid
+// kernels (fg_sph.hip) and the batched-scene kernels (fg_batch.hip):
+//  * acceleration after gravity, walls, body contact and clamp, and the
+//    velocity/position write-back (reference arithmetic, contraction off);
+//  * the fast LINEAR density and force passes over the 9 stencil rows.
+#pragma once
+#include "fg_contact.hpp"
+#include "fg_internal.hpp"
+#include "fg_walls.hpp"
+
+namespace fg {
+
+#pragma clang fp contract(off)
+
+// Acceleration after gravity, walls and clamp (K_FluidSim.cu:287-303), from
+// the pair sum `internal`.
+__device__ __forceinline__ F3 finish_accel(F3 internal, float m, const float4& pos, const float4& vel,
+                                           const fg_common_fluid_params& cp, const Contacts& bc) {
+    const F3 g = f3(cp.global_external_force.x, cp.global_external_force.y, cp.global_external_force.z);
+    F3 total = add(mul(internal, m), g);
+    // K_FluidSim.cu:389-390: body contact forces are added after the mass
+    // factor (the uncoupled kernel adds nothing here).
+    if (bc.coupled) total = add(total, fluid_external(pos, vel, bc));
+    F3 wall = walls_no_penetration(xyz(pos), xyz(vel), cp);
+    wall = add(wall, walls_no_slip(xyz(pos), xyz(vel), cp));
+    total = add(total, wall);
+    const float speed = len(total);
+    if (speed > cp.velocity_limit) {
+        const float t = cp.velocity_limit / speed;
+        total = mul(total, t);
+    }
+    return total;
+}
+
+// vel[id] = v + a*dt, and optionally pos[id] = p + vel[id]*(dt/s) where
+// p.w > 0 (K_FluidSim.cu:306-307 then :433-438).
+__device__ __forceinline__ void write_back(uint32_t o, uint32_t n, const float4& pos, const float4& vel, F3 a,
+                                           const fg_common_fluid_params& cp, float4* __restrict__ vout,
+                                           float4* __restrict__ pout) {
+    if (o >= n) return;
+    const F3 dv = mul(a, cp.deltaTimeIntegration);
+    const float4 nv = make_float4(vel.x + dv.x, vel.y + dv.y, vel.z + dv.z, vel.w + 0.0f);
+    vout[o] = nv;
+    if (pout && pos.w > 0.0f) {
+        const float q = cp.deltaTimeIntegration / cp.simulation_scale;
+        pout[o] = make_float4(pos.x + nv.x * q, pos.y + nv.y * q, pos.z + nv.z * q, pos.w + nv.w * q);
+    }
+}
+
+#pragma clang fp contract(fast)
+
+// ---------------------------------------------------------------------------
+// Fast LINEAR kernels
+// ---------------------------------------------------------------------------
+
+// Per-particle view of the 3x3 stencil rows.  A row is the x-run of cells
+// cx-1..cx+1 at (cy+dy, cz+dz); its particles are sorted[row.begin, row.end).
+struct RowWalker {
+    float gx_lo, gx_hi;  // squared scaled distance to the x-neighbour cells
+    float gy[3], gz[3];  // squared scaled distance to the y/z neighbour slabs
+    int cx, cy, cz;
+};
+
+__device__ __forceinline__ float sq(float v) { return v * v; }
+
+__device__ __forceinline__ RowWalker make_rows(const float4& p, const fg_common_fluid_params& cp) {
+    RowWalker w;
+    const CellXYZ c = linear_cell(p.x, p.y, p.z, cp);
+    w.cx = c.x;
+    w.cy = c.y;
+    w.cz = c.z;
+    const float s = cp.simulation_scale;
+    // Lower bounds of the distance from p to the neighbouring cells along
+    // each axis (zero when p lies outside its own, clamped, cell).
+    const float lx = cp.volume_min.x + (float)c.x * cp.cell_size.x, hx = lx + cp.cell_size.x;
+    const float ly = cp.volume_min.y + (float)c.y * cp.cell_size.y, hy = ly + cp.cell_size.y;
+    const float lz = cp.volume_min.z + (float)c.z * cp.cell_size.z, hz = lz + cp.cell_size.z;
+    w.gx_lo = sq(fmaxf(0.0f, p.x - lx) * s);
+    w.gx_hi = sq(fmaxf(0.0f, hx - p.x) * s);
+    w.gy[0] = sq(fmaxf(0.0f, p.y - ly) * s);
+    w.gy[1] = 0.0f;
+    w.gy[2] = sq(fmaxf(0.0f, hy - p.y) * s);
+    w.gz[0] = sq(fmaxf(0.0f, p.z - lz) * s);
+    w.gz[1] = 0.0f;
+    w.gz[2] = sq(fmaxf(0.0f, hz - p.z) * s);
+    return w;
+}
+
+// Sorted range of row (dy,dz) after culling; returns false when empty.  The
+// grid's buckets start at key_base of the key space (0 for a single fluid;
+// the scene's first bucket in a batch).
+// `cut` is h^2 enlarged by a small margin so that rounding in the cell
+// bounds can never drop a pair with r < h.
+__device__ __forceinline__ bool row_range(const RowWalker& w, int dy, int dz, float cut,
+                                          const fg_common_fluid_params& cp, uint32_t key_base,
+                                          const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                          uint32_t n, uint32_t& rb, uint32_t& re) {
+    const int cy = w.cy + dy, cz = w.cz + dz;
+    if (cy < 0 || cz < 0 || cy >= cp.bucket_size.y || cz >= cp.bucket_size.z) return false;
+    const float gyz = w.gy[dy + 1] + w.gz[dz + 1];
+    if (gyz >= cut) return false;
+    int x0 = w.cx - 1, x1 = w.cx + 1;
+    if (x0 < 0 || gyz + w.gx_lo >= cut) x0 = w.cx;
+    if (x1 >= cp.bucket_size.x || gyz + w.gx_hi >= cut) x1 = w.cx;
+    const uint32_t kb = key_base + linear_key(0, cy, cz, cp);
+    uint32_t b = kEmpty, e = 0;
+    for (int x = x0; x <= x1; x++) {
+        const uint32_t s0 = start[kb + (uint32_t)x];
+        if (s0 != kEmpty) {
+            if (b == kEmpty) b = s0;
+            e = end[kb + (uint32_t)x];
+        }
+    }
+    if (b == kEmpty) return false;
+    rb = b;
+    re = min(e, n);
+    return rb < re;
+}
+
+// Blocks are handed to the eight XCDs round-robin; remapping them so that each
+// XCD walks one contiguous slice of the sorted particles keeps the neighbour
+// rows of consecutive blocks in the same L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = nblocks / kXcd, rem = nblocks % kXcd;
+    const uint32_t x = b % kXcd, idx = b / kXcd;
+    return x * per + min(x, rem) + idx;
+}
+
+// Neighbour bitmasks: for each of the 9 stencil rows of a particle, bit k of
+// masks[row * n + i] is set when sorted particle rowBegin + k lies within h.
+// Rows longer than 64 particles are not encoded and are re-tested.
+constexpr uint32_t kRowMaskBits = 64;
+
+// Poly6 sum over four candidates; bit u of the returned nibble is set when
+// candidate u lies within h.
+__device__ __forceinline__ uint32_t density4(const float4& p, const float4& q0, const float4& q1, const float4& q2,
+                                             const float4& q3, float h2u, float hhu, float& acc) {
+    const float4 q[4] = {q0, q1, q2, q3};
+    uint32_t nib = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const float dx = p.x - q[u].x, dy = p.y - q[u].y, dz = p.z - q[u].z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float t = h2u - r2;
+        const bool hit = r2 < hhu;
+        acc = hit ? acc + t * t * t : acc;
+        nib |= hit ? (1u << u) : 0u;
+    }
+    return nib;
+}
+
+// Density of sorted particle i (i < n): rho_p[i] = (max(1, rho), k (rho - rho0)),
+// and, with MASKS, its 9 row masks.
+template <bool MASKS>
+__device__ __forceinline__ void density_particle(uint32_t i, const float4* __restrict__ sp,
+                                                 float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
+                                                 const uint32_t* __restrict__ end, uint32_t n, float k, float rho0,
+                                                 float m, const fg_common_fluid_params& cp,
+                                                 const fg_kernels_cache& kc, uint32_t key_base,
+                                                 unsigned long long* __restrict__ masks) {
+    const float4 p = sp[i];
+    if (!(p.w > 0.0f)) return;
+    // Distances are compared unscaled: r_u = r / s, so r < h <=> r_u^2 < (h/s)^2
+    // and h^2 - r^2 = s^2 (h^2/s^2 - r_u^2).
+    const float s = cp.simulation_scale, s2 = s * s;
+    const float h = cp.kernel_radius, hh = h * h;
+    const float hu = h / s, hhu = hu * hu, h2u = cp.kernel_radius_squared / s2;
+    const float cut = hh * 1.0001f;
+    const RowWalker w = make_rows(p, cp);
+    float acc = 0.0f;
+#pragma unroll
+    for (int row = 0; row < 9; row++) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        unsigned long long msk = 0ull;
+        uint32_t rb, re;
+        if (row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) {
+            const uint32_t len = re - rb, n4 = len & ~3u;
+            const float4* pp = sp + rb;
+            uint32_t kk = 0;
+            if (n4) {
+                // Software-pipelined groups of four: the next group's loads are
+                // in flight while the current group is evaluated.
+                float4 a0 = pp[0], a1 = pp[1], a2 = pp[2], a3 = pp[3];
+                for (; kk + 4 < n4; kk += 4) {
+                    pp += 4;
+                    const float4 b0 = pp[0], b1 = pp[1], b2 = pp[2], b3 = pp[3];
+                    const uint32_t nib = density4(p, a0, a1, a2, a3, h2u, hhu, acc);
+                    if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
+                    a0 = b0; a1 = b1; a2 = b2; a3 = b3;
+                }
+                const uint32_t nib = density4(p, a0, a1, a2, a3, h2u, hhu, acc);
+                if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
+                kk += 4;
+                pp += 4;
+            }
+            if (kk < len) {  // 1-3 remaining candidates; the others are masked out
+                const uint32_t rest = len - kk;
+                const float4 far = make_float4(3.0e38f, 3.0e38f, 3.0e38f, 0.0f);
+                const float4 t0 = pp[0];
+                const float4 t1 = rest > 1 ? pp[1] : far;
+                const float4 t2 = rest > 2 ? pp[2] : far;
+                const uint32_t nib = density4(p, t0, t1, t2, far, h2u, hhu, acc);
+                if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
+            }
+        }
+        // Rows longer than 64 particles are never read through their mask.
+        if (MASKS) masks[(size_t)row * n + i] = msk;
+    }
+    const float rho = m * kc.kPoly6 * (s2 * s2 * s2) * acc;
+    rho_p[i] = make_float2(fmaxf(1.0f, rho), k * (rho - rho0));
+}
+
+// Pressure and viscosity terms of one neighbour j (r < h checked again, so
+// the same code serves rows without a mask).
+struct PairAcc {
+    float px, py, pz, vx, vy, vz;
+};
+
+__device__ __forceinline__ void pair_term(PairAcc& A, uint32_t i, uint32_t j, const float4& pos, const float4& vel,
+                                          float pri, float4 q, float4 vj, float2 dpj, float h, float s, float hhu) {
+    const float dx = pos.x - q.x, dy = pos.y - q.y, dz = pos.z - q.z;
+    const float r2u = dx * dx + dy * dy + dz * dz;  // unscaled
+    if (r2u < hhu && j != i) {
+        const float iru = __builtin_amdgcn_rsqf(r2u);
+        const float hr = h - r2u * iru * s;
+        const float irj = __builtin_amdgcn_rcpf(dpj.x);
+        const float cp_ = hr * hr * (pri + dpj.y) * irj * iru;
+        A.px += dx * cp_;
+        A.py += dy * cp_;
+        A.pz += dz * cp_;
+        const float cv = hr * irj;
+        A.vx += (vj.x - vel.x) * cv;
+        A.vy += (vj.y - vel.y) * cv;
+        A.vz += (vj.z - vel.z) * cv;
+    }
+}
+
+// Pressure + viscosity + walls (+ body contact) of sorted particle i (i < n),
+// written to vout[id[i]] (and pout[id[i]] when integrating).
+template <bool MASKS>
+__device__ __forceinline__ void forces_particle(uint32_t i, const float4* __restrict__ sp,
+                                                const float4* __restrict__ sv, const float2* __restrict__ rho_p,
+                                                const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                                float4* __restrict__ vout, float4* __restrict__ pout,
+                                                const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
+                                                const fg_common_fluid_params& cp, const fg_kernels_cache& kc,
+                                                uint32_t key_base, const unsigned long long* __restrict__ masks,
+                                                const Contacts& bc) {
+    const float4 pos = sp[i];
+    const float4 vel = sv[i];
+    F3 a = f3(0.0f, 0.0f, 0.0f);
+    if (pos.w > 0.0f) {
+        const float h = cp.kernel_radius, hh = h * h;
+        const float s = cp.simulation_scale;
+        const float hu = h / s, hhu = hu * hu;
+        const float cut = hh * 1.0001f;
+        const float2 dpi = rho_p[i];
+        const RowWalker w = make_rows(pos, cp);
+        PairAcc A{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (int row = 0; row < 9; row++) {
+            const int dz = row / 3 - 1, dy = row % 3 - 1;
+            uint32_t rb, re;
+            if (!row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) continue;
+            if (MASKS && re - rb <= kRowMaskBits) {
+                unsigned long long mm = masks[(size_t)row * n + i];
+                while (mm) {
+                    const uint32_t k1 = (uint32_t)__builtin_ctzll(mm);
+                    mm &= mm - 1ull;
+                    const bool two = mm != 0ull;
+                    const uint32_t k2 = two ? (uint32_t)__builtin_ctzll(mm) : k1;
+                    if (two) mm &= mm - 1ull;
+                    const uint32_t j1 = rb + k1, j2 = rb + k2;
+                    const float4 q1 = sp[j1], q2 = sp[j2];
+                    const float4 v1 = sv[j1], v2 = sv[j2];
+                    const float2 d1 = rho_p[j1], d2 = rho_p[j2];
+                    pair_term(A, i, j1, pos, vel, dpi.y, q1, v1, d1, h, s, hhu);
+                    if (two) pair_term(A, i, j2, pos, vel, dpi.y, q2, v2, d2, h, s, hhu);
+                }
+            } else {
+                for (uint32_t j = rb; j < re; j++) pair_term(A, i, j, pos, vel, dpi.y, sp[j], sv[j], rho_p[j], h, s, hhu);
+            }
+        }
+        // pressure: (s d_u / r) = d_u / r_u, so no scale factor remains
+        const float iri = 1.0f / dpi.x;
+        const float cP = -kc.kSpikyGradient * iri;
+        const float cV = kc.kViscosityLaplacian * mu * iri;
+        const F3 internal = f3(cP * A.px + cV * A.vx, cP * A.py + cV * A.vy, cP * A.pz + cV * A.vz);
+        a = finish_accel(internal, m, pos, vel, cp, bc);
+    }
+    write_back(id[i], n, pos, vel, a, cp, vout, pout);
+}
+
+}  // namespace fg

@@ -1,0 +1,151 @@
+// fg_batch_capi.cpp -- C-ABI over BatchedSimulator (one process-wide batch).
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "fluidgrid/BatchedSimulator.h"
+#include "fluidgrid_amd_batch.h"
+
+using namespace Simulation;
+
+namespace {
+std::unique_ptr<Simulators::BatchedSimulator>& batch() {
+    static std::unique_ptr<Simulators::BatchedSimulator> b;
+    return b;
+}
+
+int fail(const std::exception& e) {
+    fg_record_error(FG_ERR_HIP, e.what());
+    return FG_ERR_HIP;
+}
+}  // namespace
+
+extern "C" {
+
+int fg_bsim_create(unsigned int numScenes, const fg_common_fluid_params* commons, const fg_fluid_params* fluids,
+                   const unsigned int* counts, const fg_float4* positions, const fg_float4* velocities) {
+    if (numScenes == 0 || !commons || !fluids || !counts || !positions || !velocities) {
+        fg_record_error(FG_ERR_INVALID_ARGUMENT, "fg_bsim_create: null pointer or no scenes");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        batch().reset();
+        std::vector<Simulators::BatchScene> scenes(numScenes);
+        size_t off = 0;
+        for (unsigned s = 0; s < numScenes; s++) {
+            scenes[s].common = Application::abi_cast<Application::CommonFluidParams>(commons[s]);
+            scenes[s].fluid = Application::abi_cast<Application::FluidParams>(fluids[s]);
+            const float4* p = reinterpret_cast<const float4*>(positions) + off;
+            const float4* v = reinterpret_cast<const float4*>(velocities) + off;
+            scenes[s].positions.assign(p, p + counts[s]);
+            scenes[s].velocities.assign(v, v + counts[s]);
+            off += counts[s];
+        }
+        batch() = std::make_unique<Simulators::BatchedSimulator>(scenes);
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_bsim_set_options(int use_graph, int stage_timing) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    batch()->setUseGraph(use_graph != 0);
+    batch()->setStageTiming(stage_timing != 0);
+    return FG_OK;
+}
+
+int fg_bsim_step(int steps) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        batch()->doTimeSteps(steps);
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_bsim_synchronize(void) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        batch()->synchronize();
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_bsim_download(fg_float4* positions, fg_float4* velocities) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        batch()->download(reinterpret_cast<float4*>(positions), reinterpret_cast<float4*>(velocities));
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_bsim_download_density(fg_float2* rho_p, unsigned int* particleId) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        batch()->downloadDensity(reinterpret_cast<float2*>(rho_p), particleId);
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_bsim_stage_times(float* out5) {
+    if (!batch() || !out5) return FG_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < 5; k++) out5[k] = batch()->stageTimes()[k];
+    return FG_OK;
+}
+
+int fg_bsim_info(unsigned int* numParticles, unsigned int* numBuckets, int* keyBits) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    if (numParticles) *numParticles = batch()->numParticles();
+    if (numBuckets) *numBuckets = batch()->numBuckets();
+    if (keyBits) *keyBits = batch()->keyBits();
+    return FG_OK;
+}
+
+int fg_bsim_autotune(int trialSteps, int* sphBlock, int* sortDigitBits, float* trials, int maxTrials,
+                     int* numTrials) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        std::vector<std::pair<Simulators::BatchTuning, float>> log;
+        const Simulators::BatchTuning t = batch()->autotune(trialSteps, &log);
+        if (sphBlock) *sphBlock = t.sphBlock;
+        if (sortDigitBits) *sortDigitBits = t.sortDigitBits;
+        int k = 0;
+        for (; trials && k < maxTrials && k < (int)log.size(); k++) {
+            trials[3 * k] = (float)log[k].first.sphBlock;
+            trials[3 * k + 1] = (float)log[k].first.sortDigitBits;
+            trials[3 * k + 2] = log[k].second;
+        }
+        if (numTrials) *numTrials = k;
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_bsim_set_tuning(int sphBlock, int sortDigitBits) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        Simulators::BatchTuning t;
+        t.sphBlock = sphBlock;
+        t.sortDigitBits = sortDigitBits;
+        batch()->setTuning(t);
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_bsim_destroy(void) {
+    batch().reset();
+    return FG_OK;
+}
+
+}  // extern "C"

@@ -103,6 +103,23 @@ SIGNATURES = {
     "fg_body_reduce_f64": (i32, [vp, vp, vp, u32, vp, vp, vp, vp]),
     "fg_body_mass_properties_f64": (i32, [vp, vp, u32, vp, vp, vp]),
     "fg_sphere_bounds": (i32, [vp, u32, f32, vp]),
+    # tuning and batched scenes
+    "fg_set_tuning": (i32, [i32, i32]),
+    "fg_get_tuning": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fg_batch_hash": (i32, [vp, u32, vp, vp, vp, u32]),
+    "fg_batch_density_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, u32]),
+    "fg_batch_forces_integrate_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
+    "fg_bsim_create": (i32, [u32, vp, vp, vp, vp, vp]),
+    "fg_bsim_set_options": (i32, [i32, i32]),
+    "fg_bsim_step": (i32, [i32]),
+    "fg_bsim_synchronize": (i32, []),
+    "fg_bsim_download": (i32, [vp, vp]),
+    "fg_bsim_download_density": (i32, [vp, vp]),
+    "fg_bsim_stage_times": (i32, [vp]),
+    "fg_bsim_info": (i32, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_int)]),
+    "fg_bsim_autotune": (i32, [i32, C.POINTER(C.c_int), C.POINTER(C.c_int), vp, i32, C.POINTER(C.c_int)]),
+    "fg_bsim_set_tuning": (i32, [i32, i32]),
+    "fg_bsim_destroy": (i32, []),
 }
 
 
@@ -305,3 +322,67 @@ class Simulator:
 
     def close(self):
         load().fg_sim_shutdown()
+
+
+class BatchSimulator:
+    """The batched-scene simulator (include/fluidgrid_amd_batch.h) over a
+    scenes.Ensemble: one launch per stage for every scene."""
+
+    def __init__(self, ens, use_graph=True, stage_timing=False):
+        L = load()
+        S = len(ens.scenes)
+        commons = (abi.CommonFluidParams * S)(*[sc.common for sc in ens.scenes])
+        fluids = (abi.FluidParams * S)(*[sc.fluid for sc in ens.scenes])
+        counts = np.ascontiguousarray([sc.n for sc in ens.scenes], np.uint32)
+        pos = np.ascontiguousarray(np.concatenate([sc.pos for sc in ens.scenes]), np.float32)
+        vel = np.ascontiguousarray(np.concatenate([sc.vel for sc in ens.scenes]), np.float32)
+        check(L.fg_bsim_create(S, C.cast(commons, C.c_void_p), C.cast(fluids, C.c_void_p), counts.ctypes.data,
+                               pos.ctypes.data, vel.ctypes.data), "fg_bsim_create")
+        self.n = int(pos.shape[0])
+        self.counts = counts
+        self.set_options(use_graph, stage_timing)
+
+    def set_options(self, use_graph=True, stage_timing=False):
+        check(load().fg_bsim_set_options(int(use_graph), int(stage_timing)), "fg_bsim_set_options")
+
+    def step(self, steps=1):
+        check(load().fg_bsim_step(int(steps)), "fg_bsim_step")
+
+    def synchronize(self):
+        check(load().fg_bsim_synchronize(), "fg_bsim_synchronize")
+
+    def state(self):
+        pos = np.empty((self.n, 4), np.float32)
+        vel = np.empty((self.n, 4), np.float32)
+        check(load().fg_bsim_download(pos.ctypes.data, vel.ctypes.data), "fg_bsim_download")
+        return pos, vel
+
+    def density(self):
+        rp = np.empty((self.n, 2), np.float32)
+        pid = np.empty(self.n, np.uint32)
+        check(load().fg_bsim_download_density(rp.ctypes.data, pid.ctypes.data), "fg_bsim_download_density")
+        return rp, pid
+
+    def stage_times(self):
+        out = (C.c_float * 5)()
+        check(load().fg_bsim_stage_times(out), "stage times")
+        return dict(zip(("hash", "sort", "cells", "density", "forces"), list(out)))
+
+    def info(self):
+        n, nb, kb = C.c_uint(), C.c_uint(), C.c_int()
+        check(load().fg_bsim_info(C.byref(n), C.byref(nb), C.byref(kb)), "fg_bsim_info")
+        return dict(particles=n.value, buckets=nb.value, key_bits=kb.value)
+
+    def autotune(self, trial_steps=10):
+        blk, bits, nt = C.c_int(), C.c_int(), C.c_int()
+        trials = np.zeros((16, 3), np.float32)
+        check(load().fg_bsim_autotune(int(trial_steps), C.byref(blk), C.byref(bits), trials.ctypes.data, 16,
+                                      C.byref(nt)), "fg_bsim_autotune")
+        return dict(sph_block=blk.value, sort_digit_bits=bits.value,
+                    trials=[(int(a), int(b), float(c)) for a, b, c in trials[:nt.value]])
+
+    def set_tuning(self, sph_block=256, sort_digit_bits=8):
+        check(load().fg_bsim_set_tuning(int(sph_block), int(sort_digit_bits)), "fg_bsim_set_tuning")
+
+    def close(self):
+        load().fg_bsim_destroy()

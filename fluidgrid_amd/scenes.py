@@ -351,3 +351,50 @@ def coupled_config_scene(cfg_id=1, seed=None):
     s.name = f"coupled_cfg{cfg_id}"
     s.info["bodies"] = bodies
     return s
+
+
+# ---------------------------------------------------------------------------
+# Batched ensembles
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Ensemble:
+    """Independent dam-break scenes advanced together (scene-major)."""
+    scenes: list
+    seed: int
+
+    @property
+    def n(self):
+        return sum(s.n for s in self.scenes)
+
+    @property
+    def num_buckets(self):
+        return sum(s.num_buckets for s in self.scenes)
+
+
+def ensemble(num_scenes=256, seed=SEED_BASE + 3, n_range=(4096, 65536), edge_range=(0.5, 1.5), fill=0.3):
+    """Scenes with log-uniform particle counts in n_range and box edges L
+    uniform in edge_range; h is chosen so that the block of N particles at
+    spacing h/2 fills `fill` of the box: N (h/2)^3 = fill L^3.  The block is
+    nx x ny x nz with nx = nz ~ (2N)^(1/3) (a dam-break slab); dt = 0.015 h
+    (the step/h ratio of the single-scene configurations); other constants as
+    in the dam-break configurations."""
+    u = uniform01(seed, 2 * num_scenes).reshape(num_scenes, 2)
+    lo, hi = math.log(n_range[0]), math.log(n_range[1])
+    scenes = []
+    for s in range(num_scenes):
+        n_target = math.exp(lo + (hi - lo) * u[s, 0])
+        edge = edge_range[0] + (edge_range[1] - edge_range[0]) * u[s, 1]
+        nx = max(2, int(round((2.0 * n_target) ** (1.0 / 3.0))))
+        ny = max(1, int(round(n_target / (nx * nx))))
+        while nx * ny * nx < n_range[0]:
+            ny += 1
+        while ny > 1 and nx * ny * nx > n_range[1]:
+            ny -= 1
+        n = nx * ny * nx
+        h = 2.0 * (fill * edge ** 3 / n) ** (1.0 / 3.0)
+        sc = dam_break((nx, ny, nx), h, (edge, edge, edge), 0.015 * h, steps=500, seed=seed * 1000 + s,
+                       name=f"ensemble_{s}")
+        sc.info["scene_index"] = s
+        scenes.append(sc)
+    return Ensemble(scenes, seed)

@@ -1,0 +1,89 @@
+// BatchedSimulator.h -- many independent SPH fluids ("scenes") advanced
+// together with one launch per stage.  No counterpart in the reference,
+// whose HeterogeneousSimulator shares one CommonFluidParams between all
+// fluids (HeterogeneousSimulator.cpp:27-47); here every scene keeps its own
+// parameter block, grid and kernel constants.
+//
+// Per step: keys (scene key base + LINEAR key) -> one stable sort of the
+// shared keys -> cell ranges + reorder over the total bucket count ->
+// density (+ neighbour masks) -> forces + walls + integration.  The step has
+// no host synchronisation and is replayed as a captured HIP graph.
+#pragma once
+
+#include <vector>
+
+#include "ApplicationData.h"
+
+namespace Simulation {
+namespace Simulators {
+
+struct BatchScene {
+    Application::CommonFluidParams common;  // LINEAR grid of this scene
+    Application::FluidParams fluid;
+    std::vector<float4> positions;          // w = active flag
+    std::vector<float4> velocities;
+};
+
+struct BatchTuning {
+    int sphBlock = 256;      // workgroup size of density and force kernels
+    int sortDigitBits = 8;   // widest radix digit
+};
+
+class BatchedSimulator {
+public:
+    explicit BatchedSimulator(const std::vector<BatchScene>& scenes);
+    ~BatchedSimulator();
+    BatchedSimulator(const BatchedSimulator&) = delete;
+    BatchedSimulator& operator=(const BatchedSimulator&) = delete;
+
+    void doTimeStep();                 // eager; records stage events when timing is on
+    void doTimeSteps(int steps);       // graph replay (eager when timing is on)
+    void setUseGraph(bool on);
+    void setStageTiming(bool on);
+    void synchronize();
+
+    // Times every launch shape in turn (trialSteps graph steps each, after a
+    // warm-up) and keeps the fastest; the particle state is restored
+    // afterwards, so tuning does not advance the simulation.
+    BatchTuning autotune(int trialSteps, std::vector<std::pair<BatchTuning, float>>* trials = nullptr);
+    void setTuning(const BatchTuning& t);
+    const BatchTuning& tuning() const { return mTuning; }
+
+    void download(float4* positions, float4* velocities) const;
+    void downloadDensity(float2* rho_p_sorted, unsigned* particleId) const;
+    // hash, sort, cells, density, forces+integrate (ms) of the last timed step
+    const float* stageTimes() const { return mTimes; }
+
+    unsigned numScenes() const { return (unsigned)mDescs.size(); }
+    unsigned numParticles() const { return mN; }
+    unsigned numBuckets() const { return mBuckets; }
+    int keyBits() const { return mKeyBits; }
+    hipStream_t stream() const { return mStream; }
+
+private:
+    void enqueueStep();
+    void invalidateGraph();
+
+    std::vector<fg_scene_desc> mDescs;
+    unsigned mN = 0, mBuckets = 0;
+    int mKeyBits = 1;
+    BatchTuning mTuning;
+    bool mUseGraph = true, mTiming = false;
+    hipStream_t mStream = nullptr;
+    hipGraph_t mGraph = nullptr;
+    hipGraphExec_t mGraphExec = nullptr;
+    hipEvent_t mEv[6] = {};
+    float mTimes[5] = {};
+
+    fg_scene_desc* d_descs = nullptr;
+    unsigned* d_sceneOf = nullptr;
+    float4 *d_pos = nullptr, *d_vel = nullptr, *d_spos = nullptr, *d_svel = nullptr;
+    float2* d_rho = nullptr;
+    unsigned *d_key = nullptr, *d_id = nullptr, *d_start = nullptr, *d_end = nullptr;
+    unsigned long long* d_masks = nullptr;
+    void* d_sortWs = nullptr;
+    size_t mSortWsBytes = 0;
+};
+
+}  // namespace Simulators
+}  // namespace Simulation

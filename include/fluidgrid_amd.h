@@ -259,6 +259,11 @@ int   fg_set_key_mode(int mode);
 int   fg_get_key_mode(void);
 int   fg_set_math_mode(int mode);
 int   fg_get_math_mode(void);
+/* Launch shape (process-wide): workgroup size of the fast density/force
+ * kernels (64, 128, 256 or 512; default 256) and the widest radix digit of
+ * the sort (6..8 bits; default 8).  Results do not depend on either. */
+int   fg_set_tuning(int sphBlock, int sortDigitBits);
+int   fg_get_tuning(int* sphBlock, int* sortDigitBits);
 int   fg_get_common_fluid_params(fg_common_fluid_params* out);
 int   fg_get_kernels_cache(fg_kernels_cache* out);
 
@@ -355,6 +360,45 @@ int fg_body_mass_properties_f64(const fg_float4* worldSpheres, const unsigned in
 
 /* Bounds of all sphere centres expanded by margin: out2[0] min, out2[1] max. */
 int fg_sphere_bounds(const fg_float4* spheres, unsigned int numSpheres, float margin, fg_float4* out2);
+
+/* ------------------------------------------------------------------------ */
+/* 3. Batched scenes (no counterpart in the reference)                       */
+/* ------------------------------------------------------------------------ */
+/* Many independent fluids advance with one launch per stage.  Particles are
+ * stored scene-major: scene s owns [particleBegin, particleBegin +
+ * particleCount) of every particle array.  Each scene has its own LINEAR
+ * grid (common.bucket_size, cell_size, volume_min); its buckets occupy
+ * [keyBase, keyBase + bucket count) of one shared key space, with keyBase
+ * increasing in scene order, so one stable sort of the shared keys sorts
+ * every scene inside its own particle range and the grid arrays
+ * (gridStart/gridEnd of the total bucket count) serve all scenes.
+ * sceneOf[i] is the scene of particle i (the same for sorted and unsorted
+ * order, since the sort keeps scene ranges); particles whose sceneOf is not
+ * below numScenes are left untouched.  Per-scene arithmetic is that of
+ * the single-fluid FG_KEY_LINEAR / FG_MATH_FAST path. */
+typedef struct fg_scene_desc {
+    fg_common_fluid_params common;     /* grid, box, gravity, dt, scale, h of this scene */
+    fg_kernels_cache kernels;
+    float mass, gasConstant, restDensity, viscosity;
+    unsigned int keyBase;              /* first bucket of this scene in the shared key space */
+    unsigned int particleBegin;
+    unsigned int particleCount;
+    unsigned int reserved[4];
+} fg_scene_desc;                       /* 160 bytes */
+
+/* key[i] = keyBase + LINEAR key of positions[i] in its scene's grid. */
+int fg_batch_hash(const fg_scene_desc* scenes, unsigned int numScenes, const unsigned int* sceneOf,
+                  const fg_float4* positions, unsigned int* hashKey, unsigned int numParticles);
+/* fg_sph_density_nbr per scene (neighbour masks always written). */
+int fg_batch_density_nbr(const fg_scene_desc* scenes, unsigned int numScenes, const unsigned int* sceneOf,
+                         fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,
+                         unsigned int* gridEnd, unsigned long long* neighborMasks, unsigned int numParticles);
+/* fg_sph_forces_integrate_nbr per scene (pos may be NULL: no integration). */
+int fg_batch_forces_integrate_nbr(const fg_scene_desc* scenes, unsigned int numScenes,
+                                  const unsigned int* sceneOf, fg_float4* orderedPositions, fg_float4* orderedVelocity,
+                                  fg_float2* density_pressure, unsigned int* gridStart, unsigned int* gridEnd,
+                                  const unsigned long long* neighborMasks, fg_float4* vel, fg_float4* pos,
+                                  unsigned int* particleId, unsigned int numParticles);
 
 /* Device memory and timing helpers (thin wrappers over the HIP runtime). */
 int fg_device_count(int* count);

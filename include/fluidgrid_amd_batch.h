@@ -1,0 +1,43 @@
+/* fluidgrid_amd_batch.h -- C-ABI over the batched-scene simulator
+ * (include/fluidgrid/BatchedSimulator.h).  No counterpart in the reference:
+ * it advances many independent fluids, each with its own parameter block,
+ * with one launch per stage.  One process-wide batch. */
+#ifndef FLUIDGRID_AMD_BATCH_H
+#define FLUIDGRID_AMD_BATCH_H
+
+#include "fluidgrid_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Creates the batch (replacing any previous one).  Scene s has counts[s]
+ * particles; positions/velocities hold all scenes back to back (scene-major).
+ * Every commons[s] describes a LINEAR grid (bucket_size, cell_size,
+ * volume_min) and the scene's box, gravity, dt and scale. */
+int fg_bsim_create(unsigned int numScenes, const fg_common_fluid_params* commons, const fg_fluid_params* fluids,
+                   const unsigned int* counts, const fg_float4* positions, const fg_float4* velocities);
+int fg_bsim_set_options(int use_graph, int stage_timing);
+int fg_bsim_step(int steps);                      /* asynchronous */
+int fg_bsim_synchronize(void);
+int fg_bsim_download(fg_float4* positions, fg_float4* velocities);
+/* (max(1,rho), p) in sorted order and the sorted particle ids of the last step. */
+int fg_bsim_download_density(fg_float2* rho_p, unsigned int* particleId);
+/* hash, sort, cells, density, forces+integrate (ms) of the last timed step. */
+int fg_bsim_stage_times(float* out5);
+/* numParticles, numBuckets, keyBits of the batch. */
+int fg_bsim_info(unsigned int* numParticles, unsigned int* numBuckets, int* keyBits);
+/* Launch-shape tuning: times each candidate for trialSteps steps, keeps the
+ * fastest and restores the particle state.  trials (optional, capacity
+ * maxTrials) receives (sphBlock, sortDigitBits, ms per step) triples;
+ * *numTrials their count. */
+int fg_bsim_autotune(int trialSteps, int* sphBlock, int* sortDigitBits, float* trials, int maxTrials,
+                     int* numTrials);
+int fg_bsim_set_tuning(int sphBlock, int sortDigitBits);
+int fg_bsim_destroy(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDGRID_AMD_BATCH_H */

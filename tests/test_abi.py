@@ -12,7 +12,8 @@ import pytest
 from fluidgrid_amd import abi, lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("fluidgrid_amd.h", "fluidgrid_amd_sim.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("fluidgrid_amd.h", "fluidgrid_amd_sim.h",
+                                                      "fluidgrid_amd_batch.h")]
 DECL = re.compile(r"^\s*(?:const\s+)?(?:void|int|size_t|char|unsigned int)\s*\*?\s*(\w+)\s*\(", re.M)
 
 
@@ -55,8 +56,8 @@ def test_struct_layouts_match_c_header(tmp_path):
         #include <stddef.h>
         #include "{ROOT}/include/fluidgrid_amd.h"
         int main(void) {{
-          printf("%zu %zu %zu %zu\\n", sizeof(fg_common_fluid_params), sizeof(fg_fluid_params),
-                 sizeof(fg_kernels_cache), sizeof(fg_common_rigid_body_params));
+          printf("%zu %zu %zu %zu %zu\\n", sizeof(fg_common_fluid_params), sizeof(fg_fluid_params),
+                 sizeof(fg_kernels_cache), sizeof(fg_common_rigid_body_params), sizeof(fg_scene_desc));
           printf("%zu %zu %zu %zu\\n", offsetof(fg_common_fluid_params, volume_min),
                  offsetof(fg_common_fluid_params, deltaTimeIntegration),
                  offsetof(fg_common_rigid_body_params, global_external_force),
@@ -67,9 +68,9 @@ def test_struct_layouts_match_c_header(tmp_path):
     subprocess.run(["gcc", "-std=c11", str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     got = list(map(int, out))
-    assert got[:4] == [C.sizeof(abi.CommonFluidParams), C.sizeof(abi.FluidParams), C.sizeof(abi.KernelsCache),
-                       C.sizeof(abi.CommonRigidBodyParams)] == [104, 28, 12, 112]
-    assert got[4:] == [abi.CommonFluidParams.volume_min.offset, abi.CommonFluidParams.deltaTimeIntegration.offset,
+    assert got[:5] == [C.sizeof(abi.CommonFluidParams), C.sizeof(abi.FluidParams), C.sizeof(abi.KernelsCache),
+                       C.sizeof(abi.CommonRigidBodyParams), C.sizeof(abi.SceneDesc)] == [104, 28, 12, 112, 160]
+    assert got[5:] == [abi.CommonFluidParams.volume_min.offset, abi.CommonFluidParams.deltaTimeIntegration.offset,
                        abi.CommonRigidBodyParams.global_external_force.offset,
                        abi.CommonRigidBodyParams.particleRadius.offset]
 

@@ -1,0 +1,148 @@
+"""GPU parity of the batched-scene form (fg_batch_* and the batch simulator).
+
+  keys                      bit-exact: keyBase + the oracle's LINEAR key per scene
+  every scene of a batch    bit-identical to the same scene stepped alone through
+                            the single-fluid simulator (same kernels, same
+                            neighbour order), over several graph-replayed steps
+  one batched step          density within 1e-5 and positions against the
+                            oracle's step per scene
+  launch-shape tuning       results independent of the shape (bit-exact), state
+                            restored after tuning
+  full 256-scene ensemble   finite state, active flags unchanged, densities positive
+"""
+import numpy as np
+import pytest
+
+from fluidgrid_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def small_ensemble(k=6, seed=11):
+    return scenes.ensemble(num_scenes=k, seed=seed, n_range=(400, 3000))
+
+
+def test_batch_hash_bit_exact(fglib, orc):
+    L = fglib.load()
+    ens = small_ensemble()
+    descs = (abi.SceneDesc * len(ens.scenes))()
+    base = 0
+    expect = []
+    scene_of = []
+    for s, sc in enumerate(ens.scenes):
+        descs[s].common = sc.common
+        descs[s].keyBase = base
+        key, _ = orc.hash_keys(sc.pos, sc.num_buckets, tuple(sc.common.cell_size), sc.common, abi.KEY_LINEAR)
+        expect.append(key.astype(np.uint64) + base)
+        scene_of.append(np.full(sc.n, s, np.uint32))
+        base += sc.num_buckets
+    pos = np.concatenate([sc.pos for sc in ens.scenes])
+    pos[5, :3] = [-3.0, 99.0, np.nan]  # clamped into its scene's grid
+    k0, _ = orc.hash_keys(pos[5:6], ens.scenes[0].num_buckets, tuple(ens.scenes[0].common.cell_size),
+                          ens.scenes[0].common, abi.KEY_LINEAR)
+    expect[0][5] = k0[0]
+    d_desc = fglib.DeviceArray.from_numpy(np.frombuffer(bytes(descs), np.uint8).copy())
+    d_scene = fglib.DeviceArray.from_numpy(np.concatenate(scene_of))
+    d_pos = fglib.DeviceArray.from_numpy(pos)
+    d_key = fglib.DeviceArray((len(pos),), np.uint32)
+    fglib.check(L.fg_batch_hash(d_desc.ptr, len(ens.scenes), d_scene.ptr, d_pos.ptr, d_key.ptr, len(pos)), "hash")
+    assert np.array_equal(d_key.numpy().astype(np.uint64), np.concatenate(expect))
+
+
+def _run_alone(fglib, sc, steps):
+    sim = fglib.Simulator(sc, fused=True, use_graph=True)
+    try:
+        sim.step(steps)
+        return sim.state()
+    finally:
+        sim.close()
+
+
+def test_each_scene_matches_single_fluid_path_bit_exact(fglib):
+    ens = small_ensemble()
+    steps = 12
+    b = fglib.BatchSimulator(ens)
+    try:
+        b.step(steps)
+        pos, vel = b.state()
+    finally:
+        b.close()
+    off = 0
+    for sc in ens.scenes:
+        p1, v1 = _run_alone(fglib, sc, steps)
+        assert np.array_equal(pos[off:off + sc.n], p1), sc.name
+        assert np.array_equal(vel[off:off + sc.n], v1), sc.name
+        off += sc.n
+
+
+def test_one_batched_step_matches_oracle(fglib, orc):
+    ens = small_ensemble(k=4, seed=12)
+    for sc in ens.scenes:
+        sc.vel[:, :3] = np.random.default_rng(sc.seed % 1000).normal(0, 0.3, (sc.n, 3)).astype(np.float32)
+    b = fglib.BatchSimulator(ens, use_graph=False)
+    try:
+        b.step(1)
+        pos, vel = b.state()
+        rp, pid = b.density()
+    finally:
+        b.close()
+    off = 0
+    for sc in ens.scenes:
+        st = orc.step(sc.pos, sc.vel, sc.fluid, sc.common, sc.kernels, abi.KEY_LINEAR, want_acc=True)
+        assert np.array_equal(pid[off:off + sc.n] - off, st.spid)  # stable permutation per scene
+        g = rp[off:off + sc.n]
+        act = st.spos[:, 3] > 0
+        assert np.all(np.abs(g[act, 0] - st.rho_p[act, 0]) <= 1e-5 * np.abs(st.rho_p[act, 0]))
+        dt = sc.common.deltaTimeIntegration
+        _, _, mag = orc.forces(st.spos, st.svel, st.rho_p.astype(np.float64), st.start, st.end, st.spid, sc.fluid,
+                               sc.common, sc.kernels, abi.KEY_LINEAR, f64=True, want_mag=True)
+        scale = np.zeros((sc.n, 1))
+        scale[st.spid, 0] = np.linalg.norm(st.acc, axis=1) + mag
+        tol_v = dt * 1e-5 * scale + 2 * np.spacing(np.abs(st.vel[:, :3]))
+        assert np.all(np.abs(vel[off:off + sc.n, :3] - st.vel[:, :3]) <= tol_v), sc.name
+        assert np.all(np.abs(pos[off:off + sc.n, :3] - st.pos[:, :3]) <= tol_v * dt + 1e-7), sc.name
+        off += sc.n
+
+
+def test_tuning_changes_no_result_and_restores_state(fglib):
+    ens = small_ensemble(k=5, seed=13)
+    b = fglib.BatchSimulator(ens)
+    try:
+        b.step(3)
+        before = b.state()
+        t = b.autotune(trial_steps=2)
+        after = b.state()
+        assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+        assert t["sph_block"] in (64, 128, 256, 512) and 6 <= t["sort_digit_bits"] <= 8
+        assert len(t["trials"]) >= 6
+        results = []
+        for blk, bits in ((256, 8), (64, 6), (512, 7)):
+            # each shape starts from the initial state of the ensemble
+            b.close()
+            b = fglib.BatchSimulator(ens)
+            b.set_tuning(blk, bits)
+            b.step(7)
+            results.append(b.state())
+        for r in results[1:]:
+            assert np.array_equal(r[0], results[0][0]) and np.array_equal(r[1], results[0][1])
+    finally:
+        b.close()
+        fglib.load().fg_set_tuning(256, 8)
+
+
+def test_full_ensemble_runs(fglib):
+    ens = scenes.ensemble()
+    b = fglib.BatchSimulator(ens)
+    try:
+        info = b.info()
+        assert info["particles"] == ens.n and info["buckets"] == ens.num_buckets
+        b.step(5)
+        pos, vel = b.state()
+        rp, pid = b.density()
+    finally:
+        b.close()
+    w0 = np.concatenate([sc.pos[:, 3] for sc in ens.scenes])
+    assert np.array_equal(pos[:, 3], w0)
+    assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
+    assert np.all(rp[:, 0] >= 1.0) and np.all(np.isfinite(rp))
+    assert np.array_equal(np.sort(pid), np.arange(ens.n, dtype=np.uint32))
