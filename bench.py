@@ -98,6 +98,59 @@ def measure(scene, steps, warmup, fused=True):
     return dict(seconds=dt, ms_per_step=1e3 * dt / steps, c_occ=c_occ, stage_ms=stage_ms, finite=finite, body=body)
 
 
+def measure_batched(ens, steps, warmup, tune_steps=5):
+    """The batched form over an ensemble: fixed launch shape (256-lane
+    workgroups, 8-bit digits), then the autotuned shape; per-stage times of
+    the tuned shape; and the same scenes stepped one at a time through the
+    single-fluid simulator (the plain form's way to run them)."""
+    out = {}
+    b = lib.BatchSimulator(ens)
+    try:
+        b.set_tuning(256, 8)
+        for label in ("fixed", "tuned"):
+            if label == "tuned":
+                t = b.autotune(trial_steps=tune_steps)
+                out["tuning"] = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
+                                 "trials_ms": [[a, c, round(m, 4)] for a, c, m in t["trials"]]}
+            b.step(warmup)
+            b.synchronize()
+            t0 = time.perf_counter()
+            b.step(steps)
+            b.synchronize()
+            dt = time.perf_counter() - t0
+            out[label] = {"ms_per_step": 1e3 * dt / steps, "value": ens.n * steps / dt}
+        b.set_options(use_graph=False, stage_timing=True)
+        acc = None
+        for _ in range(5):
+            b.step(1)
+            t = b.stage_times()
+            acc = t if acc is None else {k: acc[k] + t[k] for k in t}
+        out["stage_ms"] = {k: round(v / 5, 5) for k, v in acc.items()}
+        pos, _ = b.state()
+        out["finite"] = bool(np.all(np.isfinite(pos)))
+        info = b.info()
+    finally:
+        b.close()
+        lib.load().fg_set_tuning(256, 8)
+    # plain form: every scene alone through the single-fluid simulator
+    k = max(5, steps // 4)
+    total_ms = 0.0
+    for sc in ens.scenes:
+        sim = lib.Simulator(sc, fused=True, use_graph=True)
+        try:
+            sim.step(2)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.step(k)
+            sim.synchronize()
+            total_ms += 1e3 * (time.perf_counter() - t0) / k
+        finally:
+            sim.close()
+    out["plain_per_scene"] = {"ms_per_step": total_ms, "value": ens.n / (total_ms * 1e-3), "steps": k}
+    out["info"] = info
+    return out
+
+
 def step_bytes(scene, c_occ):
     bodies = scene.info.get("bodies")
     m = bodies.m if bodies is not None else 0
@@ -141,6 +194,8 @@ def main():
     ap.add_argument("--no-8m", action="store_true")
     ap.add_argument("--unfused", action="store_true")
     ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble line")
+    ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB ten times (reference for HBM byte counters)")
     a = ap.parse_args()
@@ -236,6 +291,25 @@ def main():
             "particles": s8.n, "steps": k8, "value": s8.n * k8 / r8["seconds"], "ms_per_step": r8["ms_per_step"],
             "step_hbm_frac": step_bytes(s8, r8["c_occ"]) / (r8["ms_per_step"] * 1e-3) / HBM_PEAK,
             "stage_ms": {k: round(v, 5) for k, v in r8["stage_ms"].items()},
+        }
+    if not a.no_batch:
+        ens = scenes.ensemble(num_scenes=a.batch_scenes)
+        log(f"ensemble: {len(ens.scenes)} scenes, n={ens.n:,}")
+        kb = max(20, a.steps // 2)
+        rb = measure_batched(ens, kb, max(5, a.warmup // 2))
+        plain = out["value"]
+        out["config"]["batched"] = {
+            "workload": f"{len(ens.scenes)} independent FP32 dam-break scenes, {ens.n:,} particles, "
+                        f"{ens.num_buckets:,} buckets; counts log-uniform in [4096, 65536], edges in [0.5, 1.5] m, "
+                        "fluid only",
+            "steps": kb,
+            "value_fixed": rb["fixed"]["value"], "value_tuned": rb["tuned"]["value"],
+            "ms_per_step_fixed": rb["fixed"]["ms_per_step"], "ms_per_step_tuned": rb["tuned"]["ms_per_step"],
+            "tuning": rb["tuning"], "stage_ms": rb["stage_ms"], "finite": rb["finite"],
+            "plain_per_scene_value": rb["plain_per_scene"]["value"],
+            "plain_per_scene_ms_per_step": rb["plain_per_scene"]["ms_per_step"],
+            "speedup_vs_plain_per_scene": rb["tuned"]["value"] / rb["plain_per_scene"]["value"],
+            "speedup_vs_plain_1M": rb["tuned"]["value"] / plain,
         }
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
