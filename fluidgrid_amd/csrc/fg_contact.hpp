@@ -7,21 +7,19 @@
 namespace fg {
 #pragma clang fp contract(off)
 
-// ProcessCellForcesD, K_DeviceFunctions.cu:32-106.  For each sphere j of the
-// bucket within r_i + r_j + eps: F += (ks*l - kd*dot(n, v_j - v_i)) * n with
+// ProcessCellForcesD, K_DeviceFunctions.cu:32-106, over the sorted range
+// [s0, e_raw) of one bucket.  For each sphere j of the bucket within r_i + r_j + eps: F += (ks*l - kd*dot(n, v_j - v_i)) * n with
 // n = (x_j - x_i)/l, l = |x_j - x_i| (the spring term uses the distance).
 // Velocities: per body (indexed by w) when w_index, else per sphere; none
 // when vel is null.
-__device__ __forceinline__ F3 contact_cell(const float4& sp, const float4& sv, float sr, float cr,
-                                           const float4* __restrict__ cpos, const float4* __restrict__ cvel,
-                                           uint32_t key, const uint32_t* __restrict__ start,
-                                           const uint32_t* __restrict__ end, float ks, float kd, bool w_index,
-                                           bool group, int w_group, float eps, uint32_t limit, uint32_t vlimit) {
+__device__ __forceinline__ F3 contact_range(const float4& sp, const float4& sv, float sr, float cr,
+                                            const float4* __restrict__ cpos, const float4* __restrict__ cvel,
+                                            uint32_t s0, uint32_t e_raw, float ks, float kd, bool w_index,
+                                            bool group, int w_group, float eps, uint32_t limit, uint32_t vlimit) {
     F3 F = f3(0.0f, 0.0f, 0.0f);
     const float mind = sr + cr + eps;
-    const uint32_t s0 = start[key];
     if (s0 != kEmpty) {
-        const uint32_t e = min(end[key], limit);
+        const uint32_t e = min(e_raw, limit);
         for (uint32_t i = s0; i < e; i++) {
             const float4 pos = cpos[i];
             float4 vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -42,9 +40,24 @@ __device__ __forceinline__ F3 contact_cell(const float4& sp, const float4& sv, f
     return F;
 }
 
+// Bucket of stencil offset (x,y,z) around sp, or kEmpty for a LINEAR cell
+// outside the grid.
+__device__ __forceinline__ uint32_t contact_key(const float4& sp, const CellXYZ& c, int x, int y, int z,
+                                                const GridDesc& g) {
+    if (g.mode == FG_KEY_REF_XOR) {
+        const float ox = g.cs.x * (float)x, oy = g.cs.y * (float)y, oz = g.cs.z * (float)z;
+        return hash_ref_xor(sp.x + ox, sp.y + oy, sp.z + oz, g.cs.x, g.cs.y, g.cs.z, g.nb);
+    }
+    const int a = c.x + x, b = c.y + y, d = c.z + z;
+    if (a < 0 || b < 0 || d < 0 || a >= g.dims.x || b >= g.dims.y || d >= g.dims.z) return kEmpty;
+    return grid_linear_key(a, b, d, g);
+}
+
 // ProcessCollisionForcesD, K_DeviceFunctions.cu:113-159: the (2r+1)^3
 // stencil in z, y, x order; REF_XOR grids rehash x + k*cs, LINEAR grids
-// step cell coordinates and skip cells outside the grid.
+// step cell coordinates and skip cells outside the grid.  For the usual
+// relation (1,1,1) the 27 cell starts are loaded together before any cell
+// is walked (same cells, same order, same sums).
 __device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv, float sr,
                                               const float4* __restrict__ cpos, const float4* __restrict__ cvel,
                                               float cr, float ks, float kd, int3 rel,
@@ -54,20 +67,27 @@ __device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv
     F3 F = f3(0.0f, 0.0f, 0.0f);
     CellXYZ c{0, 0, 0};
     if (g.mode == FG_KEY_LINEAR) c = grid_cell(sp.x, sp.y, sp.z, g);
+    if (rel.x == 1 && rel.y == 1 && rel.z == 1) {
+        uint32_t keys[27], s0[27];
+#pragma unroll
+        for (int k = 0; k < 27; k++) keys[k] = contact_key(sp, c, k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1, g);
+#pragma unroll
+        for (int k = 0; k < 27; k++) s0[k] = keys[k] != kEmpty ? start[keys[k]] : kEmpty;
+#pragma unroll
+        for (int k = 0; k < 27; k++)
+            if (keys[k] != kEmpty)  // an empty bucket adds +0, as in the reference
+                F = add(F, contact_range(sp, sv, sr, cr, cpos, cvel, s0[k], s0[k] != kEmpty ? end[keys[k]] : 0u, ks,
+                                         kd, w_index, group, w_group, eps, limit, vlimit));
+        return F;
+    }
     for (int z = -rel.z; z <= rel.z; z++)
         for (int y = -rel.y; y <= rel.y; y++)
             for (int x = -rel.x; x <= rel.x; x++) {
-                uint32_t key;
-                if (g.mode == FG_KEY_REF_XOR) {
-                    const float ox = g.cs.x * (float)x, oy = g.cs.y * (float)y, oz = g.cs.z * (float)z;
-                    key = hash_ref_xor(sp.x + ox, sp.y + oy, sp.z + oz, g.cs.x, g.cs.y, g.cs.z, g.nb);
-                } else {
-                    const int a = c.x + x, b = c.y + y, d = c.z + z;
-                    if (a < 0 || b < 0 || d < 0 || a >= g.dims.x || b >= g.dims.y || d >= g.dims.z) continue;
-                    key = grid_linear_key(a, b, d, g);
-                }
-                F = add(F, contact_cell(sp, sv, sr, cr, cpos, cvel, key, start, end, ks, kd, w_index, group, w_group,
-                                        eps, limit, vlimit));
+                const uint32_t key = contact_key(sp, c, x, y, z, g);
+                if (key == kEmpty) continue;
+                const uint32_t s0 = start[key];
+                F = add(F, contact_range(sp, sv, sr, cr, cpos, cvel, s0, s0 != kEmpty ? end[key] : 0u, ks, kd, w_index,
+                                         group, w_group, eps, limit, vlimit));
             }
     return F;
 }

@@ -96,14 +96,30 @@ __device__ __forceinline__ int32_t f2i_sat(float v) {
     return (int32_t)v;
 }
 
+// Correctly rounded IEEE single operations for the reference-arithmetic
+// code.  They are written as plain operators in a region with contraction
+// off, so no multiply is ever fused into a following add, even after
+// inlining into code compiled with contraction on (the HIP header's
+// __fadd_rn/__fmul_rn are plain operators compiled with contraction
+// allowed, and its __fsqrt_rn is the hardware approximation).  Division and
+// __builtin_sqrtf are correctly rounded under HIP's default single-precision
+// division/square-root mode.
+#pragma clang fp contract(off)
+__device__ __forceinline__ float rn_add(float a, float b) { return a + b; }
+__device__ __forceinline__ float rn_sub(float a, float b) { return a - b; }
+__device__ __forceinline__ float rn_mul(float a, float b) { return a * b; }
+__device__ __forceinline__ float rn_div(float a, float b) { return a / b; }
+__device__ __forceinline__ float rn_sqrt(float a) { return __builtin_sqrtf(a); }
+#pragma clang fp contract(fast)
+
 // REF_XOR key: K_DeviceFunctions.cu:17-26.  The add and divide are IEEE
-// single operations (hipcc rounds float division correctly by default), so
-// keys are bit-identical to a host evaluation of the same expression.
+// single operations, so keys are bit-identical to a host evaluation of the
+// same expression.
 __device__ __forceinline__ uint32_t hash_ref_xor(float px, float py, float pz, float csx, float csy,
                                                  float csz, uint32_t table) {
-    int32_t x = f2i_sat(__fdiv_rn(__fadd_rn(px, 5000.0f), csx));
-    int32_t y = f2i_sat(__fdiv_rn(__fadd_rn(py, 5000.0f), csy));
-    int32_t z = f2i_sat(__fdiv_rn(__fadd_rn(pz, 5000.0f), csz));
+    int32_t x = f2i_sat(rn_div(rn_add(px, 5000.0f), csx));
+    int32_t y = f2i_sat(rn_div(rn_add(py, 5000.0f), csy));
+    int32_t z = f2i_sat(rn_div(rn_add(pz, 5000.0f), csz));
     uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
     return h % table;
 }
@@ -112,7 +128,7 @@ __device__ __forceinline__ uint32_t hash_ref_xor(float px, float py, float pz, f
 // float before the conversion gives the same answer as converting with
 // saturation first (NaN maps to 0 in both).
 __device__ __forceinline__ int cell_coord(float p, float vmin, float cs, int n) {
-    float q = floorf(__fdiv_rn(__fsub_rn(p, vmin), cs));
+    float q = floorf(rn_div(rn_sub(p, vmin), cs));
     q = fminf(fmaxf(q, 0.0f), (float)(n - 1));
     return (int)q;
 }
@@ -155,15 +171,15 @@ __device__ __forceinline__ uint32_t dev_num_buckets(const fg_common_fluid_params
 #pragma clang fp contract(off)
 struct F3 { float x, y, z; };
 __device__ __forceinline__ F3 f3(float x, float y, float z) { return {x, y, z}; }
-__device__ __forceinline__ F3 add(F3 a, F3 b) { return {__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y), __fadd_rn(a.z, b.z)}; }
-__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y), __fsub_rn(a.z, b.z)}; }
-__device__ __forceinline__ F3 mul(F3 a, float s) { return {__fmul_rn(a.x, s), __fmul_rn(a.y, s), __fmul_rn(a.z, s)}; }
-__device__ __forceinline__ F3 mul(float s, F3 a) { return {__fmul_rn(s, a.x), __fmul_rn(s, a.y), __fmul_rn(s, a.z)}; }
-__device__ __forceinline__ F3 divs(F3 a, float s) { return {__fdiv_rn(a.x, s), __fdiv_rn(a.y, s), __fdiv_rn(a.z, s)}; }
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return {rn_add(a.x, b.x), rn_add(a.y, b.y), rn_add(a.z, b.z)}; }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {rn_sub(a.x, b.x), rn_sub(a.y, b.y), rn_sub(a.z, b.z)}; }
+__device__ __forceinline__ F3 mul(F3 a, float s) { return {rn_mul(a.x, s), rn_mul(a.y, s), rn_mul(a.z, s)}; }
+__device__ __forceinline__ F3 mul(float s, F3 a) { return {rn_mul(s, a.x), rn_mul(s, a.y), rn_mul(s, a.z)}; }
+__device__ __forceinline__ F3 divs(F3 a, float s) { return {rn_div(a.x, s), rn_div(a.y, s), rn_div(a.z, s)}; }
 __device__ __forceinline__ float dot(F3 a, F3 b) {
-    return __fadd_rn(__fadd_rn(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y)), __fmul_rn(a.z, b.z));
+    return rn_add(rn_add(rn_mul(a.x, b.x), rn_mul(a.y, b.y)), rn_mul(a.z, b.z));
 }
-__device__ __forceinline__ float len(F3 a) { return __fsqrt_rn(dot(a, a)); }
+__device__ __forceinline__ float len(F3 a) { return rn_sqrt(dot(a, a)); }
 __device__ __forceinline__ F3 xyz(float4 p) { return {p.x, p.y, p.z}; }
 #pragma clang fp contract(fast)
 

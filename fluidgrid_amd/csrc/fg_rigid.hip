@@ -118,24 +118,26 @@ __global__ void k_integrate_rb(float4* __restrict__ pos, const float* __restrict
 // workgroup per body; each lane sums a strided subset in index order, then a
 // fixed tree combines the lanes, so the result does not depend on timing.
 constexpr int kReduceThreads = 256;
+constexpr int kBodySumThreads = 1024;  // 20K spheres: 20 loads per lane, unrolled
 
-__global__ void __launch_bounds__(kReduceThreads)
+__global__ void __launch_bounds__(kBodySumThreads)
     k_body_reduce_f64(const float4* __restrict__ f, const float4* __restrict__ t, const uint32_t* __restrict__ offset,
                       uint32_t nbodies, double4* __restrict__ out_f, double4* __restrict__ out_t,
                       float4* __restrict__ acc_f, float4* __restrict__ acc_t) {
-    __shared__ double s[6][kReduceThreads];
+    __shared__ double s[6][kBodySumThreads];
     const uint32_t b = blockIdx.x;
     if (b >= nbodies) return;
     const uint32_t lo = offset[b], hi = offset[b + 1];
     double a[6] = {0, 0, 0, 0, 0, 0};
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kReduceThreads) {
+#pragma unroll 4
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBodySumThreads) {
         const float4 fi = f[i], ti = t[i];
         a[0] += fi.x; a[1] += fi.y; a[2] += fi.z;
         a[3] += ti.x; a[4] += ti.y; a[5] += ti.z;
     }
     for (int k = 0; k < 6; k++) s[k][threadIdx.x] = a[k];
     __syncthreads();
-    for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
+    for (int w = kBodySumThreads / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
             for (int k = 0; k < 6; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
         __syncthreads();
@@ -429,7 +431,7 @@ int fg_body_reduce_f64(const fg_float4* forces, const fg_float4* torques, const 
         set_error(FG_ERR_INVALID_ARGUMENT, "fg_body_reduce_f64: null pointer");
         return FG_ERR_INVALID_ARGUMENT;
     }
-    hipLaunchKernelGGL(k_body_reduce_f64, dim3(numBodies), dim3(kReduceThreads), 0, state().stream,
+    hipLaunchKernelGGL(k_body_reduce_f64, dim3(numBodies), dim3(kBodySumThreads), 0, state().stream,
                        reinterpret_cast<const float4*>(forces), reinterpret_cast<const float4*>(torques), bodyOffsets,
                        numBodies, reinterpret_cast<double4*>(outForces), reinterpret_cast<double4*>(outTorques),
                        accForces ? FG_F4(accForces) : nullptr, accTorques ? FG_F4(accTorques) : nullptr);

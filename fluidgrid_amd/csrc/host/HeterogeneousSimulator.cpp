@@ -69,14 +69,20 @@ HeterogeneousSimulator::HeterogeneousSimulator() {
 
     hip_check(hipStreamCreateWithFlags(&mStream, hipStreamNonBlocking), "hipStreamCreate");
     mOwnStream = true;
+    hip_check(hipStreamCreateWithFlags(&mBodyStream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : mEv) hip_check(hipEventCreate(&e), "hipEventCreate");
+    for (auto& e : mSync) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
 }
 
 HeterogeneousSimulator::~HeterogeneousSimulator() {
     invalidateGraph();
     if (mStream) (void)hipStreamSynchronize(mStream);
+    if (mBodyStream) (void)hipStreamSynchronize(mBodyStream);
     for (auto& e : mEv)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : mSync)
+        if (e) (void)hipEventDestroy(e);
+    if (mBodyStream) (void)hipStreamDestroy(mBodyStream);
     if (mOwnStream && mStream) (void)hipStreamDestroy(mStream);
 }
 
@@ -196,6 +202,53 @@ void HeterogeneousSimulator::integrateFluidForces(Fluid::GPUSPHFluid* fluid) {
                   fluid->getHashStart(), fluid->getHashEnd());
 }
 
+// Coupled step with the body work on a second stream (one fluid, no stage
+// timing).  Dependencies, in the order of HeterogeneousSimulator.cpp:54-127:
+//   bodies:  world transform, body grid, bounds        (needs last step's body state)
+//   fluid:   hash, sort, cells                         -> [cells]
+//   fluid:   density
+//   bodies:  forces from the sorted fluid              (after [cells])
+//   fluid:   coupled forces + integration              (after the body grid)
+//   bodies:  FP64 sums; integration                    (after the fluid forces,
+//            which read the body velocities)
+// and the fluid stream waits for the body stream at the end, so that one
+// step, and a graph captured from it, is complete when the fluid stream is.
+void HeterogeneousSimulator::enqueueCoupledOverlapped(Fluid::GPUSPHFluid* f) {
+    RigidBody::RigidBodySimulator* rb = RigidBody::RigidBodySimulator::instance();
+    enum { kFork, kBodyGrid, kCells, kForces, kJoin };
+    rb->startInteraction();
+    f->beginUpdate();
+    hip_check(hipEventRecord(mSync[kFork], mStream), "event");
+    hip_check(hipStreamWaitEvent(mBodyStream, mSync[kFork], 0), "wait");
+    fg_set_stream(mBodyStream);
+    rb->setStream(mBodyStream);
+    rb->generateCollisionData();
+    hip_check(hipEventRecord(mSync[kBodyGrid], mBodyStream), "event");
+    fg_set_stream(mStream);
+    findFluidNeighbourhood(f);
+    hip_check(hipEventRecord(mSync[kCells], mStream), "event");
+    calculateFluidDensity(f);
+    hip_check(hipStreamWaitEvent(mBodyStream, mSync[kCells], 0), "wait");
+    fg_set_stream(mBodyStream);
+    rb->calculateRigidBodyForcesFromFluid(f->getOrderedPosition(), f->getOrderedVelocities(), f->getHashStart(),
+                                          f->getHashEnd(), (unsigned)f->getNumParticles());
+    rb->applyForces();
+    fg_set_stream(mStream);
+    hip_check(hipStreamWaitEvent(mStream, mSync[kBodyGrid], 0), "wait");
+    calculateFluidForces(f, false, true);
+    if (!mOptions.fused) integrateFluidForces(f);
+    hip_check(hipEventRecord(mSync[kForces], mStream), "event");
+    hip_check(hipStreamWaitEvent(mBodyStream, mSync[kForces], 0), "wait");
+    fg_set_stream(mBodyStream);
+    rb->integrate();
+    hip_check(hipEventRecord(mSync[kJoin], mBodyStream), "event");
+    fg_set_stream(mStream);
+    rb->setStream(mStream);
+    hip_check(hipStreamWaitEvent(mStream, mSync[kJoin], 0), "wait");
+    f->endUpdate();
+    rb->endInteraction();
+}
+
 // Events: 0 start, 1 hash, 2 sort, 3 cells, 4 density, 5 bodies (collision
 // data + forces from the fluid), 6 fluid forces, 7 fluid integrate, 8 body
 // sums and integration.
@@ -205,6 +258,10 @@ void HeterogeneousSimulator::enqueueStep() {
     const bool withBodies = coupled();
     if (withBodies && mFluids.size() != 1)
         throw std::runtime_error("HeterogeneousSimulator: rigid-body coupling supports exactly one fluid");
+    if (withBodies && !t && mOptions.overlap_bodies) {
+        enqueueCoupledOverlapped(mFluids[0]);
+        return;
+    }
     if (withBodies) rb->startInteraction();
     for (Fluid::GPUSPHFluid* f : mFluids) {
         f->beginUpdate();

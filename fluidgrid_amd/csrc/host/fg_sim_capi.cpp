@@ -84,10 +84,18 @@ int fg_sim_add_fluid(const fg_fluid_params* params, const fg_float4* positions, 
 
 int fg_sim_set_options(int fused, int use_graph, int stage_timing) {
     if (!sim()) return FG_ERR_NOT_INITIALIZED;
-    Simulators::StepOptions o;
+    Simulators::StepOptions o = Simulators::HeterogeneousSimulator::instance()->options();
     o.fused = fused != 0;
     o.use_graph = use_graph != 0;
     o.stage_timing = stage_timing != 0;
+    Simulators::HeterogeneousSimulator::instance()->setOptions(o);
+    return FG_OK;
+}
+
+int fg_sim_set_overlap(int overlap_bodies) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    Simulators::StepOptions o = Simulators::HeterogeneousSimulator::instance()->options();
+    o.overlap_bodies = overlap_bodies != 0;
     Simulators::HeterogeneousSimulator::instance()->setOptions(o);
     return FG_OK;
 }

@@ -75,6 +75,7 @@ SIGNATURES = {
     "fg_sim_init": (i32, [C.POINTER(abi.CommonFluidParams), i32, i32]),
     "fg_sim_add_fluid": (i32, [C.POINTER(abi.FluidParams), vp, vp]),
     "fg_sim_set_options": (i32, [i32, i32, i32]),
+    "fg_sim_set_overlap": (i32, [i32]),
     "fg_sim_step": (i32, [i32]),
     "fg_sim_synchronize": (i32, []),
     "fg_sim_stream": (vp, []),
@@ -251,6 +252,9 @@ class Simulator:
 
     def set_options(self, fused=True, use_graph=True, stage_timing=False):
         check(load().fg_sim_set_options(int(fused), int(use_graph), int(stage_timing)), "options")
+
+    def set_overlap(self, on=True):
+        check(load().fg_sim_set_overlap(int(on)), "fg_sim_set_overlap")
 
     def step(self, steps=1):
         check(load().fg_sim_step(int(steps)), "fg_sim_step")

@@ -9,7 +9,9 @@
 //  * the sort uses the key width implied by the bucket count and a
 //    persistent workspace; forces and integration run as one pass when
 //    `fused` is set (bit-identical results);
-//  * doTimeSteps(n) can replay a captured HIP graph of one step.
+//  * doTimeSteps(n) can replay a captured HIP graph of one step;
+//  * with rigid bodies, the body work runs on a second stream beside the
+//    fluid's hash, sort and density (same dependencies, same results).
 #pragma once
 
 #include <vector>
@@ -25,6 +27,7 @@ struct StepOptions {
     bool fused = true;       // forces + integrate in one pass
     bool use_graph = true;   // doTimeSteps replays a captured step
     bool stage_timing = false;
+    bool overlap_bodies = true;  // body work on a second stream (results unchanged)
 };
 
 struct StageTimes {
@@ -58,6 +61,7 @@ private:
     void calculateFluidForces(Fluid::GPUSPHFluid* fluid, bool useStaticRB, bool useDynamicRB);
     void integrateFluidForces(Fluid::GPUSPHFluid* fluid);
     void enqueueStep();
+    void enqueueCoupledOverlapped(Fluid::GPUSPHFluid* fluid);
 
     std::vector<Fluid::GPUSPHFluid*> mFluids;
     StepOptions mOptions;
@@ -66,6 +70,8 @@ private:
     hipGraph_t mGraph = nullptr;
     hipGraphExec_t mGraphExec = nullptr;
     hipEvent_t mEv[9] = {};
+    hipStream_t mBodyStream = nullptr;  // body work of a coupled step
+    hipEvent_t mSync[5] = {};           // fork/join of the two streams
     bool coupled() const;
     StageTimes mTimes;
     static HeterogeneousSimulator* mInstance;

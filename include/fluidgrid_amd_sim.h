@@ -37,6 +37,9 @@ int fg_sim_init(const fg_common_fluid_params* common, int key_mode, int math_mod
  * negated FG_ERR_* code. */
 int fg_sim_add_fluid(const fg_fluid_params* params, const fg_float4* positions, const fg_float4* velocities);
 int fg_sim_set_options(int fused, int use_graph, int stage_timing);
+/* With rigid bodies: run the body work on a second stream beside the fluid
+ * stages (default on; results are the same either way). */
+int fg_sim_set_overlap(int overlap_bodies);
 int fg_sim_step(int steps);                       /* asynchronous */
 int fg_sim_synchronize(void);
 void* fg_sim_stream(void);                        /* hipStream_t of the simulator */

@@ -255,15 +255,17 @@ def test_reference_reduction_helpers(fglib):
 
 
 def test_coupled_simulator_runs_and_is_graph_stable(fglib):
-    """The config-1 torus scene coupled to a small fluid block: graph replay
-    equals eager stepping bit for bit, the contact sums are non-zero once the
-    torus reaches the fluid, and the state stays finite."""
+    """A torus submerged in a small fluid block: eager stepping on one stream,
+    eager stepping with the body work on a second stream, and graph replay of
+    the two-stream step agree bit for bit; the contact sums are non-zero and
+    the state stays finite."""
     s, b = _fluid_and_torus(inside=True)
     res = []
-    for graph in (False, True):
+    for graph, overlap in ((False, False), (False, True), (True, True)):
         sim = fglib.Simulator(s, fused=True, use_graph=graph)
         try:
             sim.add_bodies(b)
+            sim.set_overlap(overlap)
             sim.step(10)
             pos, vel = sim.state()
             bs = sim.body_state()
@@ -271,10 +273,12 @@ def test_coupled_simulator_runs_and_is_graph_stable(fglib):
             res.append((pos, vel, bs, sums))
         finally:
             sim.close()
-    (p0, v0, b0, s0), (p1, v1, b1, s1) = res
-    assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
-    for k in b0:
-        assert np.array_equal(b0[k], b1[k]), k
+    p0, v0, b0, s0 = res[0]
+    for p1, v1, b1, s1 in res[1:]:
+        assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
+        for k in b0:
+            assert np.array_equal(b0[k], b1[k]), k
+        assert np.array_equal(s0[0], s1[0]) and np.array_equal(s0[1], s1[1])
     assert np.all(np.isfinite(p0)) and np.all(np.isfinite(b0["pos"]))
     assert np.linalg.norm(s0[0][0, :3]) > 0
 
