@@ -68,40 +68,47 @@ __global__ void __launch_bounds__(BS)
     });
 }
 
-template <int BS>
+template <int BS, bool FLAT>
 __global__ void __launch_bounds__(BS)
     k_batch_forces(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes, const uint32_t* __restrict__ scene_of,
                    const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                    const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
                    const unsigned long long* __restrict__ masks, float4* __restrict__ vout,
                    float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n) {
+    __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
+    __shared__ unsigned long long s_mm[FLAT ? 9 * BS : 1];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
     const Contacts none{};
+    FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
         const fg_scene_desc& d = scenes[s];
         forces_particle<true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, d.mass, d.viscosity, d.common,
-                              d.kernels, d.keyBase, masks, none);
+                              d.kernels, d.keyBase, masks, none, FLAT ? &fr : nullptr);
     });
 }
 
+#define FG_BATCH_LAUNCH_BS(KERN, BS, n, ...) \
+    hipLaunchKernelGGL(KERN, dim3(grid_for(n, BS)), dim3(BS), 0, state().stream, __VA_ARGS__)
+
+// Launches KERN<BS> (or KERN<BS, FLAT>) with BS = the tuned SPH block size.
 #define FG_BATCH_LAUNCH(KERN, n, ...)                                                                       \
     do {                                                                                                    \
         switch (state().sph_block) {                                                                        \
-            case 64:                                                                                        \
-                hipLaunchKernelGGL(KERN<64>, dim3(grid_for(n, 64)), dim3(64), 0, state().stream, __VA_ARGS__); \
-                break;                                                                                      \
-            case 128:                                                                                       \
-                hipLaunchKernelGGL(KERN<128>, dim3(grid_for(n, 128)), dim3(128), 0, state().stream,         \
-                                   __VA_ARGS__);                                                            \
-                break;                                                                                      \
-            case 512:                                                                                       \
-                hipLaunchKernelGGL(KERN<512>, dim3(grid_for(n, 512)), dim3(512), 0, state().stream,         \
-                                   __VA_ARGS__);                                                            \
-                break;                                                                                      \
-            default:                                                                                        \
-                hipLaunchKernelGGL(KERN<256>, dim3(grid_for(n, 256)), dim3(256), 0, state().stream,         \
-                                   __VA_ARGS__);                                                            \
+            case 64: FG_BATCH_LAUNCH_BS((KERN<64>), 64, n, __VA_ARGS__); break;                             \
+            case 128: FG_BATCH_LAUNCH_BS((KERN<128>), 128, n, __VA_ARGS__); break;                          \
+            case 512: FG_BATCH_LAUNCH_BS((KERN<512>), 512, n, __VA_ARGS__); break;                          \
+            default: FG_BATCH_LAUNCH_BS((KERN<256>), 256, n, __VA_ARGS__);                                 \
+        }                                                                                                   \
+    } while (0)
+
+#define FG_BATCH_LAUNCH2(KERN, FLAT, n, ...)                                                                \
+    do {                                                                                                    \
+        switch (state().sph_block) {                                                                        \
+            case 64: FG_BATCH_LAUNCH_BS((KERN<64, FLAT>), 64, n, __VA_ARGS__); break;                       \
+            case 128: FG_BATCH_LAUNCH_BS((KERN<128, FLAT>), 128, n, __VA_ARGS__); break;                    \
+            case 512: FG_BATCH_LAUNCH_BS((KERN<512, FLAT>), 512, n, __VA_ARGS__); break;                    \
+            default: FG_BATCH_LAUNCH_BS((KERN<256, FLAT>), 256, n, __VA_ARGS__);                           \
         }                                                                                                   \
     } while (0)
 
@@ -152,9 +159,14 @@ int fg_batch_forces_integrate_nbr(const fg_scene_desc* scenes, unsigned int numS
         set_error(FG_ERR_INVALID_ARGUMENT, "fg_batch_forces_integrate_nbr: vel must not alias orderedVelocity");
         return FG_ERR_INVALID_ARGUMENT;
     }
-    FG_BATCH_LAUNCH(k_batch_forces, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
-                    FG_F4(orderedVelocity), FG_F2(density_pressure), gridStart, gridEnd, neighborMasks, FG_F4(vel),
-                    pos ? FG_F4(pos) : nullptr, particleId, numParticles);
+    if (state().flat_forces)
+        FG_BATCH_LAUNCH2(k_batch_forces, true, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
+                         FG_F4(orderedVelocity), FG_F2(density_pressure), gridStart, gridEnd, neighborMasks,
+                         FG_F4(vel), pos ? FG_F4(pos) : nullptr, particleId, numParticles);
+    else
+        FG_BATCH_LAUNCH2(k_batch_forces, false, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
+                         FG_F4(orderedVelocity), FG_F2(density_pressure), gridStart, gridEnd, neighborMasks,
+                         FG_F4(vel), pos ? FG_F4(pos) : nullptr, particleId, numParticles);
     return check_launch("fg_batch_forces_integrate_nbr") ? FG_OK : fg_peek_error();
 }
 

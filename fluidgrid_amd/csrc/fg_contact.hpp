@@ -56,8 +56,10 @@ __device__ __forceinline__ uint32_t contact_key(const float4& sp, const CellXYZ&
 // ProcessCollisionForcesD, K_DeviceFunctions.cu:113-159: the (2r+1)^3
 // stencil in z, y, x order; REF_XOR grids rehash x + k*cs, LINEAR grids
 // step cell coordinates and skip cells outside the grid.  For the usual
-// relation (1,1,1) the 27 cell starts are loaded together before any cell
-// is walked (same cells, same order, same sums).
+// relation (1,1,1) and PREFETCH the 27 cell starts are loaded together
+// before any cell is walked (same cells, same order, same sums; more
+// registers, so it is used by the body-sphere pass only).
+template <bool PREFETCH>
 __device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv, float sr,
                                               const float4* __restrict__ cpos, const float4* __restrict__ cvel,
                                               float cr, float ks, float kd, int3 rel,
@@ -67,7 +69,7 @@ __device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv
     F3 F = f3(0.0f, 0.0f, 0.0f);
     CellXYZ c{0, 0, 0};
     if (g.mode == FG_KEY_LINEAR) c = grid_cell(sp.x, sp.y, sp.z, g);
-    if (rel.x == 1 && rel.y == 1 && rel.z == 1) {
+    if (PREFETCH && rel.x == 1 && rel.y == 1 && rel.z == 1) {
         uint32_t keys[27], s0[27];
 #pragma unroll
         for (int k = 0; k < 27; k++) keys[k] = contact_key(sp, c, k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1, g);
@@ -100,7 +102,7 @@ __device__ __forceinline__ F3 fluid_body_contact(const float4& pos, const float4
         if (pos.x < lo.x || pos.y < lo.y || pos.z < lo.z || pos.x > hi.x || pos.y > hi.y || pos.z > hi.z)
             return f3(0.0f, 0.0f, 0.0f);
     }
-    return contact_stencil(pos, vel, bc.fluid_radius, bc.pos, bc.vel, bc.body_radius, bc.ks, bc.kd, bc.relation,
+    return contact_stencil<false>(pos, vel, bc.fluid_radius, bc.pos, bc.vel, bc.body_radius, bc.ks, bc.kd, bc.relation,
                            bc.start, bc.end, bc.grid, bc.vel != nullptr, false, 0, bc.eps, bc.count, bc.nvel);
 }
 

@@ -31,6 +31,9 @@ struct State {
     // fast density/force kernels, and digit bits per radix-sort pass.
     int sph_block = 256;
     int sort_digit_bits = 8;
+    // Force-pass schedule of the masked fast path: flat neighbour stream per
+    // lane (true) or row by row (false); results are identical.
+    bool flat_forces = true;
 };
 
 State& state();

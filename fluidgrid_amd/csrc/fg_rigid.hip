@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(256)
     const bool ok = b >= 0 && (uint32_t)b < nbodies;
     const float4 vel = ok ? rb_vel[b] : make_float4(0, 0, 0, 0);
     const float4 cm = ok ? rb_cm[b] : make_float4(0, 0, 0, 0);
-    const F3 F = contact_stencil(pos, vel, radius, sph_pos, sph_vel, sph_radius, ks, kd, relation, sph_start, sph_end,
+    const F3 F = contact_stencil<true>(pos, vel, radius, sph_pos, sph_vel, sph_radius, ks, kd, relation, sph_start, sph_end,
                                  g, false, false, 0, 0.0001f, nsph, nsph);
     out_f[i] = make_float4(F.x, F.y, F.z, (float)b);
     const float4 dir = make_float4(pos.x - cm.x, pos.y - cm.y, pos.z - cm.z, pos.w - cm.w);

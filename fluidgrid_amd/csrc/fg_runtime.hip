@@ -130,6 +130,13 @@ int fg_set_tuning(int sphBlock, int sortDigitBits) {
     return FG_OK;
 }
 
+int fg_set_force_schedule(int flat) {
+    state().flat_forces = flat != 0;
+    return FG_OK;
+}
+
+int fg_get_force_schedule(void) { return state().flat_forces ? 1 : 0; }
+
 int fg_get_tuning(int* sphBlock, int* sortDigitBits) {
     if (sphBlock) *sphBlock = state().sph_block;
     if (sortDigitBits) *sortDigitBits = state().sort_digit_bits;

@@ -126,8 +126,9 @@ __device__ __forceinline__ F3 bucket_internal_ref(uint32_t i, uint32_t s0, uint3
     return total;
 }
 
-// K_FluidSim.cu:260-309 (+ fused integrate when pout != nullptr)
-template <int MODE>
+// K_FluidSim.cu:260-309 (+ fused integrate when pout != nullptr); with
+// COUPLED, K_FluidSim.cu:315-413.
+template <int MODE, bool COUPLED>
 __global__ void __launch_bounds__(kSphThreads)
     k_forces_ref(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                  const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, float4* __restrict__ vout,
@@ -155,7 +156,7 @@ __global__ void __launch_bounds__(kSphThreads)
                                                                  cp.simulation_scale, cp.kernel_radius,
                                                                  kc.kSpikyGradient, kc.kViscosityLaplacian, mu));
                 }
-        a = finish_accel(internal, m, pos, vel, cp, bc);
+        a = finish_accel<COUPLED>(internal, m, pos, vel, cp, bc);
     }
     write_back(id[i], n, pos, vel, a, cp, vout, pout);
 }
@@ -188,7 +189,7 @@ __global__ void __launch_bounds__(BS)
     density_particle<MASKS>(i, sp, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks);
 }
 
-template <bool MASKS, int BS>
+template <bool MASKS, bool COUPLED, int BS>
 __global__ void __launch_bounds__(BS)
     k_forces_fast(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, float4* __restrict__ vout,
@@ -197,7 +198,24 @@ __global__ void __launch_bounds__(BS)
                   Contacts bc) {
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
-    forces_particle<MASKS>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u, masks, bc);
+    forces_particle<MASKS, COUPLED>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u, masks, bc);
+}
+
+// Flat schedule (row table in LDS); same results as k_forces_fast<true, COUPLED>.
+template <bool COUPLED, int BS>
+__global__ void __launch_bounds__(BS)
+    k_forces_flat(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
+                  const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, float4* __restrict__ vout,
+                  float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
+                  fg_common_fluid_params cp, fg_kernels_cache kc, const unsigned long long* __restrict__ masks,
+                  Contacts bc) {
+    __shared__ uint32_t s_rb[9 * BS];
+    __shared__ unsigned long long s_mm[9 * BS];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (i >= n) return;
+    FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
+    forces_particle<true, COUPLED>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u, masks, bc,
+                                   &fr);
 }
 
 // ---------------------------------------------------------------------------
@@ -214,25 +232,27 @@ static bool have_params(const char* who) {
 
 static bool use_fast() { return state().key_mode == FG_KEY_LINEAR && state().math_mode == FG_MATH_FAST; }
 
-// Launches kernel template KERN<MASKS, BS> with BS = the tuned SPH block size.
-#define FG_SPH_LAUNCH(KERN, MASKS, n, ...)                                                                  \
+#define FG_SPH_LAUNCH_BS(KERN, BS, n, ...) \
+    hipLaunchKernelGGL(KERN, dim3(grid_for(n, BS)), dim3(BS), 0, state().stream, __VA_ARGS__)
+
+// Launch KERN<A, BS> / KERN<A, B, BS> with BS = the tuned SPH block size.
+#define FG_SPH_LAUNCH(KERN, A, n, ...)                                                                      \
     do {                                                                                                    \
         switch (state().sph_block) {                                                                        \
-            case 64:                                                                                        \
-                hipLaunchKernelGGL((KERN<MASKS, 64>), dim3(grid_for(n, 64)), dim3(64), 0, state().stream,   \
-                                   __VA_ARGS__);                                                            \
-                break;                                                                                      \
-            case 128:                                                                                       \
-                hipLaunchKernelGGL((KERN<MASKS, 128>), dim3(grid_for(n, 128)), dim3(128), 0, state().stream, \
-                                   __VA_ARGS__);                                                            \
-                break;                                                                                      \
-            case 512:                                                                                       \
-                hipLaunchKernelGGL((KERN<MASKS, 512>), dim3(grid_for(n, 512)), dim3(512), 0, state().stream, \
-                                   __VA_ARGS__);                                                            \
-                break;                                                                                      \
-            default:                                                                                        \
-                hipLaunchKernelGGL((KERN<MASKS, 256>), dim3(grid_for(n, 256)), dim3(256), 0, state().stream, \
-                                   __VA_ARGS__);                                                            \
+            case 64: FG_SPH_LAUNCH_BS((KERN<A, 64>), 64, n, __VA_ARGS__); break;                            \
+            case 128: FG_SPH_LAUNCH_BS((KERN<A, 128>), 128, n, __VA_ARGS__); break;                         \
+            case 512: FG_SPH_LAUNCH_BS((KERN<A, 512>), 512, n, __VA_ARGS__); break;                         \
+            default: FG_SPH_LAUNCH_BS((KERN<A, 256>), 256, n, __VA_ARGS__);                                \
+        }                                                                                                   \
+    } while (0)
+
+#define FG_SPH_LAUNCH2(KERN, A, B, n, ...)                                                                  \
+    do {                                                                                                    \
+        switch (state().sph_block) {                                                                        \
+            case 64: FG_SPH_LAUNCH_BS((KERN<A, B, 64>), 64, n, __VA_ARGS__); break;                         \
+            case 128: FG_SPH_LAUNCH_BS((KERN<A, B, 128>), 128, n, __VA_ARGS__); break;                      \
+            case 512: FG_SPH_LAUNCH_BS((KERN<A, B, 512>), 512, n, __VA_ARGS__); break;                      \
+            default: FG_SPH_LAUNCH_BS((KERN<A, B, 256>), 256, n, __VA_ARGS__);                             \
         }                                                                                                   \
     } while (0)
 
@@ -259,20 +279,45 @@ int launch_forces(float4* sp, float4* sv, float2* rho_p, uint32_t* start, uint32
                   const Contacts* bcp = nullptr) {
     Contacts bc{};
     if (bcp) bc = *bcp;
+    const bool coupled = bc.coupled != 0;
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
-    if (use_fast() && masks)
-        FG_SPH_LAUNCH(k_forces_fast, true, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, st.common,
-                      st.kernels, masks, bc);
-    else if (use_fast())
-        FG_SPH_LAUNCH(k_forces_fast, false, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, st.common,
-                      st.kernels, (const unsigned long long*)nullptr, bc);
-    else if (st.key_mode == FG_KEY_LINEAR)
-        hipLaunchKernelGGL(k_forces_ref<FG_KEY_LINEAR>, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout,
-                           pout, id, n, m, mu, st.common, st.kernels, bc);
-    else
-        hipLaunchKernelGGL(k_forces_ref<FG_KEY_REF_XOR>, grid, block, 0, st.stream, sp, sv, rho_p, start, end, vout,
-                           pout, id, n, m, mu, st.common, st.kernels, bc);
+    if (use_fast() && masks && st.flat_forces) {
+        if (coupled)
+            FG_SPH_LAUNCH(k_forces_flat, true, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, st.common,
+                          st.kernels, masks, bc);
+        else
+            FG_SPH_LAUNCH(k_forces_flat, false, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, st.common,
+                          st.kernels, masks, bc);
+    } else if (use_fast()) {
+        const unsigned long long* mk = masks;
+        if (coupled && mk)
+            FG_SPH_LAUNCH2(k_forces_fast, true, true, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu,
+                           st.common, st.kernels, mk, bc);
+        else if (coupled)
+            FG_SPH_LAUNCH2(k_forces_fast, false, true, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu,
+                           st.common, st.kernels, mk, bc);
+        else if (mk)
+            FG_SPH_LAUNCH2(k_forces_fast, true, false, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu,
+                           st.common, st.kernels, mk, bc);
+        else
+            FG_SPH_LAUNCH2(k_forces_fast, false, false, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu,
+                           st.common, st.kernels, mk, bc);
+    } else if (st.key_mode == FG_KEY_LINEAR) {
+        if (coupled)
+            hipLaunchKernelGGL((k_forces_ref<FG_KEY_LINEAR, true>), grid, block, 0, st.stream, sp, sv, rho_p, start,
+                               end, vout, pout, id, n, m, mu, st.common, st.kernels, bc);
+        else
+            hipLaunchKernelGGL((k_forces_ref<FG_KEY_LINEAR, false>), grid, block, 0, st.stream, sp, sv, rho_p, start,
+                               end, vout, pout, id, n, m, mu, st.common, st.kernels, bc);
+    } else {
+        if (coupled)
+            hipLaunchKernelGGL((k_forces_ref<FG_KEY_REF_XOR, true>), grid, block, 0, st.stream, sp, sv, rho_p, start,
+                               end, vout, pout, id, n, m, mu, st.common, st.kernels, bc);
+        else
+            hipLaunchKernelGGL((k_forces_ref<FG_KEY_REF_XOR, false>), grid, block, 0, st.stream, sp, sv, rho_p,
+                               start, end, vout, pout, id, n, m, mu, st.common, st.kernels, bc);
+    }
     return check_launch("forces") ? FG_OK : fg_peek_error();
 }
 

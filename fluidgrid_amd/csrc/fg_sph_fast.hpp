@@ -14,13 +14,14 @@ namespace fg {
 
 // Acceleration after gravity, walls and clamp (K_FluidSim.cu:287-303), from
 // the pair sum `internal`.
+template <bool COUPLED>
 __device__ __forceinline__ F3 finish_accel(F3 internal, float m, const float4& pos, const float4& vel,
                                            const fg_common_fluid_params& cp, const Contacts& bc) {
     const F3 g = f3(cp.global_external_force.x, cp.global_external_force.y, cp.global_external_force.z);
     F3 total = add(mul(internal, m), g);
     // K_FluidSim.cu:389-390: body contact forces are added after the mass
     // factor (the uncoupled kernel adds nothing here).
-    if (bc.coupled) total = add(total, fluid_external(pos, vel, bc));
+    if (COUPLED && bc.coupled) total = add(total, fluid_external(pos, vel, bc));
     F3 wall = walls_no_penetration(xyz(pos), xyz(vel), cp);
     wall = add(wall, walls_no_slip(xyz(pos), xyz(vel), cp));
     total = add(total, wall);
@@ -236,9 +237,30 @@ __device__ __forceinline__ void pair_term(PairAcc& A, uint32_t i, uint32_t j, co
     }
 }
 
+// Per-lane row table of the flat force schedule, in LDS: for each of the 9
+// stencil rows, the sorted index of its first particle and its neighbour
+// mask (0 for a culled row).
+struct FlatRows {
+    uint32_t* rb;              // [9][lanes]
+    unsigned long long* mm;    // [9][lanes]
+    uint32_t lanes, lane;
+    __device__ __forceinline__ uint32_t& b(int r) { return rb[r * lanes + lane]; }
+    __device__ __forceinline__ unsigned long long& m(int r) { return mm[r * lanes + lane]; }
+};
+
 // Pressure + viscosity + walls (+ body contact) of sorted particle i (i < n),
 // written to vout[id[i]] (and pout[id[i]] when integrating).
-template <bool MASKS>
+//
+// Two schedules give bit-identical sums (rows in order, neighbours of a row
+// in ascending sorted order):
+//  * row loop: the 9 rows in turn; a wavefront waits at every row for its
+//    lane with the most neighbours in that row;
+//  * flat (MASKS with a row table): the row masks are gathered first, then
+//    each lane walks its own neighbour stream across rows, one neighbour per
+//    iteration, so a wavefront runs for its lane with the most neighbours
+//    overall.  A particle with a row longer than the mask width takes the
+//    row loop.
+template <bool MASKS, bool COUPLED = false>
 __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __restrict__ sp,
                                                 const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                                                 const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
@@ -246,7 +268,7 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                                                 const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
                                                 const fg_common_fluid_params& cp, const fg_kernels_cache& kc,
                                                 uint32_t key_base, const unsigned long long* __restrict__ masks,
-                                                const Contacts& bc) {
+                                                const Contacts& bc, FlatRows* flat = nullptr) {
     const float4 pos = sp[i];
     const float4 vel = sv[i];
     F3 a = f3(0.0f, 0.0f, 0.0f);
@@ -258,7 +280,46 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
         const float2 dpi = rho_p[i];
         const RowWalker w = make_rows(pos, cp);
         PairAcc A{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        for (int row = 0; row < 9; row++) {
+        bool rowloop = true;
+        if (MASKS && flat) {
+            bool fits = true;
+#pragma unroll
+            for (int row = 0; row < 9; row++) {
+                const int dz = row / 3 - 1, dy = row % 3 - 1;
+                uint32_t rb = 0, re = 0;
+                unsigned long long mm = 0ull;
+                if (row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) {
+                    if (re - rb <= kRowMaskBits) {
+                        mm = masks[(size_t)row * n + i];
+                        // the particle itself is in its own row's mask; it adds nothing
+                        if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
+                    } else {
+                        fits = false;
+                    }
+                }
+                flat->b(row) = rb;
+                flat->m(row) = mm;
+            }
+            if (fits) {
+                rowloop = false;
+                int row = -1;
+                uint32_t rb = 0;
+                unsigned long long mm = 0ull;
+                for (;;) {
+                    while (mm == 0ull && row < 8) {
+                        row++;
+                        mm = flat->m(row);
+                        rb = flat->b(row);
+                    }
+                    if (mm == 0ull) break;
+                    const uint32_t k = (uint32_t)__builtin_ctzll(mm);
+                    mm &= mm - 1ull;
+                    const uint32_t j = rb + k;
+                    pair_term(A, i, j, pos, vel, dpi.y, sp[j], sv[j], rho_p[j], h, s, hhu);
+                }
+            }
+        }
+        for (int row = 0; rowloop && row < 9; row++) {
             const int dz = row / 3 - 1, dy = row % 3 - 1;
             uint32_t rb, re;
             if (!row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) continue;
@@ -286,7 +347,7 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
         const float cP = -kc.kSpikyGradient * iri;
         const float cV = kc.kViscosityLaplacian * mu * iri;
         const F3 internal = f3(cP * A.px + cV * A.vx, cP * A.py + cV * A.vy, cP * A.pz + cV * A.vz);
-        a = finish_accel(internal, m, pos, vel, cp, bc);
+        a = finish_accel<COUPLED>(internal, m, pos, vel, cp, bc);
     }
     write_back(id[i], n, pos, vel, a, cp, vout, pout);
 }

@@ -107,6 +107,8 @@ SIGNATURES = {
     # tuning and batched scenes
     "fg_set_tuning": (i32, [i32, i32]),
     "fg_get_tuning": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fg_set_force_schedule": (i32, [i32]),
+    "fg_get_force_schedule": (i32, []),
     "fg_batch_hash": (i32, [vp, u32, vp, vp, vp, u32]),
     "fg_batch_density_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, u32]),
     "fg_batch_forces_integrate_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
