@@ -106,12 +106,13 @@ def measure_batched(ens, steps, warmup, tune_steps=5):
     out = {}
     b = lib.BatchSimulator(ens)
     try:
-        b.set_tuning(256, 8)
+        b.set_tuning(256, 8, False, False)
         for label in ("fixed", "tuned"):
             if label == "tuned":
                 t = b.autotune(trial_steps=tune_steps)
                 out["tuning"] = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
-                                 "trials_ms": [[a, c, round(m, 4)] for a, c, m in t["trials"]]}
+                                 "flat_density": t["flat_density"], "flat_forces": t["flat_forces"],
+                                 "trials_ms": [[a, c, fd, ff, round(m, 4)] for a, c, fd, ff, m in t["trials"]]}
             b.step(warmup)
             b.synchronize()
             t0 = time.perf_counter()
@@ -132,6 +133,7 @@ def measure_batched(ens, steps, warmup, tune_steps=5):
     finally:
         b.close()
         lib.load().fg_set_tuning(256, 8)
+        lib.load().fg_set_schedule(0, 0)
     # plain form: every scene alone through the single-fluid simulator
     k = max(5, steps // 4)
     total_ms = 0.0

@@ -54,17 +54,20 @@ __global__ void __launch_bounds__(256) k_batch_hash(const fg_scene_desc* __restr
     });
 }
 
-template <int BS>
+template <int BS, bool FLAT>
 __global__ void __launch_bounds__(BS)
     k_batch_density(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes, const uint32_t* __restrict__ scene_of,
                     const float4* __restrict__ sp, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
                     const uint32_t* __restrict__ end, unsigned long long* __restrict__ masks, uint32_t n) {
+    __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
+    __shared__ uint32_t s_re[FLAT ? 9 * BS : 1];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
+    FlatRanges fr{s_rb, s_re, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
         const fg_scene_desc& d = scenes[s];
         density_particle<true>(i, sp, rho_p, start, end, n, d.gasConstant, d.restDensity, d.mass, d.common,
-                               d.kernels, d.keyBase, masks);
+                               d.kernels, d.keyBase, masks, FLAT ? &fr : nullptr);
     });
 }
 
@@ -91,17 +94,7 @@ __global__ void __launch_bounds__(BS)
 #define FG_BATCH_LAUNCH_BS(KERN, BS, n, ...) \
     hipLaunchKernelGGL(KERN, dim3(grid_for(n, BS)), dim3(BS), 0, state().stream, __VA_ARGS__)
 
-// Launches KERN<BS> (or KERN<BS, FLAT>) with BS = the tuned SPH block size.
-#define FG_BATCH_LAUNCH(KERN, n, ...)                                                                       \
-    do {                                                                                                    \
-        switch (state().sph_block) {                                                                        \
-            case 64: FG_BATCH_LAUNCH_BS((KERN<64>), 64, n, __VA_ARGS__); break;                             \
-            case 128: FG_BATCH_LAUNCH_BS((KERN<128>), 128, n, __VA_ARGS__); break;                          \
-            case 512: FG_BATCH_LAUNCH_BS((KERN<512>), 512, n, __VA_ARGS__); break;                          \
-            default: FG_BATCH_LAUNCH_BS((KERN<256>), 256, n, __VA_ARGS__);                                 \
-        }                                                                                                   \
-    } while (0)
-
+// Launches KERN<BS, FLAT> with BS = the tuned SPH block size.
 #define FG_BATCH_LAUNCH2(KERN, FLAT, n, ...)                                                                \
     do {                                                                                                    \
         switch (state().sph_block) {                                                                        \
@@ -139,8 +132,12 @@ int fg_batch_density_nbr(const fg_scene_desc* scenes, unsigned int numScenes, co
         set_error(FG_ERR_INVALID_ARGUMENT, "fg_batch_density_nbr: null pointer or no scenes");
         return FG_ERR_INVALID_ARGUMENT;
     }
-    FG_BATCH_LAUNCH(k_batch_density, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
-                    FG_F2(mass_pressure), gridStart, gridEnd, neighborMasks, numParticles);
+    if (state().flat_density)
+        FG_BATCH_LAUNCH2(k_batch_density, true, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
+                         FG_F2(mass_pressure), gridStart, gridEnd, neighborMasks, numParticles);
+    else
+        FG_BATCH_LAUNCH2(k_batch_density, false, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
+                         FG_F2(mass_pressure), gridStart, gridEnd, neighborMasks, numParticles);
     return check_launch("fg_batch_density_nbr") ? FG_OK : fg_peek_error();
 }
 

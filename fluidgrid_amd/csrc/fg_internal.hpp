@@ -31,9 +31,10 @@ struct State {
     // fast density/force kernels, and digit bits per radix-sort pass.
     int sph_block = 256;
     int sort_digit_bits = 8;
-    // Force-pass schedule of the masked fast path: flat neighbour stream per
+    // Schedules of the masked fast path: flat candidate/neighbour stream per
     // lane (true) or row by row (false); results are identical.
-    bool flat_forces = true;
+    bool flat_forces = false;
+    bool flat_density = false;
 };
 
 State& state();

@@ -130,12 +130,17 @@ int fg_set_tuning(int sphBlock, int sortDigitBits) {
     return FG_OK;
 }
 
-int fg_set_force_schedule(int flat) {
-    state().flat_forces = flat != 0;
+int fg_set_schedule(int flatDensity, int flatForces) {
+    state().flat_density = flatDensity != 0;
+    state().flat_forces = flatForces != 0;
     return FG_OK;
 }
 
-int fg_get_force_schedule(void) { return state().flat_forces ? 1 : 0; }
+int fg_get_schedule(int* flatDensity, int* flatForces) {
+    if (flatDensity) *flatDensity = state().flat_density ? 1 : 0;
+    if (flatForces) *flatForces = state().flat_forces ? 1 : 0;
+    return FG_OK;
+}
 
 int fg_get_tuning(int* sphBlock, int* sortDigitBits) {
     if (sphBlock) *sphBlock = state().sph_block;

@@ -189,6 +189,20 @@ __global__ void __launch_bounds__(BS)
     density_particle<MASKS>(i, sp, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks);
 }
 
+// Flat schedule (row table in LDS); same results as k_density_fast<MASKS>.
+template <bool MASKS, int BS>
+__global__ void __launch_bounds__(BS)
+    k_density_flat(const float4* __restrict__ sp, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
+                   const uint32_t* __restrict__ end, uint32_t n, float k, float rho0, float m,
+                   fg_common_fluid_params cp, fg_kernels_cache kc, unsigned long long* __restrict__ masks) {
+    __shared__ uint32_t s_rb[9 * BS];
+    __shared__ uint32_t s_re[9 * BS];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (i >= n) return;
+    FlatRanges fr{s_rb, s_re, (uint32_t)BS, (uint32_t)threadIdx.x};
+    density_particle<MASKS>(i, sp, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks, &fr);
+}
+
 template <bool MASKS, bool COUPLED, int BS>
 __global__ void __launch_bounds__(BS)
     k_forces_fast(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
@@ -260,7 +274,9 @@ int launch_density(float4* sp, float2* rho_p, uint32_t* start, uint32_t* end, ui
                    float m, unsigned long long* masks = nullptr) {
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
-    if (use_fast() && masks)
+    if (use_fast() && masks && st.flat_density)
+        FG_SPH_LAUNCH(k_density_flat, true, n, sp, rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
+    else if (use_fast() && masks)
         FG_SPH_LAUNCH(k_density_fast, true, n, sp, rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
     else if (use_fast())
         FG_SPH_LAUNCH(k_density_fast, false, n, sp, rho_p, start, end, n, k, rho0, m, st.common, st.kernels,

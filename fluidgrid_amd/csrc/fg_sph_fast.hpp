@@ -133,11 +133,18 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
 // Rows longer than 64 particles are not encoded and are re-tested.
 constexpr uint32_t kRowMaskBits = 64;
 
+// xyz of a float4 array element as one 12-byte load (the w of a neighbour
+// is not used by the pair terms).
+__device__ __forceinline__ float3 ld_xyz(const float4* p) {
+    const float* f = reinterpret_cast<const float*>(p);
+    return make_float3(f[0], f[1], f[2]);
+}
+
 // Poly6 sum over four candidates; bit u of the returned nibble is set when
 // candidate u lies within h.
-__device__ __forceinline__ uint32_t density4(const float4& p, const float4& q0, const float4& q1, const float4& q2,
-                                             const float4& q3, float h2u, float hhu, float& acc) {
-    const float4 q[4] = {q0, q1, q2, q3};
+__device__ __forceinline__ uint32_t density4(const float4& p, const float3& q0, const float3& q1, const float3& q2,
+                                             const float3& q3, float h2u, float hhu, float& acc) {
+    const float3 q[4] = {q0, q1, q2, q3};
     uint32_t nib = 0;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -151,15 +158,32 @@ __device__ __forceinline__ uint32_t density4(const float4& p, const float4& q0, 
     return nib;
 }
 
+// Per-lane row table of the flat density schedule, in LDS: first and
+// one-past-last sorted index of each of the 9 rows (equal for a culled row).
+struct FlatRanges {
+    uint32_t* rb;  // [9][lanes]
+    uint32_t* re;  // [9][lanes]
+    uint32_t lanes, lane;
+    __device__ __forceinline__ uint32_t& b(int r) { return rb[r * lanes + lane]; }
+    __device__ __forceinline__ uint32_t& e(int r) { return re[r * lanes + lane]; }
+};
+
 // Density of sorted particle i (i < n): rho_p[i] = (max(1, rho), k (rho - rho0)),
-// and, with MASKS, its 9 row masks.
+// and, with MASKS, its row masks.
+//
+// Two schedules give bit-identical sums (rows in order, candidates of a row
+// in groups of four in sorted order, a short last group padded with far
+// points): the row loop, where a wavefront waits at every row for its lane
+// with the longest row, and the flat schedule (with a row table), where each
+// lane walks its own candidate stream across rows.
 template <bool MASKS>
 __device__ __forceinline__ void density_particle(uint32_t i, const float4* __restrict__ sp,
                                                  float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
                                                  const uint32_t* __restrict__ end, uint32_t n, float k, float rho0,
                                                  float m, const fg_common_fluid_params& cp,
                                                  const fg_kernels_cache& kc, uint32_t key_base,
-                                                 unsigned long long* __restrict__ masks) {
+                                                 unsigned long long* __restrict__ masks,
+                                                 FlatRanges* flat = nullptr) {
     const float4 p = sp[i];
     if (!(p.w > 0.0f)) return;
     // Distances are compared unscaled: r_u = r / s, so r < h <=> r_u^2 < (h/s)^2
@@ -170,6 +194,44 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
     const float cut = hh * 1.0001f;
     const RowWalker w = make_rows(p, cp);
     float acc = 0.0f;
+    if (flat) {
+#pragma unroll
+        for (int row = 0; row < 9; row++) {
+            const int dz = row / 3 - 1, dy = row % 3 - 1;
+            uint32_t rb = 0, re = 0;
+            if (!row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) rb = re = 0;
+            flat->b(row) = rb;
+            flat->e(row) = re;
+        }
+        const float3 far = make_float3(3.0e38f, 3.0e38f, 3.0e38f);
+        int row = -1;
+        uint32_t j = 0, rb = 0, re = 0;
+        unsigned long long msk = 0ull;
+        for (;;) {
+            while (j >= re) {
+                if (MASKS && row >= 0 && re > rb) masks[(size_t)row * n + i] = msk;
+                if (row == 8) break;
+                row++;
+                rb = flat->b(row);
+                re = flat->e(row);
+                j = rb;
+                msk = 0ull;
+            }
+            if (j >= re) break;
+            const uint32_t rest = re - j;
+            const float4* pp = sp + j;
+            const float3 t0 = ld_xyz(pp);
+            const float3 t1 = rest > 1 ? ld_xyz(pp + 1) : far;
+            const float3 t2 = rest > 2 ? ld_xyz(pp + 2) : far;
+            const float3 t3 = rest > 3 ? ld_xyz(pp + 3) : far;
+            const uint32_t nib = density4(p, t0, t1, t2, t3, h2u, hhu, acc);
+            if (MASKS) msk |= (unsigned long long)nib << ((j - rb) & 63u);
+            j += 4;
+        }
+        const float rho = m * kc.kPoly6 * (s2 * s2 * s2) * acc;
+        rho_p[i] = make_float2(fmaxf(1.0f, rho), k * (rho - rho0));
+        return;
+    }
 #pragma unroll
     for (int row = 0; row < 9; row++) {
         const int dz = row / 3 - 1, dy = row % 3 - 1;
@@ -182,10 +244,10 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
             if (n4) {
                 // Software-pipelined groups of four: the next group's loads are
                 // in flight while the current group is evaluated.
-                float4 a0 = pp[0], a1 = pp[1], a2 = pp[2], a3 = pp[3];
+                float3 a0 = ld_xyz(pp), a1 = ld_xyz(pp + 1), a2 = ld_xyz(pp + 2), a3 = ld_xyz(pp + 3);
                 for (; kk + 4 < n4; kk += 4) {
                     pp += 4;
-                    const float4 b0 = pp[0], b1 = pp[1], b2 = pp[2], b3 = pp[3];
+                    const float3 b0 = ld_xyz(pp), b1 = ld_xyz(pp + 1), b2 = ld_xyz(pp + 2), b3 = ld_xyz(pp + 3);
                     const uint32_t nib = density4(p, a0, a1, a2, a3, h2u, hhu, acc);
                     if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
                     a0 = b0; a1 = b1; a2 = b2; a3 = b3;
@@ -197,10 +259,10 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
             }
             if (kk < len) {  // 1-3 remaining candidates; the others are masked out
                 const uint32_t rest = len - kk;
-                const float4 far = make_float4(3.0e38f, 3.0e38f, 3.0e38f, 0.0f);
-                const float4 t0 = pp[0];
-                const float4 t1 = rest > 1 ? pp[1] : far;
-                const float4 t2 = rest > 2 ? pp[2] : far;
+                const float3 far = make_float3(3.0e38f, 3.0e38f, 3.0e38f);
+                const float3 t0 = ld_xyz(pp);
+                const float3 t1 = rest > 1 ? ld_xyz(pp + 1) : far;
+                const float3 t2 = rest > 2 ? ld_xyz(pp + 2) : far;
                 const uint32_t nib = density4(p, t0, t1, t2, far, h2u, hhu, acc);
                 if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
             }

@@ -118,10 +118,13 @@ BatchedSimulator::BatchedSimulator(const std::vector<BatchScene>& scenes) {
     hip_check(hipMemset(d_end, 0, sizeof(unsigned) * mBuckets), "memset");
     hip_check(hipStreamCreateWithFlags(&mStream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : mEv) hip_check(hipEventCreate(&e), "hipEventCreate");
-    int blk = 256, bits = 8;
+    int blk = 256, bits = 8, fd = 0, ff = 0;
     fg_get_tuning(&blk, &bits);
+    fg_get_schedule(&fd, &ff);
     mTuning.sphBlock = blk;
     mTuning.sortDigitBits = bits;
+    mTuning.flatDensity = fd != 0;
+    mTuning.flatForces = ff != 0;
 }
 
 BatchedSimulator::~BatchedSimulator() {
@@ -147,6 +150,7 @@ void BatchedSimulator::setStageTiming(bool on) { mTiming = on; }
 
 void BatchedSimulator::setTuning(const BatchTuning& t) {
     if (fg_set_tuning(t.sphBlock, t.sortDigitBits) != FG_OK) fg_check("setTuning");
+    fg_set_schedule(t.flatDensity ? 1 : 0, t.flatForces ? 1 : 0);
     mTuning = t;
     invalidateGraph();  // the launch shapes are recorded in the graph
 }
@@ -236,6 +240,14 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
     for (int blk : {64, 128, 256, 512}) {
         BatchTuning t = best;
         t.sphBlock = blk;
+        const float ms = time_shape(t);
+        if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+    for (int sched = 0; sched < 4; sched++) {
+        BatchTuning t = best;
+        t.flatDensity = (sched & 1) != 0;
+        t.flatForces = (sched & 2) != 0;
+        if (t.flatDensity == best.flatDensity && t.flatForces == best.flatForces) continue;
         const float ms = time_shape(t);
         if (ms < best_ms) { best_ms = ms; best = t; }
     }

@@ -109,19 +109,24 @@ int fg_bsim_info(unsigned int* numParticles, unsigned int* numBuckets, int* keyB
     return FG_OK;
 }
 
-int fg_bsim_autotune(int trialSteps, int* sphBlock, int* sortDigitBits, float* trials, int maxTrials,
-                     int* numTrials) {
+int fg_bsim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, int* numTrials) {
     if (!batch()) return FG_ERR_NOT_INITIALIZED;
     try {
         std::vector<std::pair<Simulators::BatchTuning, float>> log;
         const Simulators::BatchTuning t = batch()->autotune(trialSteps, &log);
-        if (sphBlock) *sphBlock = t.sphBlock;
-        if (sortDigitBits) *sortDigitBits = t.sortDigitBits;
+        if (tuning) {
+            tuning[0] = t.sphBlock;
+            tuning[1] = t.sortDigitBits;
+            tuning[2] = t.flatDensity ? 1 : 0;
+            tuning[3] = t.flatForces ? 1 : 0;
+        }
         int k = 0;
         for (; trials && k < maxTrials && k < (int)log.size(); k++) {
-            trials[3 * k] = (float)log[k].first.sphBlock;
-            trials[3 * k + 1] = (float)log[k].first.sortDigitBits;
-            trials[3 * k + 2] = log[k].second;
+            trials[5 * k] = (float)log[k].first.sphBlock;
+            trials[5 * k + 1] = (float)log[k].first.sortDigitBits;
+            trials[5 * k + 2] = log[k].first.flatDensity ? 1.0f : 0.0f;
+            trials[5 * k + 3] = log[k].first.flatForces ? 1.0f : 0.0f;
+            trials[5 * k + 4] = log[k].second;
         }
         if (numTrials) *numTrials = k;
         return FG_OK;
@@ -130,12 +135,14 @@ int fg_bsim_autotune(int trialSteps, int* sphBlock, int* sortDigitBits, float* t
     }
 }
 
-int fg_bsim_set_tuning(int sphBlock, int sortDigitBits) {
+int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces) {
     if (!batch()) return FG_ERR_NOT_INITIALIZED;
     try {
         Simulators::BatchTuning t;
         t.sphBlock = sphBlock;
         t.sortDigitBits = sortDigitBits;
+        t.flatDensity = flatDensity != 0;
+        t.flatForces = flatForces != 0;
         batch()->setTuning(t);
         return FG_OK;
     } catch (const std::exception& e) {

@@ -107,8 +107,8 @@ SIGNATURES = {
     # tuning and batched scenes
     "fg_set_tuning": (i32, [i32, i32]),
     "fg_get_tuning": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
-    "fg_set_force_schedule": (i32, [i32]),
-    "fg_get_force_schedule": (i32, []),
+    "fg_set_schedule": (i32, [i32, i32]),
+    "fg_get_schedule": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fg_batch_hash": (i32, [vp, u32, vp, vp, vp, u32]),
     "fg_batch_density_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, u32]),
     "fg_batch_forces_integrate_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
@@ -120,8 +120,8 @@ SIGNATURES = {
     "fg_bsim_download_density": (i32, [vp, vp]),
     "fg_bsim_stage_times": (i32, [vp]),
     "fg_bsim_info": (i32, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_int)]),
-    "fg_bsim_autotune": (i32, [i32, C.POINTER(C.c_int), C.POINTER(C.c_int), vp, i32, C.POINTER(C.c_int)]),
-    "fg_bsim_set_tuning": (i32, [i32, i32]),
+    "fg_bsim_autotune": (i32, [i32, vp, vp, i32, C.POINTER(C.c_int)]),
+    "fg_bsim_set_tuning": (i32, [i32, i32, i32, i32]),
     "fg_bsim_destroy": (i32, []),
 }
 
@@ -380,15 +380,18 @@ class BatchSimulator:
         return dict(particles=n.value, buckets=nb.value, key_bits=kb.value)
 
     def autotune(self, trial_steps=10):
-        blk, bits, nt = C.c_int(), C.c_int(), C.c_int()
-        trials = np.zeros((16, 3), np.float32)
-        check(load().fg_bsim_autotune(int(trial_steps), C.byref(blk), C.byref(bits), trials.ctypes.data, 16,
-                                      C.byref(nt)), "fg_bsim_autotune")
-        return dict(sph_block=blk.value, sort_digit_bits=bits.value,
-                    trials=[(int(a), int(b), float(c)) for a, b, c in trials[:nt.value]])
+        nt = C.c_int()
+        tun = np.zeros(4, np.int32)
+        trials = np.zeros((32, 5), np.float32)
+        check(load().fg_bsim_autotune(int(trial_steps), tun.ctypes.data, trials.ctypes.data, 32, C.byref(nt)),
+              "fg_bsim_autotune")
+        return dict(sph_block=int(tun[0]), sort_digit_bits=int(tun[1]), flat_density=bool(tun[2]),
+                    flat_forces=bool(tun[3]),
+                    trials=[(int(a), int(b), int(c), int(d), float(e)) for a, b, c, d, e in trials[:nt.value]])
 
-    def set_tuning(self, sph_block=256, sort_digit_bits=8):
-        check(load().fg_bsim_set_tuning(int(sph_block), int(sort_digit_bits)), "fg_bsim_set_tuning")
+    def set_tuning(self, sph_block=256, sort_digit_bits=8, flat_density=False, flat_forces=False):
+        check(load().fg_bsim_set_tuning(int(sph_block), int(sort_digit_bits), int(flat_density), int(flat_forces)),
+              "fg_bsim_set_tuning")
 
     def close(self):
         load().fg_bsim_destroy()

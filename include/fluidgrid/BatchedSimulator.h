@@ -25,8 +25,10 @@ struct BatchScene {
 };
 
 struct BatchTuning {
-    int sphBlock = 256;      // workgroup size of density and force kernels
-    int sortDigitBits = 8;   // widest radix digit
+    int sphBlock = 256;       // workgroup size of density and force kernels
+    int sortDigitBits = 8;    // widest radix digit
+    bool flatDensity = false; // density candidates as one stream per lane
+    bool flatForces = false;  // force neighbours as one stream per lane
 };
 
 class BatchedSimulator {
@@ -42,9 +44,11 @@ public:
     void setStageTiming(bool on);
     void synchronize();
 
-    // Times every launch shape in turn (trialSteps graph steps each, after a
-    // warm-up) and keeps the fastest; the particle state is restored
-    // afterwards, so tuning does not advance the simulation.
+    // Times launch shapes and schedules by coordinate descent (workgroup
+    // size, then the density and force schedules, then the digit width;
+    // trialSteps graph steps each after a warm-up) and keeps the fastest; the
+    // particle state is restored afterwards, so tuning does not advance the
+    // simulation.  No choice changes any result bit.
     BatchTuning autotune(int trialSteps, std::vector<std::pair<BatchTuning, float>>* trials = nullptr);
     void setTuning(const BatchTuning& t);
     const BatchTuning& tuning() const { return mTuning; }

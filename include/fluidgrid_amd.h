@@ -264,11 +264,12 @@ int   fg_get_math_mode(void);
  * the sort (6..8 bits; default 8).  Results do not depend on either. */
 int   fg_set_tuning(int sphBlock, int sortDigitBits);
 int   fg_get_tuning(int* sphBlock, int* sortDigitBits);
-/* Schedule of the masked force pass: 1 (default) walks each particle's
- * neighbours as one stream across the 9 stencil rows, 0 row by row.  The
- * summation order, and so every result bit, is the same. */
-int   fg_set_force_schedule(int flat);
-int   fg_get_force_schedule(void);
+/* Schedules of the masked fast density and force passes: 1 walks each
+ * particle's candidates (neighbours) as one stream across the 9 stencil rows
+ * with a per-lane row table in LDS, 0 (default) row by row.  The summation
+ * order, and so every result bit, is the same. */
+int   fg_set_schedule(int flatDensity, int flatForces);
+int   fg_get_schedule(int* flatDensity, int* flatForces);
 int   fg_get_common_fluid_params(fg_common_fluid_params* out);
 int   fg_get_kernels_cache(fg_kernels_cache* out);
 

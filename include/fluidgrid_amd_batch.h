@@ -28,12 +28,13 @@ int fg_bsim_stage_times(float* out5);
 /* numParticles, numBuckets, keyBits of the batch. */
 int fg_bsim_info(unsigned int* numParticles, unsigned int* numBuckets, int* keyBits);
 /* Launch-shape tuning: times each candidate for trialSteps steps, keeps the
- * fastest and restores the particle state.  trials (optional, capacity
- * maxTrials) receives (sphBlock, sortDigitBits, ms per step) triples;
+ * fastest and restores the particle state.  tuning (optional) receives
+ * (sphBlock, sortDigitBits, flatDensity, flatForces) of the choice; trials
+ * (optional, capacity maxTrials) receives one (sphBlock, sortDigitBits,
+ * flatDensity, flatForces, ms per step) record per candidate timed;
  * *numTrials their count. */
-int fg_bsim_autotune(int trialSteps, int* sphBlock, int* sortDigitBits, float* trials, int maxTrials,
-                     int* numTrials);
-int fg_bsim_set_tuning(int sphBlock, int sortDigitBits);
+int fg_bsim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, int* numTrials);
+int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces);
 int fg_bsim_destroy(void);
 
 #ifdef __cplusplus

@@ -116,11 +116,11 @@ def test_tuning_changes_no_result_and_restores_state(fglib):
         assert t["sph_block"] in (64, 128, 256, 512) and 6 <= t["sort_digit_bits"] <= 8
         assert len(t["trials"]) >= 6
         results = []
-        for blk, bits in ((256, 8), (64, 6), (512, 7)):
+        for blk, bits, fd, ff in ((256, 8, 0, 0), (64, 6, 1, 1), (512, 7, 1, 0), (128, 8, 0, 1)):
             # each shape starts from the initial state of the ensemble
             b.close()
             b = fglib.BatchSimulator(ens)
-            b.set_tuning(blk, bits)
+            b.set_tuning(blk, bits, fd, ff)
             b.step(7)
             results.append(b.state())
         for r in results[1:]:
@@ -128,6 +128,7 @@ def test_tuning_changes_no_result_and_restores_state(fglib):
     finally:
         b.close()
         fglib.load().fg_set_tuning(256, 8)
+        fglib.load().fg_set_schedule(0, 0)
 
 
 def test_full_ensemble_runs(fglib):
