@@ -31,6 +31,7 @@ constexpr int kMaxRadix = 256;
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagInclusive = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1;
+constexpr int kLookback = 8;  // predecessors read per look-back round trip
 
 struct SortPlan {
     int passes;
@@ -68,7 +69,7 @@ static SortPlan make_plan(int keyBits, int digitBits = 8) {
 }
 
 struct SortLayout {
-    size_t hist, offs, counters, status, t1k, t1v, t2k, t2v, total;
+    size_t hist, counters, status, t1k, t1v, t2k, t2v, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -83,7 +84,6 @@ static SortLayout make_layout(uint32_t n, int keyBits) {
     const size_t tiles = (n + kSortTile - 1) / kSortTile;
     size_t o = 0;
     L.hist = o; o = align256(o + sizeof(uint32_t) * kMaxPasses * kMaxRadix);
-    L.offs = o; o = align256(o + sizeof(uint32_t) * kMaxPasses * kMaxRadix);
     L.counters = o; o = align256(o + sizeof(uint32_t) * kMaxPasses + sizeof(uint32_t));
     L.status = o; o = align256(o + sizeof(uint32_t) * (size_t)max_passes * tiles * kMaxRadix);
     L.t1k = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
@@ -122,11 +122,25 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_histogram(const uint32_t*
     const uint64_t gstride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < status_words; i += gstride) status[i] = 0;
     __syncthreads();
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gstride) {
-        const uint32_t k = keys[i];
+    // A wavefront holds 64 consecutive keys; runs of equal digits (frequent:
+    // neighbouring particles share cells) are counted with one LDS atomic
+    // per run, issued by the run's first lane.
+    const int lane = threadIdx.x & 63;
+    const uint64_t n_rounds = (n + gstride - 1) / gstride;
+    for (uint64_t r = 0; r < n_rounds; r++) {
+        const uint64_t i = r * gstride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t k = valid ? keys[i] : 0u;
         for (int p = 0; p < plan.passes; p++) {
-            const uint32_t d = (k >> plan.shift[p]) & ((1u << plan.bits[p]) - 1u);
-            atomicAdd(&s_hist[wave][p][d], 1u);
+            const uint32_t d = valid ? (k >> plan.shift[p]) & ((1u << plan.bits[p]) - 1u) : 0xFFFFFFFFu;
+            const uint32_t prev = (uint32_t)__shfl_up((int)d, 1);
+            const bool head = valid && (lane == 0 || prev != d);
+            const uint64_t heads = __ballot(head) | __ballot(!valid);  // an invalid lane ends the run before it
+            if (head) {
+                const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1));
+                const uint32_t len = after ? (uint32_t)__ffsll((long long)after) : (uint32_t)(64 - lane);
+                atomicAdd(&s_hist[wave][p][d], len);
+            }
         }
     }
     __syncthreads();
@@ -140,32 +154,13 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_histogram(const uint32_t*
     }
 }
 
-// Exclusive scan of each pass's digit counts -> global digit offsets.
-__global__ void __launch_bounds__(kMaxRadix) k_sort_offsets(const uint32_t* __restrict__ hist,
-                                                            uint32_t* __restrict__ offs, int passes) {
-    __shared__ uint32_t s[kMaxRadix];
-    const int t = threadIdx.x;
-    for (int p = 0; p < passes; p++) {
-        s[t] = hist[p * kMaxRadix + t];
-        __syncthreads();
-        for (int d = 1; d < kMaxRadix; d <<= 1) {
-            uint32_t v = (t >= d) ? s[t - d] : 0u;
-            __syncthreads();
-            s[t] += v;
-            __syncthreads();
-        }
-        offs[p * kMaxRadix + t] = s[t] - hist[p * kMaxRadix + t];
-        __syncthreads();
-    }
-}
-
 // One digit pass.  Element e of tile T is base + w*64*ITEMS + k*64 + lane,
 // so (wave, item, lane) order is index order and the ranking is stable.
 __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __restrict__ kin,
                                                             const uint32_t* __restrict__ vin,
                                                             uint32_t* __restrict__ kout,
                                                             uint32_t* __restrict__ vout, uint32_t n, int shift,
-                                                            int bits, const uint32_t* __restrict__ offs,
+                                                            int bits, const uint32_t* __restrict__ hist,
                                                             uint32_t* __restrict__ status,
                                                             uint32_t* __restrict__ counter, int values_iota) {
     __shared__ uint32_t s_keys[kSortTile];
@@ -173,6 +168,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __re
     __shared__ uint32_t s_wcnt[kSortWaves][kMaxRadix];
     __shared__ uint32_t s_tstart[kMaxRadix];
     __shared__ uint32_t s_gbase[kMaxRadix];
+    __shared__ uint32_t s_doff[kMaxRadix];
     __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -231,36 +227,59 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __re
         s_tstart[tid] = tile_count;
     }
     __syncthreads();
-    // Exclusive scan over digits of the tile counts (Hillis-Steele in LDS).
+    // Exclusive scans over digits of the tile counts and of the pass's global
+    // digit counts (Hillis-Steele in LDS).
+    const uint32_t gcount = (uint32_t)tid < radix ? hist[tid] : 0u;
+    if ((uint32_t)tid < radix) s_doff[tid] = gcount;
+    __syncthreads();
     for (uint32_t d = 1; d < radix; d <<= 1) {
-        uint32_t v = 0;
-        if ((uint32_t)tid < radix && (uint32_t)tid >= d) v = s_tstart[tid - d];
+        uint32_t v = 0, g = 0;
+        if ((uint32_t)tid < radix && (uint32_t)tid >= d) {
+            v = s_tstart[tid - d];
+            g = s_doff[tid - d];
+        }
         __syncthreads();
-        if ((uint32_t)tid < radix) s_tstart[tid] += v;
+        if ((uint32_t)tid < radix) {
+            s_tstart[tid] += v;
+            s_doff[tid] += g;
+        }
         __syncthreads();
     }
     if ((uint32_t)tid < radix) {
         s_tstart[tid] -= tile_count;
-        // Decoupled look-back for this digit.
+        // Decoupled look-back for this digit, reading a window of kLookback
+        // predecessors per round trip: their aggregates are summed up to the
+        // first inclusive prefix; a predecessor that has published nothing
+        // yet ends the round, and the next round starts from it.
         uint32_t excl = 0;
         if (tile > 0) {
             int p = (int)tile - 1;
-            while (true) {
-                const uint32_t s = __hip_atomic_load(&status[(size_t)p * kMaxRadix + tid], __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t flag = s & ~kCountMask;
-                if (flag == 0u) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+            for (;;) {
+                uint32_t st[kLookback];
+#pragma unroll
+                for (int u = 0; u < kLookback; u++)
+                    st[u] = (p - u >= 0) ? __hip_atomic_load(&status[(size_t)(p - u) * kMaxRadix + tid],
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                         : 0u;
+                int used = 0;
+                bool done = false;
+#pragma unroll
+                for (int u = 0; u < kLookback; u++) {
+                    if (done || used < u) continue;  // stop at the first inclusive or unpublished entry
+                    const uint32_t flag = st[u] & ~kCountMask;
+                    if (flag == 0u) continue;
+                    excl += st[u] & kCountMask;
+                    used = u + 1;
+                    done = flag == kFlagInclusive;
                 }
-                excl += s & kCountMask;
-                if (flag == kFlagInclusive) break;
-                p--;
+                if (done) break;
+                p -= used;
+                if (used == 0) __builtin_amdgcn_s_sleep(1);
             }
             __hip_atomic_store(&status[(size_t)tile * kMaxRadix + tid], kFlagInclusive | (excl + tile_count),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_gbase[tid] = offs[tid] + excl;
+        s_gbase[tid] = (s_doff[tid] - gcount) + excl;
     }
     __syncthreads();
 
@@ -346,7 +365,6 @@ static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, voi
     }
     char* w = static_cast<char*>(ws);
     uint32_t* hist = reinterpret_cast<uint32_t*>(w + L.hist);
-    uint32_t* offs = reinterpret_cast<uint32_t*>(w + L.offs);
     uint32_t* counters = reinterpret_cast<uint32_t*>(w + L.counters);
     uint32_t* status = reinterpret_cast<uint32_t*>(w + L.status);
     uint32_t* t1k = reinterpret_cast<uint32_t*>(w + L.t1k);
@@ -362,7 +380,6 @@ static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, voi
     if (hblocks == 0) hblocks = 1;
     hipLaunchKernelGGL(k_sort_histogram, dim3(hblocks), dim3(kSortThreads), 0, st, keys, n, plan, hist, status,
                        status_words);
-    hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(kMaxRadix), 0, st, hist, offs, plan.passes);
 
     const uint32_t* src_k = keys;
     const uint32_t* src_v = vals;
@@ -378,7 +395,7 @@ static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, voi
             dk = t2k; dv = t2v;
         }
         hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(kSortThreads), 0, st, src_k, src_v, dk, dv, n,
-                           plan.shift[p], plan.bits[p], offs + p * kMaxRadix,
+                           plan.shift[p], plan.bits[p], hist + p * kMaxRadix,
                            status + (size_t)p * tiles * kMaxRadix, counters + p, (p == 0 && iota) ? 1 : 0);
         src_k = dk;
         src_v = dv;
