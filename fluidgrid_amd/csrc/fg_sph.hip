@@ -418,7 +418,8 @@ int fg_sph_forces_integrate(fg_float4* orderedPositions, fg_float4* orderedVeloc
 }
 
 size_t fg_neighbor_mask_bytes(unsigned int numParticles) {
-    return (size_t)9 * sizeof(unsigned long long) * (size_t)numParticles;
+    // 9 row masks, then (1/rho, p/rho) per particle
+    return (size_t)(9 * sizeof(unsigned long long) + 2 * sizeof(float)) * (size_t)numParticles;
 }
 
 int fg_sph_density_nbr(fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,

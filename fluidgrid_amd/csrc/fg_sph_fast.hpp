@@ -158,6 +158,18 @@ __device__ __forceinline__ uint32_t density4(const float4& p, const float3& q0, 
     return nib;
 }
 
+// rho_p[i] = (max(1, rho), k (rho - rho0)) (K_FluidSim.cu:246-249) and, when
+// inv is given, inv[i] = (1/rho_p.x, rho_p.y/rho_p.x).
+__device__ __forceinline__ void store_density(uint32_t i, float rho, float k, float rho0, float2* __restrict__ rho_p,
+                                              float2* __restrict__ inv) {
+    const float2 rp = make_float2(fmaxf(1.0f, rho), k * (rho - rho0));
+    rho_p[i] = rp;
+    if (inv) {
+        const float a = 1.0f / rp.x;
+        inv[i] = make_float2(a, rp.y * a);
+    }
+}
+
 // Per-lane row table of the flat density schedule, in LDS: first and
 // one-past-last sorted index of each of the 9 rows (equal for a culled row).
 struct FlatRanges {
@@ -185,7 +197,17 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
                                                  unsigned long long* __restrict__ masks,
                                                  FlatRanges* flat = nullptr) {
     const float4 p = sp[i];
-    if (!(p.w > 0.0f)) return;
+    // Per-particle terms the force pass reads for every neighbour j, stored
+    // after the 9 row masks: (1/rho_j, p_j/rho_j) from the stored (rho, p).
+    float2* inv = MASKS ? reinterpret_cast<float2*>(masks + (size_t)9 * n) : nullptr;
+    if (!(p.w > 0.0f)) {
+        if (MASKS) {
+            const float2 rp = rho_p[i];  // left as it is; neighbours still read it
+            const float a = 1.0f / rp.x;
+            inv[i] = make_float2(a, rp.y * a);
+        }
+        return;
+    }
     // Distances are compared unscaled: r_u = r / s, so r < h <=> r_u^2 < (h/s)^2
     // and h^2 - r^2 = s^2 (h^2/s^2 - r_u^2).
     const float s = cp.simulation_scale, s2 = s * s;
@@ -228,8 +250,7 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
             if (MASKS) msk |= (unsigned long long)nib << ((j - rb) & 63u);
             j += 4;
         }
-        const float rho = m * kc.kPoly6 * (s2 * s2 * s2) * acc;
-        rho_p[i] = make_float2(fmaxf(1.0f, rho), k * (rho - rho0));
+        store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * acc, k, rho0, rho_p, inv);
         return;
     }
 #pragma unroll
@@ -270,33 +291,41 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
         // Rows longer than 64 particles are never read through their mask.
         if (MASKS) masks[(size_t)row * n + i] = msk;
     }
-    const float rho = m * kc.kPoly6 * (s2 * s2 * s2) * acc;
-    rho_p[i] = make_float2(fmaxf(1.0f, rho), k * (rho - rho0));
+    store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * acc, k, rho0, rho_p, inv);
 }
 
 // Pressure and viscosity terms of one neighbour j (r < h checked again, so
-// the same code serves rows without a mask).
+// the same code serves rows without a mask), from aj = 1/rho_j and
+// bj = p_j/rho_j: (p_i + p_j)/rho_j = p_i aj + bj.
 struct PairAcc {
     float px, py, pz, vx, vy, vz;
 };
 
 __device__ __forceinline__ void pair_term(PairAcc& A, uint32_t i, uint32_t j, const float4& pos, const float4& vel,
-                                          float pri, float4 q, float4 vj, float2 dpj, float h, float s, float hhu) {
+                                          float pri, float3 q, float3 vj, float aj, float bj, float h, float s,
+                                          float hhu) {
     const float dx = pos.x - q.x, dy = pos.y - q.y, dz = pos.z - q.z;
     const float r2u = dx * dx + dy * dy + dz * dz;  // unscaled
     if (r2u < hhu && j != i) {
         const float iru = __builtin_amdgcn_rsqf(r2u);
         const float hr = h - r2u * iru * s;
-        const float irj = __builtin_amdgcn_rcpf(dpj.x);
-        const float cp_ = hr * hr * (pri + dpj.y) * irj * iru;
+        const float cp_ = hr * hr * (pri * aj + bj) * iru;
         A.px += dx * cp_;
         A.py += dy * cp_;
         A.pz += dz * cp_;
-        const float cv = hr * irj;
+        const float cv = hr * aj;
         A.vx += (vj.x - vel.x) * cv;
         A.vy += (vj.y - vel.y) * cv;
         A.vz += (vj.z - vel.z) * cv;
     }
+}
+
+// The same from a stored (rho_j, p_j) (rows without a mask).
+__device__ __forceinline__ void pair_term_rp(PairAcc& A, uint32_t i, uint32_t j, const float4& pos,
+                                             const float4& vel, float pri, float3 q, float3 vj, float2 dpj, float h,
+                                             float s, float hhu) {
+    const float aj = __builtin_amdgcn_rcpf(dpj.x);
+    pair_term(A, i, j, pos, vel, pri, q, vj, aj, dpj.y * aj, h, s, hhu);
 }
 
 // Per-lane row table of the flat force schedule, in LDS: for each of the 9
@@ -342,6 +371,7 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
         const float2 dpi = rho_p[i];
         const RowWalker w = make_rows(pos, cp);
         PairAcc A{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        const float2* inv = MASKS ? reinterpret_cast<const float2*>(masks + (size_t)9 * n) : nullptr;
         bool rowloop = true;
         if (MASKS && flat) {
             bool fits = true;
@@ -377,7 +407,8 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                     const uint32_t k = (uint32_t)__builtin_ctzll(mm);
                     mm &= mm - 1ull;
                     const uint32_t j = rb + k;
-                    pair_term(A, i, j, pos, vel, dpi.y, sp[j], sv[j], rho_p[j], h, s, hhu);
+                    const float2 ab = inv[j];
+                    pair_term(A, i, j, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), ab.x, ab.y, h, s, hhu);
                 }
             }
         }
@@ -394,14 +425,15 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                     const uint32_t k2 = two ? (uint32_t)__builtin_ctzll(mm) : k1;
                     if (two) mm &= mm - 1ull;
                     const uint32_t j1 = rb + k1, j2 = rb + k2;
-                    const float4 q1 = sp[j1], q2 = sp[j2];
-                    const float4 v1 = sv[j1], v2 = sv[j2];
-                    const float2 d1 = rho_p[j1], d2 = rho_p[j2];
-                    pair_term(A, i, j1, pos, vel, dpi.y, q1, v1, d1, h, s, hhu);
-                    if (two) pair_term(A, i, j2, pos, vel, dpi.y, q2, v2, d2, h, s, hhu);
+                    const float3 q1 = ld_xyz(sp + j1), q2 = ld_xyz(sp + j2);
+                    const float3 v1 = ld_xyz(sv + j1), v2 = ld_xyz(sv + j2);
+                    const float2 d1 = inv[j1], d2 = inv[j2];
+                    pair_term(A, i, j1, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s, hhu);
+                    if (two) pair_term(A, i, j2, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s, hhu);
                 }
             } else {
-                for (uint32_t j = rb; j < re; j++) pair_term(A, i, j, pos, vel, dpi.y, sp[j], sv[j], rho_p[j], h, s, hhu);
+                for (uint32_t j = rb; j < re; j++)
+                    pair_term_rp(A, i, j, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), rho_p[j], h, s, hhu);
             }
         }
         // pressure: (s d_u / r) = d_u / r_u, so no scale factor remains

@@ -302,8 +302,10 @@ int fg_sph_forces_integrate(fg_float4* orderedPositions, fg_float4* orderedVeloc
  * with FG_MATH_FAST; other modes ignore the masks).  The density launch
  * records, for each of a particle's 9 stencil rows, which particles of the
  * row lie within h; the force launch then evaluates only those pairs.
- * neighborMasks holds fg_neighbor_mask_bytes(n) bytes; the force launch must
- * follow the density launch on the same sorted arrays.  pos may be NULL
+ * neighborMasks holds fg_neighbor_mask_bytes(n) bytes (the 9 row masks of
+ * every particle, then (1/rho, p/rho) of every particle, which the force
+ * launch reads for each neighbour); the force launch must follow the density
+ * launch on the same sorted arrays.  pos may be NULL
  * (no integration). */
 size_t fg_neighbor_mask_bytes(unsigned int numParticles);
 int fg_sph_density_nbr(fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,
