@@ -142,6 +142,11 @@ __device__ __forceinline__ float3 ld_xyz(const float4* p) {
 
 // Poly6 sum over four candidates; bit u of the returned nibble is set when
 // candidate u lies within h.
+//
+// The fused multiply-adds are written out (and contraction is off here), so
+// every schedule and workgroup size that inlines this evaluates exactly the
+// same operations.
+#pragma clang fp contract(off)
 __device__ __forceinline__ uint32_t density4(const float4& p, const float3& q0, const float3& q1, const float3& q2,
                                              const float3& q3, float h2u, float hhu, float& acc) {
     const float3 q[4] = {q0, q1, q2, q3};
@@ -149,17 +154,19 @@ __device__ __forceinline__ uint32_t density4(const float4& p, const float3& q0, 
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         const float dx = p.x - q[u].x, dy = p.y - q[u].y, dz = p.z - q[u].z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         const float t = h2u - r2;
         const bool hit = r2 < hhu;
-        acc = hit ? acc + t * t * t : acc;
+        acc = hit ? __builtin_fmaf(t * t, t, acc) : acc;
         nib |= hit ? (1u << u) : 0u;
     }
     return nib;
 }
+#pragma clang fp contract(fast)
 
 // rho_p[i] = (max(1, rho), k (rho - rho0)) (K_FluidSim.cu:246-249) and, when
 // inv is given, inv[i] = (1/rho_p.x, rho_p.y/rho_p.x).
+#pragma clang fp contract(off)
 __device__ __forceinline__ void store_density(uint32_t i, float rho, float k, float rho0, float2* __restrict__ rho_p,
                                               float2* __restrict__ inv) {
     const float2 rp = make_float2(fmaxf(1.0f, rho), k * (rho - rho0));
@@ -169,6 +176,7 @@ __device__ __forceinline__ void store_density(uint32_t i, float rho, float k, fl
         inv[i] = make_float2(a, rp.y * a);
     }
 }
+#pragma clang fp contract(fast)
 
 // Per-lane row table of the flat density schedule, in LDS: first and
 // one-past-last sorted index of each of the 9 rows (equal for a culled row).
@@ -301,32 +309,35 @@ struct PairAcc {
     float px, py, pz, vx, vy, vz;
 };
 
+#pragma clang fp contract(off)
 __device__ __forceinline__ void pair_term(PairAcc& A, uint32_t i, uint32_t j, const float4& pos, const float4& vel,
                                           float pri, float3 q, float3 vj, float aj, float bj, float h, float s,
                                           float hhu) {
     const float dx = pos.x - q.x, dy = pos.y - q.y, dz = pos.z - q.z;
-    const float r2u = dx * dx + dy * dy + dz * dz;  // unscaled
+    const float r2u = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));  // unscaled
     if (r2u < hhu && j != i) {
         const float iru = __builtin_amdgcn_rsqf(r2u);
-        const float hr = h - r2u * iru * s;
-        const float cp_ = hr * hr * (pri * aj + bj) * iru;
-        A.px += dx * cp_;
-        A.py += dy * cp_;
-        A.pz += dz * cp_;
+        const float hr = __builtin_fmaf(-(r2u * iru), s, h);
+        const float cp_ = hr * hr * __builtin_fmaf(pri, aj, bj) * iru;
+        A.px = __builtin_fmaf(dx, cp_, A.px);
+        A.py = __builtin_fmaf(dy, cp_, A.py);
+        A.pz = __builtin_fmaf(dz, cp_, A.pz);
         const float cv = hr * aj;
-        A.vx += (vj.x - vel.x) * cv;
-        A.vy += (vj.y - vel.y) * cv;
-        A.vz += (vj.z - vel.z) * cv;
+        A.vx = __builtin_fmaf(vj.x - vel.x, cv, A.vx);
+        A.vy = __builtin_fmaf(vj.y - vel.y, cv, A.vy);
+        A.vz = __builtin_fmaf(vj.z - vel.z, cv, A.vz);
     }
 }
 
-// The same from a stored (rho_j, p_j) (rows without a mask).
+// The same from a stored (rho_j, p_j) (rows without a mask), with the
+// (1/rho_j, p_j/rho_j) the density pass stores beside the masks.
 __device__ __forceinline__ void pair_term_rp(PairAcc& A, uint32_t i, uint32_t j, const float4& pos,
                                              const float4& vel, float pri, float3 q, float3 vj, float2 dpj, float h,
                                              float s, float hhu) {
-    const float aj = __builtin_amdgcn_rcpf(dpj.x);
+    const float aj = 1.0f / dpj.x;
     pair_term(A, i, j, pos, vel, pri, q, vj, aj, dpj.y * aj, h, s, hhu);
 }
+#pragma clang fp contract(fast)
 
 // Per-lane row table of the flat force schedule, in LDS: for each of the 9
 // stencil rows, the sorted index of its first particle and its neighbour
@@ -440,7 +451,8 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
         const float iri = 1.0f / dpi.x;
         const float cP = -kc.kSpikyGradient * iri;
         const float cV = kc.kViscosityLaplacian * mu * iri;
-        const F3 internal = f3(cP * A.px + cV * A.vx, cP * A.py + cV * A.vy, cP * A.pz + cV * A.vz);
+        const F3 internal = f3(__builtin_fmaf(cP, A.px, cV * A.vx), __builtin_fmaf(cP, A.py, cV * A.vy),
+                               __builtin_fmaf(cP, A.pz, cV * A.vz));
         a = finish_accel<COUPLED>(internal, m, pos, vel, cp, bc);
     }
     write_back(id[i], n, pos, vel, a, cp, vout, pout);
