@@ -100,38 +100,45 @@ def measure(scene, steps, warmup, fused=True):
 
 def measure_batched(ens, steps, warmup, tune_steps=5):
     """The batched form over an ensemble: fixed launch shape (256-lane
-    workgroups, 8-bit digits), then the autotuned shape; per-stage times of
-    the tuned shape; and the same scenes stepped one at a time through the
-    single-fluid simulator (the plain form's way to run them)."""
+    workgroups, 8-bit digits, row schedules), then the autotuned shape, each
+    on a fresh simulator from the ensemble's initial state (tuning restores
+    the state, so both runs time the same particle configurations);
+    per-stage times of the tuned shape; and the same scenes stepped one at a
+    time through the single-fluid simulator (the plain form's way to run
+    them)."""
     out = {}
-    b = lib.BatchSimulator(ens)
     try:
-        b.set_tuning(256, 8, False, False)
         for label in ("fixed", "tuned"):
-            if label == "tuned":
-                t = b.autotune(trial_steps=tune_steps)
-                out["tuning"] = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
-                                 "flat_density": t["flat_density"], "flat_forces": t["flat_forces"],
-                                 "trials_ms": [[a, c, fd, ff, round(m, 4)] for a, c, fd, ff, m in t["trials"]]}
-            b.step(warmup)
-            b.synchronize()
-            t0 = time.perf_counter()
-            b.step(steps)
-            b.synchronize()
-            dt = time.perf_counter() - t0
-            out[label] = {"ms_per_step": 1e3 * dt / steps, "value": ens.n * steps / dt}
-        b.set_options(use_graph=False, stage_timing=True)
-        acc = None
-        for _ in range(5):
-            b.step(1)
-            t = b.stage_times()
-            acc = t if acc is None else {k: acc[k] + t[k] for k in t}
-        out["stage_ms"] = {k: round(v / 5, 5) for k, v in acc.items()}
-        pos, _ = b.state()
-        out["finite"] = bool(np.all(np.isfinite(pos)))
-        info = b.info()
+            b = lib.BatchSimulator(ens)
+            try:
+                if label == "fixed":
+                    b.set_tuning(256, 8, False, False)
+                else:
+                    t = b.autotune(trial_steps=tune_steps)
+                    out["tuning"] = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
+                                     "flat_density": t["flat_density"], "flat_forces": t["flat_forces"],
+                                     "trials_ms": [[a, c, fd, ff, round(m, 4)] for a, c, fd, ff, m in t["trials"]]}
+                b.step(warmup)
+                b.synchronize()
+                t0 = time.perf_counter()
+                b.step(steps)
+                b.synchronize()
+                dt = time.perf_counter() - t0
+                out[label] = {"ms_per_step": 1e3 * dt / steps, "value": ens.n * steps / dt}
+                if label == "tuned":
+                    b.set_options(use_graph=False, stage_timing=True)
+                    acc = None
+                    for _ in range(5):
+                        b.step(1)
+                        st = b.stage_times()
+                        acc = st if acc is None else {k: acc[k] + st[k] for k in st}
+                    out["stage_ms"] = {k: round(v / 5, 5) for k, v in acc.items()}
+                    pos, _ = b.state()
+                    out["finite"] = bool(np.all(np.isfinite(pos)))
+                    info = b.info()
+            finally:
+                b.close()
     finally:
-        b.close()
         lib.load().fg_set_tuning(256, 8)
         lib.load().fg_set_schedule(0, 0)
     # plain form: every scene alone through the single-fluid simulator

@@ -239,6 +239,27 @@ static GridDesc body_grid(fg_float3 cs, uint32_t nb) {
 
 using namespace fg;
 
+// Contact description of one fg_body_grid (K_FluidSim.cu:372-386 arguments).
+static BodyContact body_contact(const fg_body_grid& b) {
+    BodyContact c{};
+    c.enabled = b.numSpheres > 0;
+    c.pos = reinterpret_cast<const float4*>(b.orderedSpheres);
+    c.vel = reinterpret_cast<const float4*>(b.bodyVelocities);
+    c.start = b.gridStart;
+    c.end = b.gridEnd;
+    c.aabb = reinterpret_cast<const float4*>(b.bounds);
+    c.grid = body_grid(b.cellSize, b.numBuckets);
+    c.relation = make_int3(b.relation.x, b.relation.y, b.relation.z);
+    c.fluid_radius = b.fluidRadius;
+    c.body_radius = b.sphereRadius;
+    c.ks = b.kSpring;
+    c.kd = b.kDamping;
+    c.eps = 0.003f;
+    c.count = b.numSpheres;
+    c.nvel = b.numBodies;
+    return c;
+}
+
 extern "C" {
 
 // K_FluidSim.cu:615-680
@@ -302,30 +323,28 @@ void gSPHCalculateForcesWithSRB(fg_float4* oldPositions, fg_float4* oldVelocity,
 int fg_sph_forces_coupled(fg_float4* orderedPositions, fg_float4* orderedVelocity, fg_float2* density_pressure,
                           unsigned int* gridStart, unsigned int* gridEnd, const unsigned long long* neighborMasks,
                           fg_float4* vel, fg_float4* pos, unsigned int* particleId, unsigned int numParticles,
-                          float mass, float viscosity, const fg_body_grid* bodies) {
+                          float mass, float viscosity, const fg_body_grid* bodies,
+                          const fg_body_grid* staticBodies) {
     if (numParticles == 0) return FG_OK;
     if (!orderedPositions || !orderedVelocity || !density_pressure || !gridStart || !gridEnd || !vel ||
-        !particleId || !bodies) {
+        !particleId || (!bodies && !staticBodies)) {
         set_error(FG_ERR_INVALID_ARGUMENT, "fg_sph_forces_coupled: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    if (staticBodies && staticBodies->numSpheres > 0 &&
+        (!staticBodies->orderedSpheres || !staticBodies->gridStart || !staticBodies->gridEnd ||
+         staticBodies->numBuckets == 0)) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sph_forces_coupled: static body grid incomplete");
         return FG_ERR_INVALID_ARGUMENT;
     }
     Contacts c{};
     c.coupled = 1;
-    c.dyn.enabled = bodies->numSpheres > 0;
-    c.dyn.pos = reinterpret_cast<const float4*>(bodies->orderedSpheres);
-    c.dyn.vel = reinterpret_cast<const float4*>(bodies->bodyVelocities);
-    c.dyn.start = bodies->gridStart;
-    c.dyn.end = bodies->gridEnd;
-    c.dyn.aabb = reinterpret_cast<const float4*>(bodies->bounds);
-    c.dyn.grid = body_grid(bodies->cellSize, bodies->numBuckets);
-    c.dyn.relation = make_int3(bodies->relation.x, bodies->relation.y, bodies->relation.z);
-    c.dyn.fluid_radius = bodies->fluidRadius;
-    c.dyn.body_radius = bodies->sphereRadius;
-    c.dyn.ks = bodies->kSpring;
-    c.dyn.kd = bodies->kDamping;
-    c.dyn.eps = 0.003f;
-    c.dyn.count = bodies->numSpheres;
-    c.dyn.nvel = bodies->numBodies;
+    if (staticBodies) {
+        c.st = body_contact(*staticBodies);
+        c.st.vel = nullptr;  // static spheres do not move
+        c.st.nvel = 0;
+    }
+    if (bodies) c.dyn = body_contact(*bodies);
     return launch_forces(FG_F4(orderedPositions), FG_F4(orderedVelocity), FG_F2(density_pressure), gridStart, gridEnd,
                          FG_F4(vel), pos ? FG_F4(pos) : nullptr, particleId, numParticles, mass, viscosity,
                          neighborMasks, &c);

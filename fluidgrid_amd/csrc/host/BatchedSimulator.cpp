@@ -221,8 +221,12 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
+    // Every trial starts from the saved state, so all shapes are timed over
+    // the same particle configurations.
     auto time_shape = [&](const BatchTuning& t) {
         setTuning(t);
+        hip_check(hipMemcpyAsync(d_pos, bp, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
+        hip_check(hipMemcpyAsync(d_vel, bv, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
         doTimeSteps(2);  // capture + warm-up
         hip_check(hipEventRecord(e0, mStream), "event");
         doTimeSteps(trialSteps);
@@ -261,8 +265,9 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     setTuning(best);
-    hip_check(hipMemcpy(d_pos, bp, sizeof(float4) * mN, hipMemcpyDeviceToDevice), "restore");
-    hip_check(hipMemcpy(d_vel, bv, sizeof(float4) * mN, hipMemcpyDeviceToDevice), "restore");
+    hip_check(hipMemcpyAsync(d_pos, bp, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
+    hip_check(hipMemcpyAsync(d_vel, bv, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
+    hip_check(hipStreamSynchronize(mStream), "restore");
     dfree(bp);
     dfree(bv);
     mTiming = timing;

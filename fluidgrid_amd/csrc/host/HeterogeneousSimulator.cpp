@@ -158,13 +158,19 @@ bool HeterogeneousSimulator::coupled() const {
     return rb->getNumDynamicBodies() > 0;
 }
 
+bool HeterogeneousSimulator::hasStaticBodies() const {
+    return RigidBody::RigidBodySimulator::instance()->getNumStaticSpheres() > 0;
+}
+
 // HeterogeneousSimulator.cpp:161-205.  The coupled branch (commented out in
-// the reference, :174-186) is the dynamic-body grid of the rigid-body
+// the reference, :174-186) is the static and dynamic grids of the rigid-body
 // simulator with the fluid contact radius 2*pointRadius (:179).
-void HeterogeneousSimulator::calculateFluidForces(Fluid::GPUSPHFluid* fluid, bool, bool useDynamicRB) {
-    if (useDynamicRB) {
+void HeterogeneousSimulator::calculateFluidForces(Fluid::GPUSPHFluid* fluid, bool useStaticRB, bool useDynamicRB) {
+    if (useStaticRB || useDynamicRB) {
         const Application::CommonFluidParams& cp = Application::ApplicationData::instance()->getCommonFluidParams();
-        const fg_body_grid g = RigidBody::RigidBodySimulator::instance()->bodyGrid(2.0f * cp.pointRadius);
+        RigidBody::RigidBodySimulator* rb = RigidBody::RigidBodySimulator::instance();
+        const fg_body_grid g = rb->bodyGrid(2.0f * cp.pointRadius);
+        const fg_body_grid st = rb->staticGrid(2.0f * cp.pointRadius);
         fg_sph_forces_coupled(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
                               reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()),
                               reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashStart(),
@@ -172,7 +178,7 @@ void HeterogeneousSimulator::calculateFluidForces(Fluid::GPUSPHFluid* fluid, boo
                               reinterpret_cast<fg_float4*>(fluid->getVelocity()),
                               mOptions.fused ? reinterpret_cast<fg_float4*>(fluid->getPosition()) : nullptr,
                               fluid->getParticleID(), (unsigned)fluid->getNumParticles(), fluid->getMass(),
-                              fluid->getViscosity(), &g);
+                              fluid->getViscosity(), useDynamicRB ? &g : nullptr, useStaticRB ? &st : nullptr);
         return;
     }
     if (mOptions.fused) {
@@ -235,7 +241,7 @@ void HeterogeneousSimulator::enqueueCoupledOverlapped(Fluid::GPUSPHFluid* f) {
     rb->applyForces();
     fg_set_stream(mStream);
     hip_check(hipStreamWaitEvent(mStream, mSync[kBodyGrid], 0), "wait");
-    calculateFluidForces(f, false, true);
+    calculateFluidForces(f, hasStaticBodies(), true);
     if (!mOptions.fused) integrateFluidForces(f);
     hip_check(hipEventRecord(mSync[kForces], mStream), "event");
     hip_check(hipStreamWaitEvent(mBodyStream, mSync[kForces], 0), "wait");
@@ -256,6 +262,7 @@ void HeterogeneousSimulator::enqueueStep() {
     const bool t = mOptions.stage_timing;
     RigidBody::RigidBodySimulator* rb = RigidBody::RigidBodySimulator::instance();
     const bool withBodies = coupled();
+    const bool withStatic = hasStaticBodies();
     if (withBodies && mFluids.size() != 1)
         throw std::runtime_error("HeterogeneousSimulator: rigid-body coupling supports exactly one fluid");
     if (withBodies && !t && mOptions.overlap_bodies) {
@@ -276,7 +283,7 @@ void HeterogeneousSimulator::enqueueStep() {
                                                   f->getHashStart(), f->getHashEnd(), (unsigned)f->getNumParticles());
         }
         if (t) (void)hipEventRecord(mEv[5], mStream);
-        calculateFluidForces(f, false, withBodies);
+        calculateFluidForces(f, withStatic, withBodies);
         if (t) (void)hipEventRecord(mEv[6], mStream);
         if (!mOptions.fused) integrateFluidForces(f);
         if (t) (void)hipEventRecord(mEv[7], mStream);

@@ -3,6 +3,7 @@
 // bodies coupled to one fluid).
 #include "fluidgrid/RigidBodySimulator.h"
 
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <sstream>
@@ -105,8 +106,85 @@ RigidBodySimulator::~RigidBodySimulator() { release(); }
 
 void RigidBodySimulator::addElement(RigidBody* rb) {
     if (rb->isStatic())
-        throw std::invalid_argument("RigidBodySimulator: static bodies are not supported by this coupling");
-    mDynamic.push_back(rb);
+        mStatic.push_back(rb);
+    else
+        mDynamic.push_back(rb);
+}
+
+// RigidBodySimulator.cpp:146-214 (updateStaticElements): static spheres keyed
+// with the reference hash over st_bucket_size buckets, sorted, and the cell
+// ranges and sorted positions kept for the whole simulation.
+void RigidBodySimulator::buildStaticGrid() {
+    const Application::CommonRigidBodyParams& rp =
+        Application::ApplicationData::instance()->getCommonRigidBodyParams();
+    const Application::CommonFluidParams& fp = Application::ApplicationData::instance()->getCommonFluidParams();
+    std::vector<float4> world;
+    mStaticRadius = 0.0f;
+    for (RigidBody* b : mStatic) {
+        for (const float4& p : b->getWorldSpheres()) world.push_back(make_float4(p.x, p.y, p.z, 0.0f));
+        mStaticRadius = std::max(mStaticRadius, b->getParticleRadius());
+    }
+    mStaticCount = (unsigned)world.size();
+    mStBuckets = (unsigned)rp.getStNumBuckets();
+    if (mStaticCount == 0) return;
+    if (mStBuckets == 0) throw std::invalid_argument("RigidBodySimulator: st_bucket_size is empty");
+    const unsigned M = mStaticCount;
+    float4* d_src = dalloc<float4>(M);
+    float4* d_tmp = dalloc<float4>(M);
+    float4* d_zero = dalloc<float4>(1);
+    float4* d_ident = dalloc<float4>(1);
+    unsigned *d_k = dalloc<unsigned>(M), *d_i = dalloc<unsigned>(M);
+    d_stSorted = dalloc<float4>(M);
+    d_stStart = dalloc<unsigned>(mStBuckets);
+    d_stEnd = dalloc<unsigned>(mStBuckets);
+    d_stBounds = dalloc<float4>(2);
+    const float4 zero = make_float4(0, 0, 0, 0), ident = make_float4(0, 0, 0, 1);
+    hip_check(hipMemcpy(d_src, world.data(), sizeof(float4) * M, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(d_zero, &zero, sizeof(float4), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(d_ident, &ident, sizeof(float4), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemset(d_stEnd, 0, sizeof(unsigned) * mStBuckets), "memset");
+    const size_t wsBytes = fg_sort_workspace_size(M, key_bits_for(mStBuckets));
+    void* ws = nullptr;
+    hip_check(hipMalloc(&ws, wsBytes ? wsBytes : 256), "hipMalloc");
+    void* prev = fg_get_stream();
+    fg_set_stream(mStream);
+    // identity pose: the world positions are kept exactly
+    gCalcHashKeyWithOffset(d_k, d_i, reinterpret_cast<fg_float4*>(d_zero), reinterpret_cast<fg_float4*>(d_ident),
+                           reinterpret_cast<fg_float4*>(d_src), reinterpret_cast<fg_float4*>(d_tmp), M,
+                           abi3(rp.getCellSize()), (int)mStBuckets);
+    fg_sort_pairs_ws(d_k, d_i, M, key_bits_for(mStBuckets), ws, wsBytes, 1);
+    gFindCellOffsetOrderingPos(d_stStart, d_stEnd, d_k, d_i, reinterpret_cast<fg_float4*>(d_tmp),
+                               reinterpret_cast<fg_float4*>(d_stSorted), M, mStBuckets);
+    const float margin = 2.0f * fp.pointRadius + mStaticRadius + 0.003f;
+    fg_sphere_bounds(reinterpret_cast<fg_float4*>(d_src), M, margin * 1.01f, reinterpret_cast<fg_float4*>(d_stBounds));
+    fg_set_stream(prev);
+    fg_check("static body grid");
+    hip_check(hipStreamSynchronize(mStream), "sync");
+    (void)hipFree(ws);
+    dfree(d_src); dfree(d_tmp); dfree(d_zero); dfree(d_ident); dfree(d_k); dfree(d_i);
+}
+
+fg_body_grid RigidBodySimulator::staticGrid(float fluidRadius) const {
+    const Application::CommonRigidBodyParams& rp =
+        Application::ApplicationData::instance()->getCommonRigidBodyParams();
+    const Application::CommonFluidParams& fp = Application::ApplicationData::instance()->getCommonFluidParams();
+    const Application::SimulationParams& sp = Application::ApplicationData::instance()->getSimulationParams();
+    fg_body_grid g{};
+    g.orderedSpheres = reinterpret_cast<const fg_float4*>(d_stSorted);
+    g.bodyVelocities = nullptr;
+    g.gridStart = d_stStart;
+    g.gridEnd = d_stEnd;
+    g.bounds = reinterpret_cast<const fg_float4*>(d_stBounds);
+    g.cellSize = abi3(rp.getCellSize());
+    g.numBuckets = mStBuckets;
+    g.relation = fg_int3{fp.fluid_to_srb_relation.x, fp.fluid_to_srb_relation.y, fp.fluid_to_srb_relation.z};
+    g.numSpheres = mStaticCount;
+    g.numBodies = 0;
+    g.fluidRadius = fluidRadius;
+    g.sphereRadius = mStaticRadius;
+    g.kSpring = -sp.springForce;
+    g.kDamping = -sp.dampingForce;
+    return g;
 }
 
 void RigidBodySimulator::release() {
@@ -114,6 +192,8 @@ void RigidBodySimulator::release() {
     dfree(d_force); dfree(d_torque); dfree(d_inv); dfree(d_inertia); dfree(d_pf); dfree(d_pt); dfree(d_bounds);
     dfree(d_mass); dfree(d_sumF); dfree(d_sumT); dfree(d_massd); dfree(d_cmd); dfree(d_inertiad);
     dfree(d_offsets); dfree(d_key); dfree(d_id); dfree(d_start); dfree(d_end);
+    dfree(d_stSorted); dfree(d_stBounds); dfree(d_stStart); dfree(d_stEnd);
+    mStaticCount = 0;
     if (d_sortWs) (void)hipFree(d_sortWs);
     d_sortWs = nullptr;
 }
@@ -135,6 +215,7 @@ void RigidBodySimulator::reset() {
     }
     const size_t M = world.size();
     mBuckets = (unsigned)rp.getDnNumBuckets();
+    buildStaticGrid();
     if (B == 0) return;
     if (mBuckets == 0) throw std::invalid_argument("RigidBodySimulator: dn_bucket_size is empty");
 

@@ -193,6 +193,42 @@ int fg_sim_add_bodies(const fg_float4* worldSpheres, const unsigned int* bodyOff
     }
 }
 
+int fg_sim_add_static_spheres(const fg_float4* worldSpheres, unsigned int numSpheres, float sphereRadius,
+                              const fg_common_rigid_body_params* rigidParams, float springForce,
+                              float dampingForce) {
+    if (!sim() || !worldSpheres || !rigidParams || numSpheres == 0) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        Application::ApplicationData* ad = Application::ApplicationData::instance();
+        ad->setCommonRigidBodyParams(Application::abi_cast<Application::CommonRigidBodyParams>(*rigidParams));
+        Application::SimulationParams sp{dampingForce, springForce};
+        ad->setSimulationParams(sp);
+        std::vector<float4> s;
+        for (unsigned i = 0; i < numSpheres; i++)
+            s.push_back(make_float4(worldSpheres[i].x, worldSpheres[i].y, worldSpheres[i].z, 0.0f));
+        sim()->bodies.push_back(std::make_unique<RigidBody::RigidBody>(true, 0.0f, sphereRadius, std::move(s)));
+        RigidBody::RigidBodySimulator* rb = RigidBody::RigidBodySimulator::instance();
+        rb->addElement(sim()->bodies.back().get());
+        rb->setStream(Simulators::HeterogeneousSimulator::instance()->stream());
+        rb->reset();
+        Simulators::HeterogeneousSimulator::instance()->invalidateGraph();
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_static_grid(fg_body_grid* out) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    if (!out) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        const Application::CommonFluidParams& cp = Application::ApplicationData::instance()->getCommonFluidParams();
+        *out = RigidBody::RigidBodySimulator::instance()->staticGrid(2.0f * cp.pointRadius);
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 int fg_sim_body_state(fg_float4* positions, fg_float4* velocities, fg_float4* orientations,
                       fg_float4* angularMomenta) {
     if (!sim()) return FG_ERR_NOT_INITIALIZED;

@@ -85,6 +85,8 @@ SIGNATURES = {
     "fg_sim_stage_times": (i32, [vp]),
     "fg_sim_shutdown": (i32, []),
     "fg_sim_add_bodies": (i32, [vp, vp, u32, vp, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
+    "fg_sim_add_static_spheres": (i32, [vp, u32, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
+    "fg_sim_static_grid": (i32, [C.POINTER(abi.BodyGrid)]),
     "fg_sim_body_state": (i32, [vp, vp, vp, vp]),
     "fg_sim_body_sums": (i32, [vp, vp]),
     "fg_sim_body_mass_properties": (i32, [vp, vp]),
@@ -98,7 +100,7 @@ SIGNATURES = {
     "gReorderForceTorque": (None, [vp, vp, vp, vp, vp, u32]),
     "gAddForceTorque": (None, [vp, vp, vp, vp, vp, vp, sz]),
     "gIntegrateRB": (None, [vp, vp, vp, vp, sz, f32, abi.Float4, abi.Float3, abi.Float3, vp, vp, vp, vp, vp, f32]),
-    "fg_sph_forces_coupled": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32, vp]),
+    "fg_sph_forces_coupled": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32, vp, vp]),
     "fg_body_forces_from_fluid": (i32, [vp, vp, vp, vp, vp, u32, u32, f32, vp, vp, vp, vp, u32, f32, abi.Float3,
                                         u32, abi.Int3, f32, f32]),
     "fg_body_reduce_f64": (i32, [vp, vp, vp, u32, vp, vp, vp, vp]),
@@ -304,6 +306,30 @@ class Simulator:
                                        bodies.sphere_radius, C.byref(bodies.rb), bodies.spring, bodies.damping),
               "fg_sim_add_bodies")
         self.nbodies = bodies.b
+
+    def add_static(self, bodies):
+        """Add the static spheres of a scenes.BodySet (bodies.static) as one
+        static body the fluid collides with."""
+        sph = np.ascontiguousarray(bodies.static, np.float32)
+        check(load().fg_sim_add_static_spheres(sph.ctypes.data, sph.shape[0], bodies.sphere_radius,
+                                               C.byref(bodies.rb), bodies.spring, bodies.damping),
+              "fg_sim_add_static_spheres")
+
+    def static_grid(self):
+        """The simulator's static grid, downloaded: (sorted centres, start,
+        end, numBuckets)."""
+        g = abi.BodyGrid()
+        check(load().fg_sim_static_grid(C.byref(g)), "fg_sim_static_grid")
+        m, nb = g.numSpheres, g.numBuckets
+        if m == 0:
+            return None
+        L = load()
+        pos = np.empty((m, 4), np.float32)
+        start = np.empty(nb, np.uint32)
+        end = np.empty(nb, np.uint32)
+        for a, p in ((pos, g.orderedSpheres), (start, g.gridStart), (end, g.gridEnd)):
+            check(L.fg_memcpy_dtoh(a.ctypes.data, p, a.nbytes), "download")
+        return pos, start, end, nb, g
 
     def body_state(self):
         b = self.nbodies

@@ -255,6 +255,20 @@ def box_spheres(edge, d):
     return _surface_voxels(sdf, (-ext,) * 3, (ext,) * 3, d)
 
 
+def cylinder_spheres(radius, height, d):
+    """Surface-voxel sample of a closed cylinder (axis +y, base at y = 0,
+    centred on the y axis), the static obstacle of the obstacle scenes."""
+    ext = radius + d
+
+    def sdf(x, y, z):
+        dr = np.sqrt(x * x + z * z) - radius
+        dy = np.abs(y - 0.5 * height) - 0.5 * height
+        outside = np.sqrt(np.maximum(dr, 0.0) ** 2 + np.maximum(dy, 0.0) ** 2)
+        return outside + np.minimum(np.maximum(dr, dy), 0.0)
+
+    return _surface_voxels(sdf, (-ext, -d, -ext), (ext, height + d, ext), d)
+
+
 def quat_rotate(q, v):
     """Rotate vectors v (N,3) by unit quaternion q = (x, y, z, w)."""
     u = np.asarray(q[:3], np.float64)
@@ -267,7 +281,7 @@ def quat_rotate(q, v):
 @dataclass
 class BodySet:
     """Dynamic rigid bodies as world-space sphere sets (body-major order)."""
-    world: np.ndarray          # (M,4) float32, w = body index
+    world: np.ndarray          # (M,4) float32, w = body index (M may be 0)
     offsets: np.ndarray        # (B+1,) uint32: spheres of body b are [offsets[b], offsets[b+1])
     mass: np.ndarray           # (B,) float64
     sphere_radius: float
@@ -276,6 +290,7 @@ class BodySet:
     damping: float             # SimulationParams.dampingForce (kDamping = -damping)
     fluid_radius: float        # contact radius of a fluid particle
     names: list = field(default_factory=list)
+    static: np.ndarray = None  # (K,4) float32 static sphere centres (w = 0), or None
 
     @property
     def m(self):
@@ -286,16 +301,26 @@ class BodySet:
         return int(self.mass.shape[0])
 
 
-def make_bodies(sets, masses, h, common, spring=2000.0, damping=50.0, names=None, buckets=(64, 64, 64)):
+def make_bodies(sets, masses, h, common, spring=2000.0, damping=50.0, names=None, buckets=(64, 64, 64),
+                static_sets=None, st_buckets=(64, 64, 64)):
     """Pack sphere sets (list of (K,3) world centres) into a BodySet with the
-    coupling parameters used by the coupled configurations.
+    coupling parameters used by the coupled configurations; static_sets (list
+    of (K,3) centres) become the static spheres, hashed over st_buckets.
 
     The coupled passes take the fluid contact radius as 2 * pointRadius
     (HeterogeneousSimulator.cpp:179, RigidBodySimulator.cpp:517), so this sets
     common.pointRadius = fluid_radius / 2 on the fluid parameters passed in."""
-    world = np.concatenate([np.concatenate([s, np.full((len(s), 1), b)], 1) for b, s in enumerate(sets)]).astype(
-        np.float32)
+    if sets:
+        world = np.concatenate([np.concatenate([s, np.full((len(s), 1), b)], 1)
+                                for b, s in enumerate(sets)]).astype(np.float32)
+    else:
+        world = np.zeros((0, 4), np.float32)
     offsets = np.concatenate([[0], np.cumsum([len(s) for s in sets])]).astype(np.uint32)
+    static = None
+    if static_sets:
+        st = np.concatenate([np.asarray(s, np.float64) for s in static_sets])
+        static = np.zeros((len(st), 4), np.float32)
+        static[:, :3] = st
     sphere_r = 0.5 * h
     fluid_r = 0.25 * h
     common.pointRadius = float(np.float32(0.5 * fluid_r))
@@ -303,14 +328,14 @@ def make_bodies(sets, masses, h, common, spring=2000.0, damping=50.0, names=None
     contact = fluid_r + sphere_r + 0.003
     rb = abi.CommonRigidBodyParams()
     rb.rb_to_fluid_relation = abi.i3(1, 1, 1)
-    rb.st_bucket_size = abi.i3(1, 1, 1)
+    rb.st_bucket_size = abi.i3(*st_buckets) if static is not None else abi.i3(1, 1, 1)
     rb.dn_bucket_size = abi.i3(*buckets)
-    rb.st_keyBits = key_bits(buckets[0] * buckets[1] * buckets[2])
+    rb.st_keyBits = key_bits(rb.st_bucket_size.x * rb.st_bucket_size.y * rb.st_bucket_size.z)
     rb.velocity_limit = common.velocity_limit
     rb.volume_min = abi.Float3(*common.volume_min)
     rb.volume_max = abi.Float3(*common.volume_max)
     g = common.global_external_force.y
-    m0 = float(masses[0])
+    m0 = float(masses[0]) if len(masses) else 0.0
     rb.global_external_force.x, rb.global_external_force.y = 0.0, float(np.float32(g * m0))
     rb.global_external_force.z, rb.global_external_force.w = 0.0, 0.0
     rb.simulation_scale = common.simulation_scale
@@ -319,7 +344,7 @@ def make_bodies(sets, masses, h, common, spring=2000.0, damping=50.0, names=None
     # the 27-cell stencil finds every contact.
     rb.particleRadius = float(np.float32(contact * 1.001))
     return BodySet(world, offsets, np.asarray(masses, np.float64), float(np.float32(sphere_r)), rb, spring, damping,
-                   float(np.float32(fluid_r)), names or [])
+                   float(np.float32(fluid_r)), names or [], static)
 
 
 def coupled_config_scene(cfg_id=1, seed=None):

@@ -46,9 +46,10 @@ public:
 
     // Times launch shapes and schedules by coordinate descent (workgroup
     // size, then the density and force schedules, then the digit width;
-    // trialSteps graph steps each after a warm-up) and keeps the fastest; the
-    // particle state is restored afterwards, so tuning does not advance the
-    // simulation.  No choice changes any result bit.
+    // trialSteps graph steps each after a warm-up, every trial from the
+    // current particle state) and keeps the fastest; the state is restored
+    // afterwards, so tuning does not advance the simulation.  No choice
+    // changes any result bit.
     BatchTuning autotune(int trialSteps, std::vector<std::pair<BatchTuning, float>>* trials = nullptr);
     void setTuning(const BatchTuning& t);
     const BatchTuning& tuning() const { return mTuning; }

@@ -1,5 +1,7 @@
-// RigidBodySimulator.h -- dynamic rigid bodies coupled to the fluid, after
-// the reference's Simulators/RigidBodySimulator.h:130-165 and RigidBody.h.
+// RigidBodySimulator.h -- rigid bodies coupled to the fluid, after the
+// reference's Simulators/RigidBodySimulator.h:130-165 and RigidBody.h:
+// dynamic bodies move under the fluid's forces; static bodies are a fixed
+// sphere grid the fluid collides with.
 //
 // The reference wires this path only in commented code
 // (HeterogeneousSimulator.cpp:57, 82-102, 113-126); here it is live:
@@ -72,9 +74,13 @@ public:
 
     int getNumDynamicBodies() const { return (int)mMass.size(); }
     int getNumDynamicSpheres() const { return (int)mRel.size(); }
+    int getNumStaticSpheres() const { return (int)mStaticCount; }
     float4* getDynamicVelocities() { return d_vel; }
     // The dynamic-body grid for the coupled fluid force pass.
     fg_body_grid bodyGrid(float fluidRadius) const;
+    // The static-body grid (RigidBodySimulator.cpp:146-214): built once at
+    // reset(), REF_XOR keys over st_bucket_size buckets, no velocities.
+    fg_body_grid staticGrid(float fluidRadius) const;
 
     // Extensions: state readback (positions, velocities, orientations,
     // angular momenta: float4 per body) and the FP64 sums of the last step.
@@ -88,6 +94,7 @@ private:
     ~RigidBodySimulator();
 
     std::vector<RigidBody*> mDynamic;
+    std::vector<RigidBody*> mStatic;
     std::vector<float4> mRel;          // spheres relative to the centre of mass, w = body
     std::vector<unsigned int> mOffsets;
     std::vector<float> mMass;
@@ -106,6 +113,12 @@ private:
     void* d_sortWs = nullptr;
     size_t mSortWsBytes = 0;
     unsigned int mBuckets = 0;
+    // static bodies
+    float4 *d_stSorted = nullptr, *d_stBounds = nullptr;
+    unsigned int *d_stStart = nullptr, *d_stEnd = nullptr;
+    unsigned int mStaticCount = 0, mStBuckets = 0;
+    float mStaticRadius = 0.0f;
+    void buildStaticGrid();
 
     static RigidBodySimulator* mInstance;
 };

@@ -316,11 +316,12 @@ int fg_sph_forces_integrate_nbr(fg_float4* orderedPositions, fg_float4* orderedV
                                 const unsigned long long* neighborMasks, fg_float4* vel, fg_float4* pos,
                                 unsigned int* particleId, unsigned int numParticles, float mass, float viscosity);
 
-/* Body-sphere grid seen by the coupled fluid force pass (the dynamic-body
- * arguments of gSPHCalculateForcesWithSRB, with explicit counts). */
+/* Body-sphere grid seen by the coupled fluid force pass (the static- or
+ * dynamic-body arguments of gSPHCalculateForcesWithSRB, with explicit
+ * counts).  A static grid has bodyVelocities NULL and numBodies 0. */
 typedef struct fg_body_grid {
     const fg_float4* orderedSpheres;   /* world sphere centres in grid order, w = body index */
-    const fg_float4* bodyVelocities;   /* [numBodies] */
+    const fg_float4* bodyVelocities;   /* [numBodies]; NULL for static spheres */
     const unsigned int* gridStart;     /* [numBuckets] */
     const unsigned int* gridEnd;       /* [numBuckets] */
     const fg_float4* bounds;           /* [2] expanded sphere bounds (fg_sphere_bounds) or NULL */
@@ -333,14 +334,17 @@ typedef struct fg_body_grid {
     float kSpring, kDamping;           /* F += (kSpring*l - kDamping*dot(n, dv)) n */
 } fg_body_grid;
 
-/* Coupled fluid forces: gSPHCalculateForcesWithSRB for the dynamic grid,
- * plus the optional neighbour masks of fg_sph_density_nbr, optional fused
- * integration (pos may be NULL) and sphere bounds that skip the body walk
- * for particles that cannot touch a body. */
+/* Coupled fluid forces: gSPHCalculateForcesWithSRB (K_FluidSim.cu:315, :616)
+ * for a static and a dynamic body grid (either may be NULL, not both; the
+ * static contact is added first, as in the reference), plus the optional
+ * neighbour masks of fg_sph_density_nbr, optional fused integration (pos may
+ * be NULL) and sphere bounds that skip the body walk for particles that
+ * cannot touch a body. */
 int fg_sph_forces_coupled(fg_float4* orderedPositions, fg_float4* orderedVelocity, fg_float2* density_pressure,
                           unsigned int* gridStart, unsigned int* gridEnd, const unsigned long long* neighborMasks,
                           fg_float4* vel, fg_float4* pos, unsigned int* particleId, unsigned int numParticles,
-                          float mass, float viscosity, const fg_body_grid* bodies);
+                          float mass, float viscosity, const fg_body_grid* bodies,
+                          const fg_body_grid* staticBodies);
 
 /* gCalculateRigidBodyForcesFromFluids with explicit sphere/body/fluid counts
  * (reads are bounded by them).  The fluid grid is walked with the current

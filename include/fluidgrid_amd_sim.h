@@ -58,6 +58,17 @@ int fg_sim_stage_times(float* out7);
 int fg_sim_add_bodies(const fg_float4* worldSpheres, const unsigned int* bodyOffsets, unsigned int numBodies,
                       const double* masses, float sphereRadius, const fg_common_rigid_body_params* rigidParams,
                       float springForce, float dampingForce);
+/* Static rigid spheres (one static body, RigidBody::isStatic; the static
+ * grid of RigidBodySimulator.cpp:146-214) that the fluid collides with:
+ * keyed with the reference hash over st_bucket_size buckets of the rigid
+ * cell size, sorted once here.  Every call adds a body; call before the
+ * first step (the rigid-body state is rebuilt). */
+int fg_sim_add_static_spheres(const fg_float4* worldSpheres, unsigned int numSpheres, float sphereRadius,
+                              const fg_common_rigid_body_params* rigidParams, float springForce,
+                              float dampingForce);
+/* The static grid built by fg_sim_add_static_spheres (device pointers owned
+ * by the simulator; numSpheres is 0 when there is none). */
+int fg_sim_static_grid(fg_body_grid* out);
 /* Per-body centre-of-mass position, velocity, orientation (x,y,z,w) and
  * angular momentum; any pointer may be NULL. */
 int fg_sim_body_state(fg_float4* positions, fg_float4* velocities, fg_float4* orientations,

@@ -572,16 +572,26 @@ V3<R> process_collision(const fg_float4& sp, const fg_float4& sv, R sr, const fg
     return F;
 }
 
-/* Coupled fluid forces, K_FluidSim.cu:315-413 (dynamic bodies only; the
- * static-body branch is the same call against a static grid with no
- * velocities). */
+/* Static-body grid of the coupled force pass: sorted sphere centres, cell
+ * ranges, grid and sphere radius (no velocities). */
+struct OStatic {
+    const fg_float4* pos;
+    const uint32_t* start;
+    const uint32_t* end;
+    OGrid grid;
+    float radius;
+};
+
+/* Coupled fluid forces, K_FluidSim.cu:315-413: the static grid (st, may be
+ * null; contact against spheres with no velocity) is added to the external
+ * force before the dynamic grid (rb_pos may be null). */
 template <class R, class In>
 void forces_coupled_t(const fg_float4* sp, const fg_float4* sv, const In* rho_p, const uint32_t* start,
                       const uint32_t* end, R* v_out, const uint32_t* id, uint32_t n, float m, float mu,
                       const fg_common_fluid_params& cp, const fg_kernels_cache& kc, int mode, float fluid_radius,
                       fg_int3 relation, float body_radius, float ks, float kd, const fg_float4* rb_pos,
                       const fg_float4* rb_vel, const uint32_t* rb_start, const uint32_t* rb_end, const OGrid& rbg,
-                      R* acc_out) {
+                      const OStatic* st, R* acc_out) {
     const R scale = (R)cp.simulation_scale, h = (R)cp.kernel_radius;
     const R kSpiky = (R)kc.kSpikyGradient, kVisc = (R)kc.kViscosityLaplacian;
     const R vlim = (R)cp.velocity_limit, dt = (R)cp.deltaTimeIntegration;
@@ -622,9 +632,15 @@ void forces_coupled_t(const fg_float4* sp, const fg_float4* sv, const In* rho_p,
                 }
                 internal += part;
             }
-            V3<R> external = process_collision<R>(pos, vel, (R)fluid_radius, rb_pos, rb_vel, (R)body_radius, (R)ks,
-                                                  (R)kd, relation, rb_start, rb_end, rbg, true, false, 0,
-                                                  (R)0.003f);
+            V3<R> external = v3<R>(0, 0, 0);
+            if (st)
+                external += process_collision<R>(pos, vel, (R)fluid_radius, st->pos, nullptr, (R)st->radius, (R)ks,
+                                                 (R)kd, relation, st->start, st->end, st->grid, false, false, 0,
+                                                 (R)0.003f);
+            if (rb_pos)
+                external += process_collision<R>(pos, vel, (R)fluid_radius, rb_pos, rb_vel, (R)body_radius, (R)ks,
+                                                 (R)kd, relation, rb_start, rb_end, rbg, true, false, 0,
+                                                 (R)0.003f);
             total = internal * (R)m + g + external;
             V3<R> w = wall_terms<R>(pos, vel, total, cp);
             total += w;
@@ -678,9 +694,10 @@ void orc_forces_coupled_f32(const fg_float4* sp, const fg_float4* sv, const floa
                             const fg_common_fluid_params* cp, const fg_kernels_cache* kc, int mode,
                             float fluid_radius, fg_int3 relation, float body_radius, float ks, float kd,
                             const fg_float4* rb_pos, const fg_float4* rb_vel, const uint32_t* rb_start,
-                            const uint32_t* rb_end, const OGrid* rbg, float* acc_out) {
+                            const uint32_t* rb_end, const OGrid* rbg, const OStatic* st, float* acc_out) {
     forces_coupled_t<float, float>(sp, sv, rho_p, start, end, v_out, id, n, m, mu, *cp, *kc, mode, fluid_radius,
-                                   relation, body_radius, ks, kd, rb_pos, rb_vel, rb_start, rb_end, *rbg, acc_out);
+                                   relation, body_radius, ks, kd, rb_pos, rb_vel, rb_start, rb_end, *rbg, st,
+                                   acc_out);
 }
 
 void orc_forces_coupled_f64(const fg_float4* sp, const fg_float4* sv, const double* rho_p, const uint32_t* start,
@@ -688,9 +705,9 @@ void orc_forces_coupled_f64(const fg_float4* sp, const fg_float4* sv, const doub
                             const fg_common_fluid_params* cp, const fg_kernels_cache* kc, int mode,
                             float fluid_radius, fg_int3 relation, float body_radius, float ks, float kd,
                             const fg_float4* rb_pos, const fg_float4* rb_vel, const uint32_t* rb_start,
-                            const uint32_t* rb_end, const OGrid* rbg, double* acc_out) {
+                            const uint32_t* rb_end, const OGrid* rbg, const OStatic* st, double* acc_out) {
     forces_coupled_t<double, double>(sp, sv, rho_p, start, end, v_out, id, n, m, mu, *cp, *kc, mode, fluid_radius,
-                                     relation, body_radius, ks, kd, rb_pos, rb_vel, rb_start, rb_end, *rbg,
+                                     relation, body_radius, ks, kd, rb_pos, rb_vel, rb_start, rb_end, *rbg, st,
                                      acc_out);
 }
 

@@ -213,6 +213,11 @@ def grid_of_bodies(cell, nb):
     return g
 
 
+class OStatic(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("start", C.c_void_p), ("end", C.c_void_p), ("grid", OGrid),
+                ("radius", C.c_float)]
+
+
 def _rigid_sigs(L):
     cpp = C.POINTER(abi.CommonFluidParams)
     kcp = C.POINTER(abi.KernelsCache)
@@ -220,10 +225,12 @@ def _rigid_sigs(L):
     L.orc_body_transform.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, og, _f32p, _u32p, _u32p]
     L.orc_forces_coupled_f32.argtypes = [_f32p, _f32p, _f32p, _u32p, _u32p, _f32p, _u32p, C.c_uint32, C.c_float,
                                          C.c_float, cpp, kcp, C.c_int, C.c_float, abi.Int3, C.c_float, C.c_float,
-                                         C.c_float, _f32p, _f32p, _u32p, _u32p, og, C.c_void_p]
+                                         C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, og, C.c_void_p,
+                                         C.c_void_p]
     L.orc_forces_coupled_f64.argtypes = [_f32p, _f32p, _f64p, _u32p, _u32p, _f64p, _u32p, C.c_uint32, C.c_float,
                                          C.c_float, cpp, kcp, C.c_int, C.c_float, abi.Int3, C.c_float, C.c_float,
-                                         C.c_float, _f32p, _f32p, _u32p, _u32p, og, C.c_void_p]
+                                         C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, og, C.c_void_p,
+                                         C.c_void_p]
     L.orc_body_forces_from_fluid.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, C.c_uint32, C.c_float, _f32p, _f32p,
                                              _u32p, _u32p, C.c_float, og, abi.Int3, C.c_float, C.c_float]
     L.orc_body_reduce_f64.argtypes = [_f32p, _f32p, C.c_uint32, C.c_uint32, _f64p, _f64p]
@@ -254,20 +261,55 @@ def body_transform(rel, cm, q, grid):
     return world, key, pid
 
 
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
 def forces_coupled(spos, svel, rho_p, start, end, pid, fluid, common, kernels, mode, fluid_radius, relation,
                    body_radius, ks, kd, rb_pos, rb_vel, rb_start, rb_end, rb_grid, f64=False, v_out_init=None,
-                   want_acc=False):
+                   want_acc=False, static=None):
+    """Coupled fluid forces (K_FluidSim.cu:315-413).  rb_pos None means no
+    dynamic grid; static = (sorted sphere centres, start, end, OGrid,
+    sphere radius) adds the static grid before the dynamic one."""
     spos, svel = _c(spos, np.float32), _c(svel, np.float32)
     n = spos.shape[0]
     dt = np.float64 if f64 else np.float32
     v_out = np.zeros((n, 4), dt) if v_out_init is None else _c(v_out_init, dt).copy()
     acc = np.zeros((n, 3), dt) if want_acc else None
     fn = _rl().orc_forces_coupled_f64 if f64 else _rl().orc_forces_coupled_f32
+    keep = []
+    if rb_pos is not None:
+        rb = [_c(rb_pos, np.float32), _c(rb_vel, np.float32), _c(rb_start, np.uint32), _c(rb_end, np.uint32)]
+    else:
+        rb = [None, None, None, None]
+        rb_grid = rb_grid if rb_grid is not None else OGrid()
+    keep.append(rb)
+    st_arg = None
+    if static is not None:
+        sp_, ss_, se_, sg_, sr_ = static
+        arrs = (_c(sp_, np.float32), _c(ss_, np.uint32), _c(se_, np.uint32))
+        keep.append(arrs)
+        ost = OStatic(arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data, sg_, sr_)
+        keep.append(ost)
+        st_arg = C.addressof(ost)
     fn(spos, svel, _c(rho_p, dt), _c(start, np.uint32), _c(end, np.uint32), v_out, _c(pid, np.uint32), n,
        fluid.mass, fluid.viscosity, C.byref(common), C.byref(kernels), mode, fluid_radius, abi.Int3(*relation),
-       body_radius, ks, kd, _c(rb_pos, np.float32), _c(rb_vel, np.float32), _c(rb_start, np.uint32),
-       _c(rb_end, np.uint32), C.byref(rb_grid), acc.ctypes.data if want_acc else None)
+       body_radius, ks, kd, _ptr(rb[0]), _ptr(rb[1]), _ptr(rb[2]), _ptr(rb[3]), C.byref(rb_grid), st_arg,
+       acc.ctypes.data if want_acc else None)
     return v_out, acc
+
+
+def static_grid(spheres, cell, nb, common):
+    """Static-body grid of RigidBodySimulator.cpp:146-214: keys of the sphere
+    centres by the reference hash (gCalcHashKey, K_HashGrid.cu:26-38) over nb
+    buckets of edge cell, stable sort, cell ranges and sorted centres
+    (w = 0).  Returns (sorted centres, start, end, OGrid)."""
+    sph = _c(spheres, np.float32).copy()
+    sph[:, 3] = 0.0
+    key, pid = hash_keys(sph, nb, (cell, cell, cell), common, abi.KEY_REF_XOR)
+    skey, spid = sort_pairs(key, pid)
+    start, end, spos, _ = cell_offsets(skey, spid, sph, None, nb)
+    return spos, start, end, grid_of_bodies(cell, nb)
 
 
 def body_forces_from_fluid(rb_pos, rb_vel, rb_cm, radius, spos, svel, start, end, fluid_radius, fluid_grid,
@@ -319,6 +361,11 @@ class CoupledState:
     def __init__(self, scene, bodies):
         self.scene = scene
         self.bodies = bodies
+        self.static = None
+        if bodies.static is not None and len(bodies.static):
+            rb = bodies.rb
+            nst = rb.st_bucket_size.x * rb.st_bucket_size.y * rb.st_bucket_size.z
+            self.static = (*static_grid(bodies.static, rb.particleRadius, nst, scene.common), bodies.sphere_radius)
         self.pos, self.vel = scene.pos.copy(), scene.vel.copy()
         self.rho_p = np.zeros((scene.n, 2), np.float32)
         nb = bodies.b
@@ -345,8 +392,10 @@ def coupled_step(st):
     """One FP32 coupled step in the stage order of the HIP simulator
     (HeterogeneousSimulator.cpp:54-127 with the coupled branch of :174-186
     and RigidBodySimulator.cpp:451-611): fluid grid and density; body world
-    transform, grid and forces from the fluid; coupled fluid forces and
-    integration; per-body FP64 sums and body integration."""
+    transform, grid and forces from the fluid; coupled fluid forces (static
+    grid, then dynamic grid) and integration; per-body FP64 sums and body
+    integration.  With no dynamic bodies only the fluid advances, against
+    the static grid."""
     s, b = st.scene, st.bodies
     common, fluid, kern, mode = s.common, s.fluid, s.kernels, s.key_mode
     nb = common.num_buckets()
@@ -354,19 +403,26 @@ def coupled_step(st):
     skey, spid = sort_pairs(key, pid)
     start, end, spos, svel = cell_offsets(skey, spid, st.pos, st.vel, nb)
     rho_p = density(spos, start, end, fluid, common, kern, mode, rho_p_init=st.rho_p)
+    fluid_r = float(np.float32(2.0 * np.float32(common.pointRadius)))
+    ks, kd = float(np.float32(-b.spring)), float(np.float32(-b.damping))
+    if b.b == 0:
+        v_new, _ = forces_coupled(spos, svel, rho_p, start, end, spid, fluid, common, kern, mode, fluid_r,
+                                  tuple(common.fluid_to_srb_relation), b.sphere_radius, ks, kd, None, None, None,
+                                  None, None, v_out_init=st.vel, static=st.static)
+        st.pos = integrate(st.pos, v_new, common)
+        st.vel, st.rho_p = v_new, rho_p
+        return st
     cell = b.rb.particleRadius
     nbb = b.rb.dn_bucket_size.x * b.rb.dn_bucket_size.y * b.rb.dn_bucket_size.z
     bgrid = grid_of_bodies(cell, nbb)
     world, bkey, bpid = body_transform(st.rel, st.cm, st.orient, bgrid)
     bskey, bspid = sort_pairs(bkey, bpid)
     bstart, bend, bpos, _ = cell_offsets(bskey, bspid, world, None, nbb)
-    fluid_r = float(np.float32(2.0 * np.float32(common.pointRadius)))
-    ks, kd = float(np.float32(-b.spring)), float(np.float32(-b.damping))
     f, t = body_forces_from_fluid(world, st.bvel, st.cm, b.sphere_radius, spos, svel, start, end, fluid_r,
                                   grid_of_fluid(common, mode), tuple(b.rb.rb_to_fluid_relation), ks, kd)
     v_new, _ = forces_coupled(spos, svel, rho_p, start, end, spid, fluid, common, kern, mode, fluid_r,
                               tuple(common.fluid_to_srb_relation), b.sphere_radius, ks, kd, bpos, st.bvel, bstart,
-                              bend, bgrid, v_out_init=st.vel)
+                              bend, bgrid, v_out_init=st.vel, static=st.static)
     st.pos = integrate(st.pos, v_new, common)
     st.vel, st.rho_p = v_new, rho_p
     sum_f = np.zeros((b.b, 3))

@@ -142,7 +142,7 @@ def test_coupled_fluid_forces_match_oracle(fglib, orc):
         g.fluidRadius, g.sphereRadius, g.kSpring, g.kDamping = b.fluid_radius, b.sphere_radius, ks, kd
         fglib.check(L.fg_sph_forces_coupled(d["spos"].ptr, d["svel"].ptr, d["rp"].ptr, d["start"].ptr,
                                             d["end"].ptr, masks.ptr, vel.ptr, None, d["sp"].ptr, s.n, s.fluid.mass,
-                                            s.fluid.viscosity, C.byref(g)), "coupled forces")
+                                            s.fluid.viscosity, C.byref(g), None), "coupled forces")
         v_gpu = vel.numpy()
         scale = np.zeros(s.n)
         scale[sp] = np.linalg.norm(acc64, axis=1) + mag + ext
