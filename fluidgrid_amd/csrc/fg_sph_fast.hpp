@@ -58,7 +58,8 @@ __device__ __forceinline__ void write_back(uint32_t o, uint32_t n, const float4&
 // cx-1..cx+1 at (cy+dy, cz+dz); its particles are sorted[row.begin, row.end).
 struct RowWalker {
     float gx_lo, gx_hi;  // squared scaled distance to the x-neighbour cells
-    float gy[3], gz[3];  // squared scaled distance to the y/z neighbour slabs
+    float gy_lo, gy_hi;  // ... to the y-neighbour slabs (0 for the own slab)
+    float gz_lo, gz_hi;  // ... to the z-neighbour slabs
     int cx, cy, cz;
 };
 
@@ -78,12 +79,10 @@ __device__ __forceinline__ RowWalker make_rows(const float4& p, const fg_common_
     const float lz = cp.volume_min.z + (float)c.z * cp.cell_size.z, hz = lz + cp.cell_size.z;
     w.gx_lo = sq(fmaxf(0.0f, p.x - lx) * s);
     w.gx_hi = sq(fmaxf(0.0f, hx - p.x) * s);
-    w.gy[0] = sq(fmaxf(0.0f, p.y - ly) * s);
-    w.gy[1] = 0.0f;
-    w.gy[2] = sq(fmaxf(0.0f, hy - p.y) * s);
-    w.gz[0] = sq(fmaxf(0.0f, p.z - lz) * s);
-    w.gz[1] = 0.0f;
-    w.gz[2] = sq(fmaxf(0.0f, hz - p.z) * s);
+    w.gy_lo = sq(fmaxf(0.0f, p.y - ly) * s);
+    w.gy_hi = sq(fmaxf(0.0f, hy - p.y) * s);
+    w.gz_lo = sq(fmaxf(0.0f, p.z - lz) * s);
+    w.gz_hi = sq(fmaxf(0.0f, hz - p.z) * s);
     return w;
 }
 
@@ -98,7 +97,11 @@ __device__ __forceinline__ bool row_range(const RowWalker& w, int dy, int dz, fl
                                           uint32_t n, uint32_t& rb, uint32_t& re) {
     const int cy = w.cy + dy, cz = w.cz + dz;
     if (cy < 0 || cz < 0 || cy >= cp.bucket_size.y || cz >= cp.bucket_size.z) return false;
-    const float gyz = w.gy[dy + 1] + w.gz[dz + 1];
+    // selects, not an array indexed by the row: the row loops are not always
+    // unrolled, and an indexed array would live in scratch memory
+    const float gy = dy < 0 ? w.gy_lo : (dy > 0 ? w.gy_hi : 0.0f);
+    const float gz = dz < 0 ? w.gz_lo : (dz > 0 ? w.gz_hi : 0.0f);
+    const float gyz = gy + gz;
     if (gyz >= cut) return false;
     int x0 = w.cx - 1, x1 = w.cx + 1;
     if (x0 < 0 || gyz + w.gx_lo >= cut) x0 = w.cx;

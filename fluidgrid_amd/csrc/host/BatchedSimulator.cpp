@@ -223,7 +223,7 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
     hip_check(hipEventCreate(&e1), "hipEventCreate");
     // Every trial starts from the saved state, so all shapes are timed over
     // the same particle configurations.
-    auto time_shape = [&](const BatchTuning& t) {
+    auto time_shape = [&](const BatchTuning& t, bool record = true) {
         setTuning(t);
         hip_check(hipMemcpyAsync(d_pos, bp, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
         hip_check(hipMemcpyAsync(d_vel, bv, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
@@ -235,9 +235,11 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
         float ms = 0;
         hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
         ms /= (float)trialSteps;
-        if (trials) trials->push_back({t, ms});
+        if (trials && record) trials->push_back({t, ms});
         return ms;
     };
+    // one discarded trial first: the clocks of an idle GPU ramp up over it
+    (void)time_shape(mTuning, false);
     // Coordinate descent: workgroup size first, then the digit width.
     BatchTuning best = mTuning;
     float best_ms = 1e30f;
