@@ -1,7 +1,8 @@
 // fg_rigid.hip -- two-way fluid/rigid-body coupling (the work of
 // GPU_Kernels/K_RigidBodySim.cu and the coupled branch of K_FluidSim.cu):
 //  * fluid forces with penalty contact against static and dynamic body grids;
-//  * per body sphere: penalty force from the fluid and torque about the body
+//  * per body sphere: penalty force from the fluid, from the spheres of the
+//    other bodies and from the static spheres, and torque about the body
 //    centre of mass;
 //  * per-body force/torque sums in FP64 (replacing the CUDPP FP32 segmented
 //    scan), one workgroup per body with a fixed reduction order;
@@ -41,6 +42,58 @@ __global__ void __launch_bounds__(256)
     float4 t = vec3_cross(dir, make_float4(F.x, F.y, F.z, 0.0f));
     t.w = (float)b;
     out_t[i] = t;
+}
+
+__device__ __forceinline__ bool inside_box(const float4& p, const float4& lo, const float4& hi) {
+    return p.x >= lo.x && p.y >= lo.y && p.z >= lo.z && p.x <= hi.x && p.y <= hi.y && p.z <= hi.z;
+}
+
+// K_RigidBodySim.cu:18-61: per body sphere, penalty contact against the
+// spheres of the other dynamic bodies (those of its own body are skipped by
+// w) and against the static spheres; torque about the body's centre of mass.
+// The broad phase replaces the host one of RigidBodySimulator.cpp:334-406:
+// with body bounds (per body, the box of its sphere centres expanded by the
+// contact distance) the dynamic walk is skipped for a sphere inside no other
+// body's box, and the static walk for a sphere outside the static box; a
+// skipped walk would only have added +0.  ACCUM adds the force and torque to
+// what out_f/out_t hold (the fluid's part), else they are written.
+template <bool ACCUM>
+__global__ void __launch_bounds__(256)
+    k_body_forces_from_bodies(const float4* __restrict__ rb_pos, const float4* __restrict__ rb_vel,
+                              const float4* __restrict__ rb_cm, float4* __restrict__ out_f,
+                              float4* __restrict__ out_t, uint32_t m, uint32_t nbodies, float radius, BodyContact dyn,
+                              BodyContact st, const float4* __restrict__ body_bounds, uint32_t nbounds) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const float4 pos = rb_pos[i];
+    const int b = f2i_sat(pos.w);
+    const bool ok = b >= 0 && (uint32_t)b < nbodies;
+    const float4 vel = ok ? rb_vel[b] : make_float4(0, 0, 0, 0);
+    const float4 cm = ok ? rb_cm[b] : make_float4(0, 0, 0, 0);
+    F3 F = f3(0.0f, 0.0f, 0.0f);
+    if (dyn.enabled) {
+        bool near = body_bounds == nullptr;
+        for (uint32_t k = 0; !near && k < nbounds; k++)
+            if ((int)k != b) near = inside_box(pos, body_bounds[2 * k], body_bounds[2 * k + 1]);
+        if (near)
+            F = add(F, contact_stencil<false>(pos, vel, radius, dyn.pos, dyn.vel, dyn.body_radius, dyn.ks, dyn.kd,
+                                              dyn.relation, dyn.start, dyn.end, dyn.grid, true, true, b, dyn.eps,
+                                              dyn.count, dyn.nvel));
+    }
+    if (st.enabled && (st.aabb == nullptr || inside_box(pos, st.aabb[0], st.aabb[1])))
+        F = add(F, contact_stencil<false>(pos, vel, radius, st.pos, nullptr, st.body_radius, st.ks, st.kd,
+                                          st.relation, st.start, st.end, st.grid, false, false, 0, st.eps, st.count,
+                                          0u));
+    const float4 dir = make_float4(pos.x - cm.x, pos.y - cm.y, pos.z - cm.z, pos.w - cm.w);
+    const float4 t = vec3_cross(dir, make_float4(F.x, F.y, F.z, 0.0f));
+    if (ACCUM) {
+        const float4 f0 = out_f[i], t0 = out_t[i];
+        out_f[i] = make_float4(f0.x + F.x, f0.y + F.y, f0.z + F.z, (float)b);
+        out_t[i] = make_float4(t0.x + t.x, t0.y + t.y, t0.z + t.z, (float)b);
+    } else {
+        out_f[i] = make_float4(F.x, F.y, F.z, (float)b);
+        out_t[i] = make_float4(t.x, t.y, t.z, (float)b);
+    }
 }
 
 // K_RigidBodySim.cu:102-111
@@ -227,6 +280,39 @@ __global__ void __launch_bounds__(1024) k_sphere_bounds(const float4* __restrict
     }
 }
 
+// Per-body bounds: out[2b] = min, out[2b+1] = max of the centres of the
+// spheres [offset[b], offset[b+1]), expanded by margin; one workgroup per
+// body.  A body without spheres gets an empty box (min > max).
+__global__ void __launch_bounds__(256) k_body_bounds(const float4* __restrict__ p, const uint32_t* __restrict__ offset,
+                                                     uint32_t nbodies, float margin, float4* __restrict__ out) {
+    __shared__ float s[6][256];
+    const uint32_t b = blockIdx.x;
+    if (b >= nbodies) return;
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (uint32_t i = offset[b] + threadIdx.x; i < offset[b + 1]; i += 256) {
+        const float4 q = p[i];
+        lo[0] = fminf(lo[0], q.x); lo[1] = fminf(lo[1], q.y); lo[2] = fminf(lo[2], q.z);
+        hi[0] = fmaxf(hi[0], q.x); hi[1] = fmaxf(hi[1], q.y); hi[2] = fmaxf(hi[2], q.z);
+    }
+    for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = lo[k];
+        s[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * b] = make_float4(s[0][0] - margin, s[1][0] - margin, s[2][0] - margin, 0.0f);
+        out[2 * b + 1] = make_float4(s[3][0] + margin, s[4][0] + margin, s[5][0] + margin, 0.0f);
+    }
+}
+
 static GridDesc body_grid(fg_float3 cs, uint32_t nb) {
     GridDesc g{};
     g.cs = make_float3(cs.x, cs.y, cs.z);
@@ -393,6 +479,111 @@ int fg_body_forces_from_fluid(fg_float4* spheres, fg_float4* bodyVelocities, fg_
                        fluidGridStart, fluidGridEnd, numFluid, fluidRadius, g,
                        make_int3(relation.x, relation.y, relation.z), kSpring, kDamping);
     return check_launch("fg_body_forces_from_fluid") ? FG_OK : fg_peek_error();
+}
+
+// K_RigidBodySim.cu:280-305.  The sphere array is both the query set and the
+// (sorted) grid array, as in the reference; the body count is unknown, so the
+// velocity and centre reads are bounded by the caller's data only.
+void gCalculateRigidBodyForcesFromRB(fg_float4* _in_RBPositions, fg_float4* _in_RBVelocites,
+                                     fg_float4* _in_RBCMPosition, fg_float4* _out_torque, fg_float4* _out_Forces,
+                                     unsigned int* _in_RBGridStart, unsigned int* _in_RBGridEnd,
+                                     unsigned int _in_NumParticles, fg_float3 _in_CellSize, float _in_Radius,
+                                     int _in_RBHashTableSize, int _in_SRBHashTableSize, fg_float4* _in_SRBPositions,
+                                     unsigned int* _in_SRBGridStart, unsigned int* _in_SRBGridEnd,
+                                     float _in_RBkSpring, float _in_RBkDamping, int _in_numSRB) {
+    if (_in_NumParticles == 0) return;
+    if (!_in_RBPositions || !_in_RBVelocites || !_in_RBCMPosition || !_out_torque || !_out_Forces ||
+        !_in_RBGridStart || !_in_RBGridEnd || _in_RBHashTableSize <= 0 ||
+        (_in_numSRB > 0 && (!_in_SRBPositions || !_in_SRBGridStart || !_in_SRBGridEnd || _in_SRBHashTableSize <= 0))) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "gCalculateRigidBodyForcesFromRB: null pointer or empty grid");
+        return;
+    }
+    BodyContact dyn{};
+    dyn.enabled = 1;
+    dyn.pos = FG_F4(_in_RBPositions);
+    dyn.vel = FG_F4(_in_RBVelocites);
+    dyn.start = _in_RBGridStart;
+    dyn.end = _in_RBGridEnd;
+    dyn.grid = body_grid(_in_CellSize, (uint32_t)_in_RBHashTableSize);
+    dyn.relation = make_int3(1, 1, 1);
+    dyn.body_radius = _in_Radius;
+    dyn.ks = _in_RBkSpring;
+    dyn.kd = _in_RBkDamping;
+    dyn.eps = 0.001f;
+    dyn.count = _in_NumParticles;
+    dyn.nvel = 0xFFFFFFFFu;
+    BodyContact st = dyn;
+    st.enabled = _in_numSRB > 0;
+    st.pos = FG_F4(_in_SRBPositions);
+    st.vel = nullptr;
+    st.start = _in_SRBGridStart;
+    st.end = _in_SRBGridEnd;
+    st.grid = body_grid(_in_CellSize, _in_numSRB > 0 ? (uint32_t)_in_SRBHashTableSize : 1u);
+    st.count = 0xFFFFFFFFu;
+    st.nvel = 0;
+    hipLaunchKernelGGL(k_body_forces_from_bodies<false>, dim3(grid_for(_in_NumParticles, 256)), dim3(256), 0,
+                       state().stream, FG_F4(_in_RBPositions), FG_F4(_in_RBVelocites), FG_F4(_in_RBCMPosition),
+                       FG_F4(_out_Forces), FG_F4(_out_torque), _in_NumParticles, 0xFFFFFFFFu, _in_Radius, dyn, st,
+                       (const float4*)nullptr, 0u);
+    check_launch("gCalculateRigidBodyForcesFromRB");
+}
+
+int fg_body_forces_from_bodies(const fg_float4* spheres, const fg_float4* bodyVelocities, const fg_float4* bodyCM,
+                               fg_float4* outForces, fg_float4* outTorques, unsigned int numSpheres,
+                               unsigned int numBodies, float radius, const fg_body_grid* bodies,
+                               const fg_float4* bodyBounds, const fg_body_grid* staticBodies, int accumulate) {
+    if (numSpheres == 0) return FG_OK;
+    if (!spheres || !bodyVelocities || !bodyCM || !outForces || !outTorques || (!bodies && !staticBodies)) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_body_forces_from_bodies: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    for (const fg_body_grid* g : {bodies, staticBodies})
+        if (g && g->numSpheres > 0 && (!g->orderedSpheres || !g->gridStart || !g->gridEnd || g->numBuckets == 0)) {
+            set_error(FG_ERR_INVALID_ARGUMENT, "fg_body_forces_from_bodies: body grid incomplete");
+            return FG_ERR_INVALID_ARGUMENT;
+        }
+    if (bodies && bodies->numSpheres > 0 && !bodies->bodyVelocities) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_body_forces_from_bodies: dynamic grid without body velocities");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    BodyContact dyn{}, st{};
+    if (bodies) {
+        dyn = body_contact(*bodies);
+        dyn.eps = 0.001f;
+        dyn.aabb = nullptr;
+    }
+    if (staticBodies) {
+        st = body_contact(*staticBodies);
+        st.eps = 0.001f;
+        st.vel = nullptr;
+        st.nvel = 0;
+    }
+    const float4* bb = bodies ? reinterpret_cast<const float4*>(bodyBounds) : nullptr;
+    if (accumulate)
+        hipLaunchKernelGGL(k_body_forces_from_bodies<true>, dim3(grid_for(numSpheres, 256)), dim3(256), 0,
+                           state().stream, reinterpret_cast<const float4*>(spheres),
+                           reinterpret_cast<const float4*>(bodyVelocities), reinterpret_cast<const float4*>(bodyCM),
+                           FG_F4(outForces), FG_F4(outTorques), numSpheres, numBodies, radius, dyn, st, bb,
+                           numBodies);
+    else
+        hipLaunchKernelGGL(k_body_forces_from_bodies<false>, dim3(grid_for(numSpheres, 256)), dim3(256), 0,
+                           state().stream, reinterpret_cast<const float4*>(spheres),
+                           reinterpret_cast<const float4*>(bodyVelocities), reinterpret_cast<const float4*>(bodyCM),
+                           FG_F4(outForces), FG_F4(outTorques), numSpheres, numBodies, radius, dyn, st, bb,
+                           numBodies);
+    return check_launch("fg_body_forces_from_bodies") ? FG_OK : fg_peek_error();
+}
+
+int fg_body_bounds(const fg_float4* spheres, const unsigned int* bodyOffsets, unsigned int numBodies, float margin,
+                   fg_float4* out2) {
+    if (numBodies == 0) return FG_OK;
+    if (!spheres || !bodyOffsets || !out2) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_body_bounds: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    hipLaunchKernelGGL(k_body_bounds, dim3(numBodies), dim3(256), 0, state().stream,
+                       reinterpret_cast<const float4*>(spheres), bodyOffsets, numBodies, margin, FG_F4(out2));
+    return check_launch("fg_body_bounds") ? FG_OK : fg_peek_error();
 }
 
 // K_RigidBodySim.cu:344-352

@@ -155,7 +155,9 @@ void RigidBodySimulator::buildStaticGrid() {
     fg_sort_pairs_ws(d_k, d_i, M, key_bits_for(mStBuckets), ws, wsBytes, 1);
     gFindCellOffsetOrderingPos(d_stStart, d_stEnd, d_k, d_i, reinterpret_cast<fg_float4*>(d_tmp),
                                reinterpret_cast<fg_float4*>(d_stSorted), M, mStBuckets);
-    const float margin = 2.0f * fp.pointRadius + mStaticRadius + 0.003f;
+    // the static box serves the fluid contact (fluid radius 2*pointRadius,
+    // margin 0.003) and the body contact (body sphere radius, margin 0.001)
+    const float margin = std::max(2.0f * fp.pointRadius, mSphereRadius) + mStaticRadius + 0.003f;
     fg_sphere_bounds(reinterpret_cast<fg_float4*>(d_src), M, margin * 1.01f, reinterpret_cast<fg_float4*>(d_stBounds));
     fg_set_stream(prev);
     fg_check("static body grid");
@@ -192,7 +194,7 @@ void RigidBodySimulator::release() {
     dfree(d_force); dfree(d_torque); dfree(d_inv); dfree(d_inertia); dfree(d_pf); dfree(d_pt); dfree(d_bounds);
     dfree(d_mass); dfree(d_sumF); dfree(d_sumT); dfree(d_massd); dfree(d_cmd); dfree(d_inertiad);
     dfree(d_offsets); dfree(d_key); dfree(d_id); dfree(d_start); dfree(d_end);
-    dfree(d_stSorted); dfree(d_stBounds); dfree(d_stStart); dfree(d_stEnd);
+    dfree(d_stSorted); dfree(d_stBounds); dfree(d_stStart); dfree(d_stEnd); dfree(d_bodyBounds);
     mStaticCount = 0;
     if (d_sortWs) (void)hipFree(d_sortWs);
     d_sortWs = nullptr;
@@ -244,6 +246,7 @@ void RigidBodySimulator::reset() {
     d_sumF = dalloc<double>(4 * B);
     d_sumT = dalloc<double>(4 * B);
     d_bounds = dalloc<float4>(2);
+    if (B >= 2) d_bodyBounds = dalloc<float4>(2 * B);
     mSortWsBytes = fg_sort_workspace_size((unsigned)M, key_bits_for(mBuckets));
     hip_check(hipMalloc(&d_sortWs, mSortWsBytes ? mSortWsBytes : 256), "hipMalloc");
 
@@ -311,7 +314,6 @@ void RigidBodySimulator::generateCollisionData() {
     const Application::ApplicationData* ad = Application::ApplicationData::instance();
     const Application::CommonRigidBodyParams& rp = const_cast<Application::ApplicationData*>(ad)->getCommonRigidBodyParams();
     const Application::CommonFluidParams& fp = const_cast<Application::ApplicationData*>(ad)->getCommonFluidParams();
-    (void)B;
     gCalcHashKeyWithOffset(d_key, d_id, reinterpret_cast<fg_float4*>(d_cm), reinterpret_cast<fg_float4*>(d_orient),
                            reinterpret_cast<fg_float4*>(d_rel), reinterpret_cast<fg_float4*>(d_world), M,
                            abi3(rp.getCellSize()), (int)mBuckets);
@@ -320,6 +322,11 @@ void RigidBodySimulator::generateCollisionData() {
                                reinterpret_cast<fg_float4*>(d_sorted), M, mBuckets);
     const float margin = 2.0f * fp.pointRadius + mSphereRadius + 0.003f;
     fg_sphere_bounds(reinterpret_cast<fg_float4*>(d_world), M, margin * 1.01f, reinterpret_cast<fg_float4*>(d_bounds));
+    // Broad phase of the body-body contact (RigidBodySimulator.cpp:334-406 on
+    // the host in the reference): per-body boxes grown by the contact distance.
+    if (B >= 2)
+        fg_body_bounds(reinterpret_cast<fg_float4*>(d_world), d_offsets, B, (2.0f * mSphereRadius + 0.001f) * 1.01f,
+                       reinterpret_cast<fg_float4*>(d_bodyBounds));
     fg_check("generateCollisionData");
 }
 
@@ -343,7 +350,23 @@ void RigidBodySimulator::calculateRigidBodyForcesFromFluid(float4* pos, float4* 
                               fg_int3{rp.rb_to_fluid_relation.x, rp.rb_to_fluid_relation.y,
                                       rp.rb_to_fluid_relation.z},
                               -sp.springForce, -sp.dampingForce);
+    calculateRigidBodyForcesFromRB();
     fg_check("calculateRigidBodyForcesFromFluid");
+}
+
+// RigidBodySimulator.cpp:481-501: body-body and body-static contact of every
+// dynamic sphere (stencil (1,1,1), margin 0.001), added to the fluid's part
+// of the per-sphere forces and torques.
+void RigidBodySimulator::calculateRigidBodyForcesFromRB() {
+    const unsigned M = (unsigned)mRel.size(), B = (unsigned)mMass.size();
+    if (M == 0 || (B < 2 && mStaticCount == 0)) return;
+    fg_body_grid dyn = bodyGrid(0.0f), st = staticGrid(0.0f);
+    dyn.relation = st.relation = fg_int3{1, 1, 1};
+    fg_body_forces_from_bodies(reinterpret_cast<fg_float4*>(d_world), reinterpret_cast<fg_float4*>(d_vel),
+                               reinterpret_cast<fg_float4*>(d_cm), reinterpret_cast<fg_float4*>(d_pf),
+                               reinterpret_cast<fg_float4*>(d_pt), M, B, mSphereRadius, B >= 2 ? &dyn : nullptr,
+                               B >= 2 ? reinterpret_cast<fg_float4*>(d_bodyBounds) : nullptr,
+                               mStaticCount ? &st : nullptr, 1);
 }
 
 // RigidBodySimulator.cpp:522-566: per-body sums (FP64 here).

@@ -103,6 +103,10 @@ SIGNATURES = {
     "fg_sph_forces_coupled": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32, vp, vp]),
     "fg_body_forces_from_fluid": (i32, [vp, vp, vp, vp, vp, u32, u32, f32, vp, vp, vp, vp, u32, f32, abi.Float3,
                                         u32, abi.Int3, f32, f32]),
+    "gCalculateRigidBodyForcesFromRB": (None, [vp, vp, vp, vp, vp, vp, vp, u32, abi.Float3, f32, i32, i32, vp, vp,
+                                               vp, f32, f32, i32]),
+    "fg_body_forces_from_bodies": (i32, [vp, vp, vp, vp, vp, u32, u32, f32, vp, vp, vp, i32]),
+    "fg_body_bounds": (i32, [vp, vp, u32, f32, vp]),
     "fg_body_reduce_f64": (i32, [vp, vp, vp, u32, vp, vp, vp, vp]),
     "fg_body_mass_properties_f64": (i32, [vp, vp, u32, vp, vp, vp]),
     "fg_sphere_bounds": (i32, [vp, u32, f32, vp]),

@@ -340,9 +340,10 @@ def make_bodies(sets, masses, h, common, spring=2000.0, damping=50.0, names=None
     rb.global_external_force.z, rb.global_external_force.w = 0.0, 0.0
     rb.simulation_scale = common.simulation_scale
     rb.deltaTimeIntegration = common.deltaTimeIntegration
-    # The body grid cell (getCellSize) spans the whole contact distance, so
-    # the 27-cell stencil finds every contact.
-    rb.particleRadius = float(np.float32(contact * 1.001))
+    # The body grid cell (getCellSize) spans the whole contact distance of
+    # both the fluid contact and the body-body contact (2 r + 0.001), so the
+    # 27-cell stencil finds every contact.
+    rb.particleRadius = float(np.float32(max(contact, 2.0 * sphere_r + 0.001) * 1.001))
     return BodySet(world, offsets, np.asarray(masses, np.float64), float(np.float32(sphere_r)), rb, spring, damping,
                    float(np.float32(fluid_r)), names or [], static)
 

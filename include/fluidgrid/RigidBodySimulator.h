@@ -8,6 +8,8 @@
 //   generateCollisionData()              body spheres to world space, keyed
 //                                        and sorted into the body grid
 //   calculateRigidBodyForcesFromFluid()  penalty force and torque per sphere
+//                                        from the fluid, then from the other
+//                                        bodies and the static spheres
 //   applyForces()                        per-body sums in FP64
 //   integrate()                          IntegrateRBK per body
 // Centre of mass and inertia tensor are computed in FP64 on the GPU from the
@@ -69,6 +71,10 @@ public:
     void calculateRigidBodyForcesFromFluid(float4* dfluidParticlePos, float4* dfluidParticleVel,
                                            unsigned int* dfluidHashStart, unsigned int* dfluidHashEnd,
                                            unsigned int numFluid);
+    // Body-body and body-static contact added to the per-sphere forces (run
+    // by calculateRigidBodyForcesFromFluid; a no-op with one body and no
+    // static spheres).
+    void calculateRigidBodyForcesFromRB();
     void applyForces();
     void integrate();
 
@@ -113,6 +119,7 @@ private:
     void* d_sortWs = nullptr;
     size_t mSortWsBytes = 0;
     unsigned int mBuckets = 0;
+    float4* d_bodyBounds = nullptr;  // [2B] broad phase of the body-body contact
     // static bodies
     float4 *d_stSorted = nullptr, *d_stBounds = nullptr;
     unsigned int *d_stStart = nullptr, *d_stEnd = nullptr;

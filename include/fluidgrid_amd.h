@@ -180,6 +180,20 @@ void gSPHCalculateForcesWithSRB(fg_float4* oldPositions, fg_float4* oldVelocity,
                                 unsigned int _in_rb_num_buckets,
                                 bool useStaticRB, bool useDynamicRB);
 
+/* K_RigidBodySim.cuh:5-25 -- per body sphere: penalty force from the spheres
+ * of the other dynamic bodies (same-body spheres skipped by w, body
+ * velocities indexed by w) and, when _in_numSRB > 0, from the static spheres;
+ * contact within 2*_in_Radius + 0.001, stencil (1,1,1).  _in_RBPositions is
+ * both the query set and the sorted grid array.  Writes F and the torque
+ * (x - cm[w]) x F, both with w = body id. */
+void gCalculateRigidBodyForcesFromRB(fg_float4* _in_RBPositions, fg_float4* _in_RBVelocites,
+                                     fg_float4* _in_RBCMPosition, fg_float4* _out_torque, fg_float4* _out_Forces,
+                                     unsigned int* _in_RBGridStart, unsigned int* _in_RBGridEnd,
+                                     unsigned int _in_NumParticles, fg_float3 _in_CellSize, float _in_Radius,
+                                     int _in_RBHashTableSize, int _in_SRBHashTableSize, fg_float4* _in_SRBPositions,
+                                     unsigned int* _in_SRBGridStart, unsigned int* _in_SRBGridEnd,
+                                     float _in_RBkSpring, float _in_RBkDamping, int _in_numSRB);
+
 /* K_RigidBodySim.cuh:27-50 -- per body sphere: penalty force from the fluid
  * grid and torque (x - cm[w]) x F; F.w = torque.w = body id. */
 void gCalculateRigidBodyForcesFromFluids(fg_float4* _in_RBPositions, fg_float4* _in_RBVelocites,
@@ -355,6 +369,27 @@ int fg_body_forces_from_fluid(fg_float4* spheres, fg_float4* bodyVelocities, fg_
                               unsigned int* fluidGridStart, unsigned int* fluidGridEnd, unsigned int numFluid,
                               float fluidRadius, fg_float3 fluidCellSize, unsigned int fluidNumBuckets,
                               fg_int3 relation, float kSpring, float kDamping);
+
+/* gCalculateRigidBodyForcesFromRB with explicit counts: spheres (body order,
+ * w = body id, any order works) against the dynamic grid `bodies` (the
+ * spheres of the own body skipped; may be NULL) and the static grid
+ * `staticBodies` (may be NULL, not both), contact within radius +
+ * grid.sphereRadius + 0.001 over grid.relation.  bodyBounds (optional,
+ * [2*numBodies], fg_body_bounds expanded by the contact distance) and the
+ * static grid's bounds skip walks that cannot find a contact.  accumulate !=
+ * 0 adds force and torque to outForces/outTorques (for instance the fluid's
+ * part from fg_body_forces_from_fluid), else writes them. */
+int fg_body_forces_from_bodies(const fg_float4* spheres, const fg_float4* bodyVelocities, const fg_float4* bodyCM,
+                               fg_float4* outForces, fg_float4* outTorques, unsigned int numSpheres,
+                               unsigned int numBodies, float radius, const fg_body_grid* bodies,
+                               const fg_float4* bodyBounds, const fg_body_grid* staticBodies, int accumulate);
+
+/* Per-body bounds of the sphere centres (spheres of body b occupy
+ * [bodyOffsets[b], bodyOffsets[b+1])), expanded by margin: out2[2b] min,
+ * out2[2b+1] max.  The broad phase of fg_body_forces_from_bodies; replaces
+ * the host bounding-box test of RigidBodySimulator.cpp:334-406. */
+int fg_body_bounds(const fg_float4* spheres, const unsigned int* bodyOffsets, unsigned int numBodies, float margin,
+                   fg_float4* out2);
 
 /* Per-body force and torque sums in FP64 (replaces the CUDPP segmented scan
  * of RigidBodySimulator.cpp:551-553): spheres of body b occupy

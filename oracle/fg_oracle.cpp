@@ -733,6 +733,44 @@ void orc_body_forces_from_fluid(const fg_float4* rb_pos, const fg_float4* rb_vel
     }
 }
 
+/* K_RigidBodySim.cu:18-61: per body sphere, penalty force from the spheres of
+ * the other bodies in the dynamic grid (same-body spheres skipped by w,
+ * velocities by w) and from the static grid (no velocities), margin 0.001,
+ * stencil (1,1,1); torque (x - cm[w]) x F.  dyn_pos or st may be null.  With
+ * accumulate the force and torque are added to out_f/out_t (the fluid's
+ * part), in float, else written; w = (float)(int)w. */
+void orc_body_forces_from_bodies(const fg_float4* rb_pos, const fg_float4* rb_vel, const fg_float4* rb_cm,
+                                 fg_float4* out_f, fg_float4* out_t, uint32_t m, float radius,
+                                 const fg_float4* dyn_pos, const uint32_t* dyn_start, const uint32_t* dyn_end,
+                                 const OGrid* dyn_grid, float dyn_radius, const OStatic* st, float ks, float kd,
+                                 int accumulate) {
+    const fg_int3 rel{1, 1, 1};
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < (int64_t)m; ii++) {
+        const uint32_t i = (uint32_t)ii;
+        const fg_float4 pos = rb_pos[i];
+        const int b = f2i_sat(pos.w);
+        const fg_float4 vel = rb_vel[b], cm = rb_cm[b];
+        V3<float> F = v3<float>(0, 0, 0);
+        if (dyn_pos)
+            F += process_collision<float>(pos, vel, radius, dyn_pos, rb_vel, dyn_radius, ks, kd, rel, dyn_start,
+                                          dyn_end, *dyn_grid, true, true, b, 0.001f);
+        if (st)
+            F += process_collision<float>(pos, vel, radius, st->pos, nullptr, st->radius, ks, kd, rel, st->start,
+                                          st->end, st->grid, false, false, 0, 0.001f);
+        const fg_float4 dir{pos.x - cm.x, pos.y - cm.y, pos.z - cm.z, pos.w - cm.w};
+        const fg_float4 t{(dir.y * F.z) - (dir.z * F.y), (dir.z * F.x) - (dir.x * F.z), (dir.x * F.y) - (dir.y * F.x),
+                          (float)b};
+        if (accumulate) {
+            out_f[i] = fg_float4{out_f[i].x + F.x, out_f[i].y + F.y, out_f[i].z + F.z, (float)b};
+            out_t[i] = fg_float4{out_t[i].x + t.x, out_t[i].y + t.y, out_t[i].z + t.z, (float)b};
+        } else {
+            out_f[i] = fg_float4{F.x, F.y, F.z, (float)b};
+            out_t[i] = t;
+        }
+    }
+}
+
 /* Per-body sums in FP64 of per-sphere forces and torques (the reduction of
  * K_RigidBodySim.cu:102-155 and RigidBodySimulator.cpp:522-566, whose CUDPP
  * scan sums in FP32); body index from F.w, sums in sphere order. */

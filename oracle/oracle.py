@@ -233,6 +233,9 @@ def _rigid_sigs(L):
                                          C.c_void_p]
     L.orc_body_forces_from_fluid.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, C.c_uint32, C.c_float, _f32p, _f32p,
                                              _u32p, _u32p, C.c_float, og, abi.Int3, C.c_float, C.c_float]
+    L.orc_body_forces_from_bodies.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, C.c_uint32, C.c_float,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                              C.c_void_p, C.c_float, C.c_float, C.c_int]
     L.orc_body_reduce_f64.argtypes = [_f32p, _f32p, C.c_uint32, C.c_uint32, _f64p, _f64p]
     L.orc_integrate_rb.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_uint32, C.c_float, abi.Float4, abi.Float3,
                                    abi.Float3, _f32p, _f32p, _f32p, _f32p, C.c_float]
@@ -325,6 +328,36 @@ def body_forces_from_fluid(rb_pos, rb_vel, rb_cm, radius, spos, svel, start, end
     return f, t
 
 
+def body_forces_from_bodies(rb_pos, rb_vel, rb_cm, radius, ks, kd, dyn=None, static=None, f_init=None,
+                            t_init=None):
+    """Body-body and body-static contact per sphere (K_RigidBodySim.cu:18-61).
+    dyn = (sorted sphere centres, start, end, OGrid, sphere radius) of the
+    dynamic grid, static likewise for the static grid; either may be None.
+    With f_init/t_init the results are added to them (float), else written."""
+    rb_pos = _c(rb_pos, np.float32)
+    m = rb_pos.shape[0]
+    acc = f_init is not None
+    f = _c(f_init, np.float32).copy() if acc else np.zeros((m, 4), np.float32)
+    t = _c(t_init, np.float32).copy() if acc else np.zeros((m, 4), np.float32)
+    keep = []
+    dargs = [None, None, None, None, 0.0]
+    if dyn is not None:
+        dp, ds, de, dg, dr = dyn
+        arrs = (_c(dp, np.float32), _c(ds, np.uint32), _c(de, np.uint32))
+        keep += [arrs, dg]
+        dargs = [arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data, C.addressof(dg), dr]
+    st_arg = None
+    if static is not None:
+        sp_, ss_, se_, sg_, sr_ = static
+        arrs = (_c(sp_, np.float32), _c(ss_, np.uint32), _c(se_, np.uint32))
+        ost = OStatic(arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data, sg_, sr_)
+        keep += [arrs, ost]
+        st_arg = C.addressof(ost)
+    _rl().orc_body_forces_from_bodies(rb_pos, _c(rb_vel, np.float32), _c(rb_cm, np.float32), f, t, m, radius,
+                                      *dargs, st_arg, ks, kd, int(acc))
+    return f, t
+
+
 def body_reduce_f64(f, t, nbodies):
     f, t = _c(f, np.float32), _c(t, np.float32)
     of = np.zeros((nbodies, 3), np.float64)
@@ -392,9 +425,9 @@ def coupled_step(st):
     """One FP32 coupled step in the stage order of the HIP simulator
     (HeterogeneousSimulator.cpp:54-127 with the coupled branch of :174-186
     and RigidBodySimulator.cpp:451-611): fluid grid and density; body world
-    transform, grid and forces from the fluid; coupled fluid forces (static
-    grid, then dynamic grid) and integration; per-body FP64 sums and body
-    integration.  With no dynamic bodies only the fluid advances, against
+    transform, grid and forces from the fluid, then from the other bodies and
+    the static spheres; coupled fluid forces (static grid, then dynamic grid)
+    and integration; per-body FP64 sums and body integration.  With no dynamic bodies only the fluid advances, against
     the static grid."""
     s, b = st.scene, st.bodies
     common, fluid, kern, mode = s.common, s.fluid, s.kernels, s.key_mode
@@ -420,6 +453,10 @@ def coupled_step(st):
     bstart, bend, bpos, _ = cell_offsets(bskey, bspid, world, None, nbb)
     f, t = body_forces_from_fluid(world, st.bvel, st.cm, b.sphere_radius, spos, svel, start, end, fluid_r,
                                   grid_of_fluid(common, mode), tuple(b.rb.rb_to_fluid_relation), ks, kd)
+    if b.b >= 2 or st.static is not None:
+        dyn = (bpos, bstart, bend, bgrid, b.sphere_radius) if b.b >= 2 else None
+        f, t = body_forces_from_bodies(world, st.bvel, st.cm, b.sphere_radius, ks, kd, dyn=dyn, static=st.static,
+                                       f_init=f, t_init=t)
     v_new, _ = forces_coupled(spos, svel, rho_p, start, end, spid, fluid, common, kern, mode, fluid_r,
                               tuple(common.fluid_to_srb_relation), b.sphere_radius, ks, kd, bpos, st.bvel, bstart,
                               bend, bgrid, v_out_init=st.vel, static=st.static)

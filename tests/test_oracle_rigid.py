@@ -14,7 +14,9 @@
         rotation by q^-1 (numpy)
   both coupling directions                       O(N*M) FP64 brute force over
         all fluid/sphere pairs (margins 0.003 fluid side, 0.0001 body side),
-        and equal-and-opposite pair forces on the common pairs
+        each fluid-side pair counted as often as the reference's hashed
+        stencil visits the sphere's bucket, and equal-and-opposite pair
+        forces on the common pairs
   coupled step                                   dt^2 free fall of a body
         above the fluid; upward contact force once it touches
   mass properties (FP64)                         numpy FP64 sums; thin-shell
@@ -256,9 +258,26 @@ def test_contact_forces_brute_force_and_antisymmetry(orc):
         dv = bvel[0, :3].astype(np.float64) - svel[i, :3]
         return i, j, (ks * np.linalg.norm(r, axis=1) - kd * np.einsum("ij,ij->i", n, dv))[:, None] * n
 
+    # The fluid side walks the body grid with the reference stencil: the
+    # buckets of hash(p + k*cs), k in {-1,0,1}^3 (K_DeviceFunctions.cu:128-150).
+    # FP32 rounding of p + k*cs + 5000 (ulp 2^-11 there) can land two offsets
+    # in one cell and skip its neighbour, so a pair counts as many times as the
+    # stencil visits the sphere's bucket (zero for about 1% of the pairs here).
+    c32 = np.float32(cell)
+    visits = []
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                q = spos.copy()
+                for ax, k in ((0, dx), (1, dy), (2, dz)):
+                    q[:, ax] = spos[:, ax] + np.float32(c32 * np.float32(k))
+                visits.append(orc.hash_keys(q, nbb, (cell,) * 3, s.common, abi.KEY_REF_XOR)[0])
+    visits = np.stack(visits, 1)
     i, j, ff = pair_forces(0.003)
+    mult = (visits[i] == key[j][:, None]).sum(1)
+    assert np.count_nonzero(mult == 0) < 0.05 * len(i)
     want = np.zeros_like(X)
-    np.add.at(want, i, ff)
+    np.add.at(want, i, ff * mult[:, None])
     assert len(i) > 1000
     assert np.allclose(ext, want, rtol=1e-5, atol=1e-5 * np.abs(ff).max())
     i, j, ff = pair_forces(0.0001)
