@@ -70,12 +70,15 @@ def measure(scene, steps, warmup, fused=True):
     try:
         if bodies is not None:
             sim.add_bodies(bodies)
+        c_start = sim.conservation()
         sim.step(warmup)
         sim.synchronize()
         t0 = time.perf_counter()
         sim.step(steps)
         sim.synchronize()
         dt = time.perf_counter() - t0
+        c_end = sim.conservation()
+        drift = drift_report(scene, c_start, c_end, warmup + steps)
         c_occ = occupied_cells(sim)
         # per-stage durations: eager steps with events between stages
         sim.set_options(fused=fused, use_graph=False, stage_timing=True)
@@ -95,7 +98,26 @@ def measure(scene, steps, warmup, fused=True):
             body = {"centre_y_m": [round(float(v), 6) for v in bs["pos"][:, 1]]}
     finally:
         sim.close()
-    return dict(seconds=dt, ms_per_step=1e3 * dt / steps, c_occ=c_occ, stage_ms=stage_ms, finite=finite, body=body)
+    return dict(seconds=dt, ms_per_step=1e3 * dt / steps, c_occ=c_occ, stage_ms=stage_ms, finite=finite, body=body,
+                drift=drift)
+
+
+def drift_report(scene, c0, c1, steps):
+    """Conservation over the warm-up and timed steps, from the FP64 GPU sums:
+    mass (exact), the momentum change beside the gravity impulse N m g t
+    (walls, bodies and the clamp account for the rest), and the energies
+    (a dissipation report, not a conservation claim)."""
+    t = steps * float(np.float32(scene.common.deltaTimeIntegration))
+    g = np.array(tuple(scene.common.global_external_force), np.float64)
+    e0 = c0["kinetic"] + c0["potential"] + c0["internal"]
+    e1 = c1["kinetic"] + c1["potential"] + c1["internal"]
+    return {"steps": steps, "active_start": int(c0["count"]), "active_end": int(c1["count"]),
+            "mass_change": float(c1["mass"] - c0["mass"]),
+            "momentum_change": [float(v) for v in c1["momentum"] - c0["momentum"]],
+            "gravity_impulse": [float(v) for v in c0["mass"] * g * t],
+            "energy_start": {"kinetic": c0["kinetic"], "potential": c0["potential"], "internal": c0["internal"]},
+            "energy_end": {"kinetic": c1["kinetic"], "potential": c1["potential"], "internal": c1["internal"]},
+            "energy_change_rel": float((e1 - e0) / abs(e0)) if e0 else None}
 
 
 def measure_batched(ens, steps, warmup, tune_steps=5):
@@ -277,6 +299,7 @@ def main():
             "step_hbm_frac": step_gbs / HBM_PEAK,
             "finite": r["finite"],
             "bodies": r["body"],
+            "drift": r["drift"],
         },
         "roofline": {
             "bound": "hbm",
@@ -291,16 +314,21 @@ def main():
         },
     }
     if not a.no_8m and a.config != 2:
-        s8 = scenes.config_scene(2)
-        log(f"scene {s8.name}: n={s8.n:,}")
         k8 = max(20, a.steps // 4)
-        r8 = measure(s8, k8, max(5, a.warmup // 2), fused)
-        out["config"]["also_8M"] = {
-            "workload": f"plain FP32 dam break, {s8.n:,} particles, fluid only",
-            "particles": s8.n, "steps": k8, "value": s8.n * k8 / r8["seconds"], "ms_per_step": r8["ms_per_step"],
-            "step_hbm_frac": step_bytes(s8, r8["c_occ"]) / (r8["ms_per_step"] * 1e-3) / HBM_PEAK,
-            "stage_ms": {k: round(v, 5) for k, v in r8["stage_ms"].items()},
-        }
+        for key, s8 in (("also_8M", scenes.coupled_config_scene(2)), ("also_8M_fluid_only", scenes.config_scene(2))):
+            log(f"scene {s8.name}: n={s8.n:,}")
+            r8 = measure(s8, k8, max(5, a.warmup // 2), fused)
+            b8 = s8.info.get("bodies")
+            out["config"][key] = {
+                "workload": f"plain FP32 dam break, {s8.n:,} particles, "
+                            + (f"coupled two-way with {b8.b} rigid boxes ({b8.m:,} boundary spheres)"
+                               if b8 is not None else "fluid only"),
+                "particles": s8.n, "steps": k8, "value": s8.n * k8 / r8["seconds"],
+                "ms_per_step": r8["ms_per_step"],
+                "step_hbm_frac": step_bytes(s8, r8["c_occ"]) / (r8["ms_per_step"] * 1e-3) / HBM_PEAK,
+                "stage_ms": {k: round(v, 5) for k, v in r8["stage_ms"].items()},
+                "finite": r8["finite"], "drift": r8["drift"],
+            }
     if not a.no_batch:
         ens = scenes.ensemble(num_scenes=a.batch_scenes)
         log(f"ensemble: {len(ens.scenes)} scenes, n={ens.n:,}")

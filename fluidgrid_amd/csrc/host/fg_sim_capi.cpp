@@ -15,6 +15,12 @@ struct SimState {
     std::vector<std::unique_ptr<Emitter::ArrayEmitter>> emitters;
     std::vector<std::unique_ptr<Fluid::GPUSPHFluid>> fluids;
     std::vector<std::unique_ptr<RigidBody::RigidBody>> bodies;
+    void* diagWs = nullptr;     // fg_sim_conservation workspace
+    double* diagOut = nullptr;  // [8]
+    ~SimState() {
+        if (diagWs) (void)hipFree(diagWs);
+        if (diagOut) (void)hipFree(diagOut);
+    }
 };
 
 SimState*& sim() {
@@ -130,6 +136,36 @@ int fg_sim_download(int idx, fg_float4* positions, fg_float4* velocities) {
     if (!f) return FG_ERR_INVALID_ARGUMENT;
     try {
         f->downloadState(reinterpret_cast<float4*>(positions), reinterpret_cast<float4*>(velocities));
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_conservation(int idx, double* out8) {
+    Fluid::GPUSPHFluid* f = fluid_at(idx);
+    if (!f || !out8) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        SimState* st = sim();
+        const size_t bytes = fg_conservation_workspace_size();
+        if (!st->diagWs && hipMalloc(&st->diagWs, bytes) != hipSuccess)
+            throw std::runtime_error("fg_sim_conservation: hipMalloc failed");
+        if (!st->diagOut && hipMalloc(reinterpret_cast<void**>(&st->diagOut), 8 * sizeof(double)) != hipSuccess)
+            throw std::runtime_error("fg_sim_conservation: hipMalloc failed");
+        hipStream_t stream = Simulators::HeterogeneousSimulator::instance()->stream();
+        void* prev = fg_get_stream();
+        fg_set_stream(stream);
+        const int rc = fg_conservation_f64(reinterpret_cast<fg_float4*>(f->getPosition()),
+                                           reinterpret_cast<fg_float4*>(f->getVelocity()),
+                                           reinterpret_cast<fg_float4*>(f->getOrderedPosition()),
+                                           reinterpret_cast<fg_float2*>(f->getDensityField()),
+                                           (unsigned)f->getNumParticles(), f->getMass(), f->getRestDensity(),
+                                           f->getGasConstant(), st->diagOut, st->diagWs, bytes);
+        fg_set_stream(prev);
+        if (rc != FG_OK) return rc;
+        if (hipStreamSynchronize(stream) != hipSuccess ||
+            hipMemcpy(out8, st->diagOut, 8 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            throw std::runtime_error("fg_sim_conservation: copy failed");
         return FG_OK;
     } catch (const std::exception& e) {
         return fail(e);

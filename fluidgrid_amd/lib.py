@@ -83,6 +83,9 @@ SIGNATURES = {
     "fg_sim_upload": (i32, [i32, vp, vp]),
     "fg_sim_fluid_buffers": (i32, [i32, vp]),
     "fg_sim_stage_times": (i32, [vp]),
+    "fg_sim_conservation": (i32, [i32, vp]),
+    "fg_conservation_workspace_size": (sz, []),
+    "fg_conservation_f64": (i32, [vp, vp, vp, vp, u32, f32, f32, f32, vp, vp, sz]),
     "fg_sim_shutdown": (i32, []),
     "fg_sim_add_bodies": (i32, [vp, vp, u32, vp, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
     "fg_sim_add_static_spheres": (i32, [vp, u32, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
@@ -260,6 +263,14 @@ class Simulator:
 
     def set_options(self, fused=True, use_graph=True, stage_timing=False):
         check(load().fg_sim_set_options(int(fused), int(use_graph), int(stage_timing)), "options")
+
+    def conservation(self):
+        """FP64 sums over active particles: dict of count, mass, momentum (3),
+        kinetic, potential (gravity) and internal (EOS) energy."""
+        out = np.zeros(8, np.float64)
+        check(load().fg_sim_conservation(self.idx, out.ctypes.data), "fg_sim_conservation")
+        return {"count": out[0], "mass": out[1], "momentum": out[2:5].copy(), "kinetic": out[5],
+                "potential": out[6], "internal": out[7]}
 
     def set_overlap(self, on=True):
         check(load().fg_sim_set_overlap(int(on)), "fg_sim_set_overlap")

@@ -408,6 +408,23 @@ int fg_body_mass_properties_f64(const fg_float4* worldSpheres, const unsigned in
 /* Bounds of all sphere centres expanded by margin: out2[0] min, out2[1] max. */
 int fg_sphere_bounds(const fg_float4* spheres, unsigned int numSpheres, float margin, fg_float4* out2);
 
+/* FP64 global sums of one fluid for conservation reports (no counterpart in
+ * the reference).  out8 (device memory) receives, over active particles
+ * (w > 0):
+ *   [0] count, [1] mass, [2..4] momentum m v, [5] kinetic energy m|v|^2/2,
+ *   [6] gravitational potential energy -m g.(x s), with g and the
+ *       simulation scale s of the current common parameters,
+ *   [7] internal energy of the equation of state p = k (rho - rho0),
+ *       m k (ln(rho/rho0) + rho0/rho - 1), from the sorted densities and the
+ *       active flags of the sorted positions.
+ * positions/velocities in particle order; the two partial passes use a
+ * caller workspace of fg_conservation_workspace_size() bytes.  The result
+ * does not depend on timing (fixed reduction trees). */
+size_t fg_conservation_workspace_size(void);
+int fg_conservation_f64(const fg_float4* positions, const fg_float4* velocities, const fg_float4* orderedPositions,
+                        const fg_float2* density_pressure, unsigned int numParticles, float mass, float restDensity,
+                        float gasConstant, double* out8, void* workspace, size_t workspace_bytes);
+
 /* ------------------------------------------------------------------------ */
 /* 3. Batched scenes (no counterpart in the reference)                       */
 /* ------------------------------------------------------------------------ */

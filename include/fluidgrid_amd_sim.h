@@ -50,6 +50,11 @@ int fg_sim_fluid_buffers(int fluid, fg_fluid_buffers* out);
  * run with stage_timing on (7 floats). */
 int fg_sim_stage_times(float* out7);
 
+/* FP64 conservation sums of fluid idx (fg_conservation_f64) into host out8:
+ * count, mass, momentum x/y/z, kinetic, gravitational potential and EOS
+ * internal energy.  Synchronises the simulator's stream. */
+int fg_sim_conservation(int idx, double* out8);
+
 /* Dynamic rigid bodies coupled to the (single) fluid: sphere centres in
  * world space, body b owning [bodyOffsets[b], bodyOffsets[b+1]); masses in
  * kg; the coupling parameters (ApplicationData::setCommonRigidBodyParams /

@@ -159,6 +159,26 @@ def step(pos, vel, fluid, common, kernels, mode, rho_p_prev=None, end_init=None,
                 rho_p=rho_p, vel=v_new, pos=p_new, acc=acc, mag=mag)
 
 
+def conservation(pos, vel, spos, rho_p, mass, gravity, scale, rest_density, gas_constant):
+    """FP64 global sums of one fluid (fg_conservation_f64): over particles with
+    w > 0, count, mass, momentum m v, kinetic energy m |v|^2 / 2, potential
+    energy -m g.(x s); and over the sorted particles with w > 0 the internal
+    energy of p = k (rho - rho0), whose specific energy is the integral of
+    p / rho^2 from rho0: k (ln(rho/rho0) + rho0/rho - 1)."""
+    pos = np.asarray(pos, np.float32)
+    vel = np.asarray(vel, np.float32)
+    act = pos[:, 3] > 0
+    v = vel[act, :3].astype(np.float64)
+    x = pos[act, :3].astype(np.float64)
+    m = float(mass)
+    g = np.asarray(gravity, np.float64)
+    sact = np.asarray(spos, np.float32)[:, 3] > 0
+    r = np.asarray(rho_p, np.float32)[sact, 0].astype(np.float64)
+    r0, k = float(rest_density), float(gas_constant)
+    return np.array([act.sum(), m * act.sum(), *(m * v.sum(0)), 0.5 * m * (v * v).sum(),
+                     -m * float(scale) * (x @ g).sum(), (m * k * (np.log(r / r0) + r0 / r - 1.0)).sum()])
+
+
 def set_num_threads(t):
     lib().orc_set_num_threads(int(t))
 
