@@ -99,6 +99,13 @@ class CommonRigidBodyParams(C.Structure):
     ]
 
 
+class InflowDesc(C.Structure):
+    """fg_inflow_desc (include/fluidgrid_amd.h)"""
+    _fields_ = [("origin", Float3), ("axisU", Float3), ("axisV", Float3), ("normal", Float3), ("spacing", C.c_float),
+                ("speed", C.c_float), ("nu", C.c_uint), ("nv", C.c_uint), ("firstSlot", C.c_uint),
+                ("numLayers", C.c_uint)]
+
+
 class BodyGrid(C.Structure):
     """fg_body_grid (include/fluidgrid_amd.h)"""
     _fields_ = [("orderedSpheres", C.c_void_p), ("bodyVelocities", C.c_void_p), ("gridStart", C.c_void_p),

@@ -197,6 +197,7 @@ static_assert(sizeof(fg_common_fluid_params) == 104, "CommonFluidParams layout")
 static_assert(sizeof(fg_common_rigid_body_params) == 112, "CommonRigidBodyParams layout");
 static_assert(sizeof(fg_kernels_cache) == 12, "KernelsCache layout");
 static_assert(sizeof(fg_fluid_params) == 28, "FluidParams layout");
+static_assert(sizeof(fg_inflow_desc) == 72, "fg_inflow_desc layout");
 static_assert(sizeof(fg_scene_desc) == 160, "scene descriptor layout");
 
 #define FG_F4(p) reinterpret_cast<float4*>(p)

@@ -81,6 +81,7 @@ GPUSPHFluid::~GPUSPHFluid() {
     (void)hipFree(d_ordered_vel);
     (void)hipFree(d_sortWorkspace);
     (void)hipFree(d_neighborMasks);
+    (void)hipFree(d_emitStep);
 }
 
 int GPUSPHFluid::copyPositionToCPU(float4** particlePositions) {
@@ -104,6 +105,31 @@ void GPUSPHFluid::downloadState(float4* pos, float4* vel) {
     hip_check(hipDeviceSynchronize(), "downloadState sync");
     if (pos) hip_check(hipMemcpy(pos, d_position, sizeof(float4) * n, hipMemcpyDeviceToHost), "downloadState");
     if (vel) hip_check(hipMemcpy(vel, d_velocity, sizeof(float4) * n, hipMemcpyDeviceToHost), "downloadState");
+}
+
+}  // namespace Fluid
+}  // namespace Simulation
+
+namespace Simulation {
+namespace Fluid {
+
+void GPUSPHFluid::setInflow(const fg_inflow_desc& desc) {
+    const uint64_t reserve = (uint64_t)desc.nu * desc.nv * desc.numLayers;
+    if ((uint64_t)desc.firstSlot + reserve > (uint64_t)getNumParticles())
+        throw std::invalid_argument("GPUSPHFluid::setInflow: reserve slots beyond max_particles");
+    if (desc.nu == 0 || desc.nv == 0 || !(desc.spacing > 0.0f) || !(desc.speed > 0.0f))
+        throw std::invalid_argument("GPUSPHFluid::setInflow: empty window or non-positive spacing/speed");
+    mInflow = desc;
+    if (!d_emitStep && hipMalloc(reinterpret_cast<void**>(&d_emitStep), sizeof(unsigned int)) != hipSuccess)
+        throw std::runtime_error("GPUSPHFluid::setInflow: hipMalloc failed");
+    if (hipMemset(d_emitStep, 0, sizeof(unsigned int)) != hipSuccess)
+        throw std::runtime_error("GPUSPHFluid::setInflow: hipMemset failed");
+}
+
+void GPUSPHFluid::emitInflow(float dt) {
+    if (d_emitStep)
+        fg_emit_inflow(reinterpret_cast<fg_float4*>(d_position), reinterpret_cast<fg_float4*>(d_velocity), d_emitStep,
+                       &mInflow, dt);
 }
 
 }  // namespace Fluid

@@ -153,6 +153,13 @@ void HeterogeneousSimulator::calculateFluidDensity(Fluid::GPUSPHFluid* fluid) {
                      fluid->getRestDensity(), fluid->getMass());
 }
 
+// Inflow emission before the neighbourhood search (the reference never calls
+// IEmitter::update; its only emitter's update is empty, Emitter.h:46).
+void HeterogeneousSimulator::emit(Fluid::GPUSPHFluid* f) {
+    if (f->hasInflow())
+        f->emitInflow(Application::ApplicationData::instance()->getCommonFluidParams().deltaTimeIntegration);
+}
+
 bool HeterogeneousSimulator::coupled() const {
     RigidBody::RigidBodySimulator* rb = RigidBody::RigidBodySimulator::instance();
     return rb->getNumDynamicBodies() > 0;
@@ -224,6 +231,7 @@ void HeterogeneousSimulator::enqueueCoupledOverlapped(Fluid::GPUSPHFluid* f) {
     enum { kFork, kBodyGrid, kCells, kForces, kJoin };
     rb->startInteraction();
     f->beginUpdate();
+    emit(f);
     hip_check(hipEventRecord(mSync[kFork], mStream), "event");
     hip_check(hipStreamWaitEvent(mBodyStream, mSync[kFork], 0), "wait");
     fg_set_stream(mBodyStream);
@@ -272,6 +280,7 @@ void HeterogeneousSimulator::enqueueStep() {
     if (withBodies) rb->startInteraction();
     for (Fluid::GPUSPHFluid* f : mFluids) {
         f->beginUpdate();
+        emit(f);
         if (t) (void)hipEventRecord(mEv[0], mStream);
         findFluidNeighbourhood(f);
         if (t) (void)hipEventRecord(mEv[3], mStream);

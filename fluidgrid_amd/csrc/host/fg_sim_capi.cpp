@@ -142,6 +142,22 @@ int fg_sim_download(int idx, fg_float4* positions, fg_float4* velocities) {
     }
 }
 
+int fg_sim_set_inflow(int idx, const fg_inflow_desc* desc) {
+    Fluid::GPUSPHFluid* f = fluid_at(idx);
+    if (!f || !desc) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        (void)hipStreamSynchronize(Simulators::HeterogeneousSimulator::instance()->stream());
+        f->setInflow(*desc);
+        Simulators::HeterogeneousSimulator::instance()->invalidateGraph();
+        return FG_OK;
+    } catch (const std::invalid_argument& e) {
+        fg_record_error(FG_ERR_INVALID_ARGUMENT, e.what());
+        return FG_ERR_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 int fg_sim_conservation(int idx, double* out8) {
     Fluid::GPUSPHFluid* f = fluid_at(idx);
     if (!f || !out8) return FG_ERR_INVALID_ARGUMENT;

@@ -84,6 +84,8 @@ SIGNATURES = {
     "fg_sim_fluid_buffers": (i32, [i32, vp]),
     "fg_sim_stage_times": (i32, [vp]),
     "fg_sim_conservation": (i32, [i32, vp]),
+    "fg_sim_set_inflow": (i32, [i32, C.POINTER(abi.InflowDesc)]),
+    "fg_emit_inflow": (i32, [vp, vp, vp, C.POINTER(abi.InflowDesc), f32]),
     "fg_conservation_workspace_size": (sz, []),
     "fg_conservation_f64": (i32, [vp, vp, vp, vp, u32, f32, f32, f32, vp, vp, sz]),
     "fg_sim_shutdown": (i32, []),
@@ -263,6 +265,10 @@ class Simulator:
 
     def set_options(self, fused=True, use_graph=True, stage_timing=False):
         check(load().fg_sim_set_options(int(fused), int(use_graph), int(stage_timing)), "options")
+
+    def set_inflow(self, desc):
+        """Attach an abi.InflowDesc: reserve slots are emitted from the next step."""
+        check(load().fg_sim_set_inflow(self.idx, C.byref(desc)), "fg_sim_set_inflow")
 
     def conservation(self):
         """FP64 sums over active particles: dict of count, mass, momentum (3),

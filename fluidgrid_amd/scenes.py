@@ -143,6 +143,47 @@ def config_scene(cfg_id, key_mode=abi.KEY_LINEAR):
                      name=f"dam_break_cfg{cfg_id}", key_mode=key_mode)
 
 
+def with_inflow(scene, window_min, window_max, speed=0.5, layers=4, spacing=None):
+    """Adds an inflow through the x-min face (normal +x) of the box: a lattice
+    window [y0, y1] x [z0, z1] at x = volume_min.x + spacing, emitted layer by
+    layer at `speed` (fg_inflow_desc).  The reserve (layers x nu x nv slots)
+    is appended to the particle arrays as inactive particles (w = -1, zero
+    velocity) parked one per cell in the top cell layers of the box, where no
+    active particle reaches them; inactive particles still take part in the
+    grid, as in the reference.  Returns the scene (arrays extended) with the
+    descriptor in scene.info["inflow"]."""
+    cp = scene.common
+    h = cp.kernel_radius
+    s = float(np.float32(spacing if spacing is not None else 0.5 * h))
+    (y0, z0), (y1, z1) = window_min, window_max
+    nu = int(math.floor((y1 - y0) / s + 1e-9)) + 1
+    nv = int(math.floor((z1 - z0) / s + 1e-9)) + 1
+    reserve = layers * nu * nv
+    cs = cp.cell_size.x
+    bx, by, bz = cp.bucket_size.x, cp.bucket_size.y, cp.bucket_size.z
+    per_layer = bx * bz
+    if reserve > per_layer * by // 4:
+        raise ValueError("reserve does not fit in the top quarter of the box")
+    k = np.arange(reserve)
+    park = np.zeros((reserve, 4), np.float32)
+    park[:, 0] = cp.volume_min.x + (k % bx + 0.5) * cs
+    park[:, 2] = cp.volume_min.z + ((k // bx) % bz + 0.5) * cs
+    park[:, 1] = cp.volume_max.y - (k // per_layer + 0.5) * cs
+    park[:, 3] = -1.0
+    first = scene.n
+    scene.pos = np.concatenate([scene.pos, park])
+    scene.vel = np.concatenate([scene.vel, np.zeros_like(park)])
+    scene.fluid.max_particles = scene.n
+    d = abi.InflowDesc()
+    d.origin = abi.f3(cp.volume_min.x + s, y0, z0)
+    d.axisU, d.axisV, d.normal = abi.f3(0, 1, 0), abi.f3(0, 0, 1), abi.f3(1, 0, 0)
+    d.spacing, d.speed = s, float(np.float32(speed))
+    d.nu, d.nv, d.firstSlot, d.numLayers = nu, nv, first, layers
+    scene.info["inflow"] = d
+    scene.info["active"] = first
+    return scene
+
+
 def small_dam_break(lattice=(12, 8, 10), h=0.05, box=(1.0, 1.0, 1.0), dt=4e-4, seed=SEED_BASE + 100,
                     key_mode=abi.KEY_LINEAR):
     return dam_break(lattice, h, box, dt, 10, seed=seed, name="small_dam_break", key_mode=key_mode)

@@ -90,6 +90,12 @@ public:
     // Extensions: state transfer and the persistent sort workspace.
     void uploadState(const float4* pos, const float4* vel);
     void downloadState(float4* pos, float4* vel);
+    // Inflow emission (IEmitter::update, a no-op in the reference): the
+    // particle array's reserve slots are emitted layer by layer at the start
+    // of each step (fg_emit_inflow), from a step counter kept on the device.
+    void setInflow(const fg_inflow_desc& desc);
+    bool hasInflow() const { return d_emitStep != nullptr; }
+    void emitInflow(float dt);
     void* getSortWorkspace() { return d_sortWorkspace; }
     unsigned long long* getNeighborMasks() { return d_neighborMasks; }
     size_t getSortWorkspaceBytes() const { return mSortWorkspaceBytes; }
@@ -112,6 +118,8 @@ private:
     void* d_sortWorkspace = nullptr;
     unsigned long long* d_neighborMasks = nullptr;
     size_t mSortWorkspaceBytes = 0;
+    fg_inflow_desc mInflow{};
+    unsigned int* d_emitStep = nullptr;
 };
 
 }  // namespace Fluid

@@ -72,6 +72,7 @@ private:
     hipEvent_t mEv[9] = {};
     hipStream_t mBodyStream = nullptr;  // body work of a coupled step
     hipEvent_t mSync[5] = {};           // fork/join of the two streams
+    void emit(Fluid::GPUSPHFluid* fluid);
     bool coupled() const;        // dynamic bodies present
     bool hasStaticBodies() const;
     StageTimes mTimes;

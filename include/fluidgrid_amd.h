@@ -408,6 +408,31 @@ int fg_body_mass_properties_f64(const fg_float4* worldSpheres, const unsigned in
 /* Bounds of all sphere centres expanded by margin: out2[0] min, out2[1] max. */
 int fg_sphere_bounds(const fg_float4* spheres, unsigned int numSpheres, float margin, fg_float4* out2);
 
+/* Inflow emission (IEmitter::update, Emitter.h:29; the reference's only
+ * emitter leaves it empty, Emitter.h:46).  The particle array holds a
+ * reserve of inactive slots (w <= 0) parked where no active particle
+ * reaches; layers of it are activated on a lattice in the inflow window. */
+typedef struct fg_inflow_desc {
+    fg_float3 origin;        /* lattice site (0, 0) of the inflow window */
+    fg_float3 axisU, axisV;  /* in-plane lattice axes (unit vectors) */
+    fg_float3 normal;        /* inflow direction (unit vector) */
+    float spacing;           /* lattice spacing, and the distance between layers */
+    float speed;             /* inflow speed along normal */
+    unsigned int nu, nv;     /* lattice sites per layer */
+    unsigned int firstSlot;  /* first reserve slot */
+    unsigned int numLayers;  /* the reserve: numLayers * nu * nv slots from firstSlot */
+} fg_inflow_desc;
+
+/* Emission at the start of a step.  With k = *stepCounter and period =
+ * spacing / speed, layers before step k = floor(k dt / period) (FP64); when
+ * that count grows from k to k+1, layer l = floor(k dt / period), if
+ * l < numLayers, is emitted: slot firstSlot + l nu nv + iu + nu iv gets
+ * position origin + (iu spacing) axisU + (iv spacing) axisV (FP32, left to
+ * right), w = 1, velocity speed * normal.  *stepCounter is then incremented.
+ * Two launches, no host synchronisation (graph-capturable). */
+int fg_emit_inflow(fg_float4* positions, fg_float4* velocities, unsigned int* stepCounter, const fg_inflow_desc* desc,
+                   float dt);
+
 /* FP64 global sums of one fluid for conservation reports (no counterpart in
  * the reference).  out8 (device memory) receives, over active particles
  * (w > 0):

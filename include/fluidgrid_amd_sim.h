@@ -50,6 +50,11 @@ int fg_sim_fluid_buffers(int fluid, fg_fluid_buffers* out);
  * run with stage_timing on (7 floats). */
 int fg_sim_stage_times(float* out7);
 
+/* Attaches an inflow to fluid idx: its reserve slots (inactive, parked away
+ * from the fluid) are emitted layer by layer at the start of every step from
+ * now on (fg_emit_inflow; the emission step counter starts at 0). */
+int fg_sim_set_inflow(int idx, const fg_inflow_desc* desc);
+
 /* FP64 conservation sums of fluid idx (fg_conservation_f64) into host out8:
  * count, mass, momentum x/y/z, kinetic, gravitational potential and EOS
  * internal energy.  Synchronises the simulator's stream. */

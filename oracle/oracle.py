@@ -159,6 +159,35 @@ def step(pos, vel, fluid, common, kernels, mode, rho_p_prev=None, end_init=None,
                 rho_p=rho_p, vel=v_new, pos=p_new, acc=acc, mag=mag)
 
 
+def emit_inflow(pos, vel, step, inflow, dt):
+    """Inflow emission at the start of step `step` (fg_emit_inflow): when
+    floor((k+1) dt / period) > floor(k dt / period), period = spacing / speed
+    (FP64), layer l = floor(k dt / period) of nu x nv reserve slots from
+    firstSlot is placed on the window lattice origin + iu s U + iv s V
+    (FP32, left to right, no fused multiply-add) with velocity speed * normal;
+    layers beyond numLayers are not emitted.  Returns the number of particles
+    emitted."""
+    d = inflow
+    period = float(np.float32(d.spacing)) / float(np.float32(d.speed))
+    l0 = int(np.floor(step * float(np.float32(dt)) / period))
+    l1 = int(np.floor((step + 1) * float(np.float32(dt)) / period))
+    if l1 == l0 or l0 >= d.numLayers:
+        return 0
+    per = d.nu * d.nv
+    t = np.arange(per)
+    iu, iv = (t % d.nu).astype(np.float32), (t // d.nu).astype(np.float32)
+    s32 = np.float32(d.spacing)
+    fu, fv = iu * s32, iv * s32
+    o, U, V, N = (np.array(tuple(a), np.float32) for a in (d.origin, d.axisU, d.axisV, d.normal))
+    slots = d.firstSlot + l0 * per + t
+    for ax in range(3):
+        pos[slots, ax] = (o[ax] + fu * U[ax]) + fv * V[ax]
+        vel[slots, ax] = np.float32(d.speed) * N[ax]
+    pos[slots, 3] = 1.0
+    vel[slots, 3] = 0.0
+    return per
+
+
 def conservation(pos, vel, spos, rho_p, mass, gravity, scale, rest_density, gas_constant):
     """FP64 global sums of one fluid (fg_conservation_f64): over particles with
     w > 0, count, mass, momentum m v, kinetic energy m |v|^2 / 2, potential

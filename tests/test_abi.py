@@ -56,8 +56,9 @@ def test_struct_layouts_match_c_header(tmp_path):
         #include <stddef.h>
         #include "{ROOT}/include/fluidgrid_amd.h"
         int main(void) {{
-          printf("%zu %zu %zu %zu %zu\\n", sizeof(fg_common_fluid_params), sizeof(fg_fluid_params),
-                 sizeof(fg_kernels_cache), sizeof(fg_common_rigid_body_params), sizeof(fg_scene_desc));
+          printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(fg_common_fluid_params), sizeof(fg_fluid_params),
+                 sizeof(fg_kernels_cache), sizeof(fg_common_rigid_body_params), sizeof(fg_scene_desc),
+                 sizeof(fg_inflow_desc), sizeof(fg_body_grid));
           printf("%zu %zu %zu %zu\\n", offsetof(fg_common_fluid_params, volume_min),
                  offsetof(fg_common_fluid_params, deltaTimeIntegration),
                  offsetof(fg_common_rigid_body_params, global_external_force),
@@ -70,7 +71,8 @@ def test_struct_layouts_match_c_header(tmp_path):
     got = list(map(int, out))
     assert got[:5] == [C.sizeof(abi.CommonFluidParams), C.sizeof(abi.FluidParams), C.sizeof(abi.KernelsCache),
                        C.sizeof(abi.CommonRigidBodyParams), C.sizeof(abi.SceneDesc)] == [104, 28, 12, 112, 160]
-    assert got[5:] == [abi.CommonFluidParams.volume_min.offset, abi.CommonFluidParams.deltaTimeIntegration.offset,
+    assert got[5:7] == [C.sizeof(abi.InflowDesc), C.sizeof(abi.BodyGrid)] and got[5] == 72
+    assert got[7:] == [abi.CommonFluidParams.volume_min.offset, abi.CommonFluidParams.deltaTimeIntegration.offset,
                        abi.CommonRigidBodyParams.global_external_force.offset,
                        abi.CommonRigidBodyParams.particleRadius.offset]
 
