@@ -70,6 +70,8 @@ def measure(scene, steps, warmup, fused=True):
     try:
         if bodies is not None:
             sim.add_bodies(bodies)
+        if scene.info.get("inflow") is not None:
+            sim.set_inflow(scene.info["inflow"])
         c_start = sim.conservation()
         sim.step(warmup)
         sim.synchronize()
@@ -223,6 +225,7 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-8m", action="store_true")
+    ap.add_argument("--no-33m", action="store_true", help="skip the 33.5M-particle config-4 line (--no-8m skips it too)")
     ap.add_argument("--unfused", action="store_true")
     ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble line")
@@ -329,6 +332,24 @@ def main():
                 "stage_ms": {k: round(v, 5) for k, v in r8["stage_ms"].items()},
                 "finite": r8["finite"], "drift": r8["drift"],
             }
+    if not a.no_33m and not a.no_8m and a.config == 1:
+        s4 = scenes.config4_scene()
+        b4 = s4.info["bodies"]
+        log(f"scene {s4.name}: n={s4.n:,} ({s4.info['active']:,} active), {b4.b} bodies, {b4.m:,} spheres")
+        k4 = min(a.steps, s4.steps)  # the configuration's 100 steps: two inflow layers are emitted
+        r4 = measure(s4, k4, 5, fused)
+        act = s4.info["active"]
+        out["config"]["also_33M"] = {
+            "workload": f"plain FP32 dam break, {act:,} active particles (+{s4.n - act:,} inflow reserve slots), "
+                        f"{s4.num_buckets:,} LINEAR buckets, h=0.003, dt=8e-5, 0.5 m/s inflow through the x-min "
+                        f"face, coupled two-way with {b4.b} rigid bodies (tori and boxes, {b4.m:,} spheres)",
+            "particles": act, "slots": s4.n, "steps": k4, "value": act * k4 / r4["seconds"],
+            "ms_per_step": r4["ms_per_step"],
+            "step_hbm_frac": step_bytes(s4, r4["c_occ"]) / (r4["ms_per_step"] * 1e-3) / HBM_PEAK,
+            "stage_ms": {k: round(v, 5) for k, v in r4["stage_ms"].items()},
+            "finite": r4["finite"], "drift": r4["drift"],
+        }
+        del s4
     if not a.no_batch:
         ens = scenes.ensemble(num_scenes=a.batch_scenes)
         log(f"ensemble: {len(ens.scenes)} scenes, n={ens.n:,}")

@@ -96,14 +96,20 @@ def dam_break(lattice, h, box, dt, steps=100, seed=SEED_BASE, name="dam_break",
     nx, ny, nz = lattice
     n = nx * ny * nz
     half = 0.5 * h
-    iy, iz, ix = np.meshgrid(np.arange(ny), np.arange(nz), np.arange(nx), indexing="ij")
-    lat = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1).astype(np.float64)
-    base = half * (1.0 + lat)
-    if jitter:
-        u = uniform01(seed, 3 * n).reshape(n, 3)
-        base = base + (2.0 * u - 1.0) * (jitter * h)
     pos = np.zeros((n, 4), np.float32)
-    pos[:, :3] = base.astype(np.float32)
+    # particle l = (iy nz + iz) nx + ix (y layers outermost, x innermost);
+    # its jitter is draws 3l..3l+2 of the stream.  Built in chunks so that
+    # large lattices stay in cache; every element is computed the same way.
+    chunk = 1 << 21
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        l = np.arange(a, b, dtype=np.int64)
+        lat = np.stack([l % nx, l // (nx * nz), (l // nx) % nz], axis=1).astype(np.float64)
+        base = half * (1.0 + lat)
+        if jitter:
+            u = uniform01(seed, 3 * (b - a), offset=3 * a).reshape(b - a, 3)
+            base = base + (2.0 * u - 1.0) * (jitter * h)
+        pos[a:b, :3] = base.astype(np.float32)
     pos[:, 3] = 1.0
     vel = np.zeros((n, 4), np.float32)
 
@@ -416,6 +422,43 @@ def coupled_config_scene(cfg_id=1, seed=None):
     else:
         raise ValueError(cfg_id)
     s.name = f"coupled_cfg{cfg_id}"
+    s.info["bodies"] = bodies
+    return s
+
+
+def config4_scene(seed=None, num_bodies=64):
+    """Config 4: the 33,554,432-particle dam break (512x256x256 lattice at
+    h/2, h = 0.003, box [0,2]x[0,1]x[0,1], dt = 8e-5) with a 0.5 m/s inflow
+    through the x-min face above the block, and 64 rigid bodies, half tori
+    and half boxes, overall size uniform in [0.03, 0.1] m and density uniform
+    in [300, 900] kg/m^3, on three layers above the fluid (seeded poses)."""
+    s = config_scene(4)
+    h = s.info["h"]
+    rng = np.random.default_rng(SEED_BASE + 4 if seed is None else seed)
+    top = float(s.pos[:, 1].max())
+    sets, masses, names = [], [], []
+    nx, nz = 8, 4
+    for b in range(num_bodies):
+        size = rng.uniform(0.03, 0.1)
+        rho = rng.uniform(300.0, 900.0)
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        ix, iz, iy = b % nx, (b // nx) % nz, b // (nx * nz)
+        c = np.array([0.2 + 0.225 * ix, top + 0.1 + 0.15 * iy, 0.125 + 0.25 * iz])
+        if b % 2 == 0:
+            R = size / 2.6
+            r = 0.3 * R
+            pts = torus_spheres(R, r, 0.5 * h)
+            masses.append(rho * 2.0 * np.pi ** 2 * R * r * r)
+            names.append(f"torus{b}")
+        else:
+            pts = box_spheres(size, 0.5 * h)
+            masses.append(rho * size ** 3)
+            names.append(f"box{b}")
+        sets.append(quat_rotate(q, pts) + c)
+    bodies = make_bodies(sets, masses, h, s.common, names=names, buckets=(128, 128, 128))
+    s = with_inflow(s, (top + 0.05, 0.35), (top + 0.2, 0.5), speed=0.5, layers=6)
+    s.name = "inflow_cfg4"
     s.info["bodies"] = bodies
     return s
 
