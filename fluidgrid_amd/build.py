@@ -72,7 +72,24 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_app()
     return LIB
+
+
+APP_SRC = os.path.join(HERE, "app", "AppMain.cpp")
+APP = os.path.join(HERE, "fg_appmain")
+
+
+def build_app():
+    """The C++ host program (fluidgrid_amd/app/AppMain.cpp) linked against the
+    library, found next to it at run time."""
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), f"--offload-arch={ARCH}",
+           APP_SRC, "-o", APP + ".tmp", "-L", HERE, "-lfluidgrid_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"app build failed\n{' '.join(cmd)}\n{r.stderr}")
+    os.replace(APP + ".tmp", APP)
+    return APP
 
 
 def main():
