@@ -59,7 +59,7 @@ __device__ __forceinline__ uint32_t contact_key(const float4& sp, const CellXYZ&
 // relation (1,1,1) and PREFETCH the 27 cell starts are loaded together
 // before any cell is walked (same cells, same order, same sums; more
 // registers, so it is used by the body-sphere pass only).
-template <bool PREFETCH>
+template <bool PREFETCH, bool XOR_ONLY = false>
 __device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv, float sr,
                                               const float4* __restrict__ cpos, const float4* __restrict__ cvel,
                                               float cr, float ks, float kd, int3 rel,
@@ -68,11 +68,13 @@ __device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv
                                               uint32_t limit, uint32_t vlimit) {
     F3 F = f3(0.0f, 0.0f, 0.0f);
     CellXYZ c{0, 0, 0};
-    if (g.mode == FG_KEY_LINEAR) c = grid_cell(sp.x, sp.y, sp.z, g);
+    GridDesc gg = g;
+    if (XOR_ONLY) gg.mode = FG_KEY_REF_XOR;  // body grids: only the hash is compiled in
+    if (!XOR_ONLY && g.mode == FG_KEY_LINEAR) c = grid_cell(sp.x, sp.y, sp.z, g);
     if (PREFETCH && rel.x == 1 && rel.y == 1 && rel.z == 1) {
         uint32_t keys[27], s0[27];
 #pragma unroll
-        for (int k = 0; k < 27; k++) keys[k] = contact_key(sp, c, k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1, g);
+        for (int k = 0; k < 27; k++) keys[k] = contact_key(sp, c, k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1, gg);
 #pragma unroll
         for (int k = 0; k < 27; k++) s0[k] = keys[k] != kEmpty ? start[keys[k]] : kEmpty;
 #pragma unroll
@@ -85,7 +87,7 @@ __device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv
     for (int z = -rel.z; z <= rel.z; z++)
         for (int y = -rel.y; y <= rel.y; y++)
             for (int x = -rel.x; x <= rel.x; x++) {
-                const uint32_t key = contact_key(sp, c, x, y, z, g);
+                const uint32_t key = contact_key(sp, c, x, y, z, gg);
                 if (key == kEmpty) continue;
                 const uint32_t s0 = start[key];
                 F = add(F, contact_range(sp, sv, sr, cr, cpos, cvel, s0, s0 != kEmpty ? end[key] : 0u, ks, kd, w_index,
@@ -102,8 +104,10 @@ __device__ __forceinline__ F3 fluid_body_contact(const float4& pos, const float4
         if (pos.x < lo.x || pos.y < lo.y || pos.z < lo.z || pos.x > hi.x || pos.y > hi.y || pos.z > hi.z)
             return f3(0.0f, 0.0f, 0.0f);
     }
-    return contact_stencil<false>(pos, vel, bc.fluid_radius, bc.pos, bc.vel, bc.body_radius, bc.ks, bc.kd, bc.relation,
-                           bc.start, bc.end, bc.grid, bc.vel != nullptr, false, 0, bc.eps, bc.count, bc.nvel);
+    // body grids are keyed with the reference hash (body_grid in fg_rigid.hip)
+    return contact_stencil<false, true>(pos, vel, bc.fluid_radius, bc.pos, bc.vel, bc.body_radius, bc.ks, bc.kd,
+                                        bc.relation, bc.start, bc.end, bc.grid, bc.vel != nullptr, false, 0, bc.eps,
+                                        bc.count, bc.nvel);
 }
 
 // External force of K_FluidSim.cu:369-386: static grid first, then the
