@@ -1,6 +1,7 @@
 """Throughput of the SPH step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--fluid-only] [--no-cpu-baseline]
+                    [--no-8m] [--no-33m] [--no-batch] [--batch-scenes S]
 
 A step is one pass of the per-timestep pipeline over every particle of a
 synthetic dam-break scene, driven through the C++ HeterogeneousSimulator and
@@ -12,8 +13,13 @@ before the timed region.  `value` is particle-steps/s.
 
 The default workload is the 1,048,576-particle dam break coupled two-way to a
 rigid torus (R 0.15 m, r 0.05 m, ~20K boundary spheres, density 500) dropped
-from 0.3 m above the fluid; the second line is the plain FP32 8,388,608-
-particle fluid step.
+from 0.3 m above the fluid.  Further lines in `config`: the 8,388,608-particle
+step with four rigid boxes (`also_8M`) and without them
+(`also_8M_fluid_only`); the 33,554,432-particle step with 64 rigid bodies and
+a 0.5 m/s inflow (`also_33M`); and the batched form over 256 independent
+scenes, fixed and autotuned, beside the same scenes stepped one at a time
+(`batched`).  Every single-fluid line carries a `drift` report from the FP64
+conservation sums (mass, momentum against the gravity impulse, energies).
 
 Roofline: algorithmic bytes per particle (hash 24, one sort pass 16, cells +
 reorder 72, density 24, forces 60, integrate 48) plus 4 B per bucket for the
