@@ -313,23 +313,40 @@ struct PairAcc {
 };
 
 #pragma clang fp contract(off)
+// The accumulation of one pair with r2u = |x_i - x_j|^2 (unscaled) < h^2.
+__device__ __forceinline__ void pair_accumulate(PairAcc& A, float dx, float dy, float dz, float r2u,
+                                                const float4& vel, float pri, float3 vj, float aj, float bj,
+                                                float h, float s) {
+    const float iru = __builtin_amdgcn_rsqf(r2u);
+    const float hr = __builtin_fmaf(-(r2u * iru), s, h);
+    const float cp_ = hr * hr * __builtin_fmaf(pri, aj, bj) * iru;
+    A.px = __builtin_fmaf(dx, cp_, A.px);
+    A.py = __builtin_fmaf(dy, cp_, A.py);
+    A.pz = __builtin_fmaf(dz, cp_, A.pz);
+    const float cv = hr * aj;
+    A.vx = __builtin_fmaf(vj.x - vel.x, cv, A.vx);
+    A.vy = __builtin_fmaf(vj.y - vel.y, cv, A.vy);
+    A.vz = __builtin_fmaf(vj.z - vel.z, cv, A.vz);
+}
+
 __device__ __forceinline__ void pair_term(PairAcc& A, uint32_t i, uint32_t j, const float4& pos, const float4& vel,
                                           float pri, float3 q, float3 vj, float aj, float bj, float h, float s,
                                           float hhu) {
     const float dx = pos.x - q.x, dy = pos.y - q.y, dz = pos.z - q.z;
     const float r2u = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));  // unscaled
-    if (r2u < hhu && j != i) {
-        const float iru = __builtin_amdgcn_rsqf(r2u);
-        const float hr = __builtin_fmaf(-(r2u * iru), s, h);
-        const float cp_ = hr * hr * __builtin_fmaf(pri, aj, bj) * iru;
-        A.px = __builtin_fmaf(dx, cp_, A.px);
-        A.py = __builtin_fmaf(dy, cp_, A.py);
-        A.pz = __builtin_fmaf(dz, cp_, A.pz);
-        const float cv = hr * aj;
-        A.vx = __builtin_fmaf(vj.x - vel.x, cv, A.vx);
-        A.vy = __builtin_fmaf(vj.y - vel.y, cv, A.vy);
-        A.vz = __builtin_fmaf(vj.z - vel.z, cv, A.vz);
-    }
+    if (r2u < hhu && j != i) pair_accumulate(A, dx, dy, dz, r2u, vel, pri, vj, aj, bj, h, s);
+}
+
+// A pair from a neighbour mask (self bit cleared).  The density pass set the
+// bit after the same test on the same r2u (the same operations on the same
+// sorted positions), so r2u < h^2 holds and is not tested again: without
+// the branch the neighbour's velocity and inverse-density loads are issued
+// together with its position load instead of after the test.
+__device__ __forceinline__ void pair_term_masked(PairAcc& A, const float4& pos, const float4& vel, float pri,
+                                                 float3 q, float3 vj, float aj, float bj, float h, float s) {
+    const float dx = pos.x - q.x, dy = pos.y - q.y, dz = pos.z - q.z;
+    const float r2u = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    pair_accumulate(A, dx, dy, dz, r2u, vel, pri, vj, aj, bj, h, s);
 }
 
 // The same from a stored (rho_j, p_j) (rows without a mask), with the
@@ -422,7 +439,7 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                     mm &= mm - 1ull;
                     const uint32_t j = rb + k;
                     const float2 ab = inv[j];
-                    pair_term(A, i, j, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), ab.x, ab.y, h, s, hhu);
+                    pair_term_masked(A, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), ab.x, ab.y, h, s);
                 }
             }
         }
@@ -432,6 +449,8 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
             if (!row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) continue;
             if (MASKS && re - rb <= kRowMaskBits) {
                 unsigned long long mm = masks[(size_t)row * n + i];
+                // the particle itself is in its own row's mask; it adds nothing
+                if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
                 while (mm) {
                     const uint32_t k1 = (uint32_t)__builtin_ctzll(mm);
                     mm &= mm - 1ull;
@@ -442,8 +461,13 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                     const float3 q1 = ld_xyz(sp + j1), q2 = ld_xyz(sp + j2);
                     const float3 v1 = ld_xyz(sv + j1), v2 = ld_xyz(sv + j2);
                     const float2 d1 = inv[j1], d2 = inv[j2];
-                    pair_term(A, i, j1, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s, hhu);
-                    if (two) pair_term(A, i, j2, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s, hhu);
+                    // Keeps the second pair's loads beside the first pair's (one
+                    // memory round trip per iteration): without a use here the
+                    // compiler sinks them into the `two` branch below.
+                    asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(v2.x), "v"(v2.y), "v"(v2.z), "v"(d2.x),
+                                 "v"(d2.y));
+                    pair_term_masked(A, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s);
+                    if (two) pair_term_masked(A, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s);
                 }
             } else {
                 for (uint32_t j = rb; j < re; j++)
