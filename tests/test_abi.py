@@ -83,3 +83,32 @@ def test_errors_are_recorded_not_fatal():
     assert L.fg_last_error() == 1
     assert b"unknown mode" in L.fg_last_error_string()
     assert L.fg_last_error() == 0
+
+
+def test_new_entry_points_reject_bad_arguments_before_any_launch():
+    """Argument checks of the extension entry points run on the host and
+    record FG_ERR_INVALID_ARGUMENT without touching a device (null pointers,
+    an empty inflow window, a short workspace, missing grids)."""
+    import ctypes as C
+    L = lib.load()
+    bad = 1  # FG_ERR_INVALID_ARGUMENT
+    d = abi.InflowDesc()
+    d.nu, d.nv, d.spacing, d.speed, d.numLayers = 4, 4, 0.01, 1.0, 2
+    cases = [
+        lambda: L.fg_emit_inflow(None, None, None, C.byref(d), 1e-3),
+        lambda: L.fg_emit_inflow(C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), None, 1e-3),
+        lambda: L.fg_emit_inflow(C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), C.byref(abi.InflowDesc()), 1e-3),
+        lambda: L.fg_conservation_f64(None, None, None, None, 10, 1.0, 1000.0, 3.0, None, None, 0),
+        lambda: L.fg_conservation_f64(C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), 10, 1.0,
+                                      1000.0, 3.0, C.c_void_p(16), C.c_void_p(16), 8),
+        lambda: L.fg_body_forces_from_bodies(C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), C.c_void_p(16),
+                                             C.c_void_p(16), 10, 1, 0.01, None, None, None, 0),
+        lambda: L.fg_body_bounds(None, None, 3, 0.1, None),
+        lambda: L.fg_sph_forces_coupled(C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), C.c_void_p(16),
+                                        C.c_void_p(16), None, C.c_void_p(16), None, C.c_void_p(16), 10, 1.0, 1.0,
+                                        None, None),
+    ]
+    for k, call in enumerate(cases):
+        assert call() == bad, k
+        assert L.fg_last_error() == bad, k
+        assert L.fg_last_error() == 0
