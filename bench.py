@@ -220,7 +220,19 @@ def cpu_baseline(scene, seconds_budget=15.0):
             steps += 1
         what = f"oracle steps of the {scene.n:,}-particle scene"
     return dict(value=scene.n * steps / t_total, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"{steps} {what} ({t_total:.1f} s, {threads} OpenMP threads)")
+                sample=f"{steps} {what} ({t_total:.1f} s, {threads} OpenMP threads)", cpu=cpu_model(),
+                host_cpus=os.cpu_count())
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
