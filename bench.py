@@ -1,7 +1,7 @@
 """Throughput of the SPH step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--fluid-only] [--no-cpu-baseline]
-                    [--no-8m] [--no-33m] [--no-batch] [--batch-scenes S]
+                    [--no-8m] [--no-33m] [--no-batch] [--batch-scenes S] [--no-mesh]
 
 A step is one pass of the per-timestep pipeline over every particle of a
 synthetic dam-break scene, driven through the C++ HeterogeneousSimulator and
@@ -16,10 +16,11 @@ rigid torus (R 0.15 m, r 0.05 m, ~20K boundary spheres, density 500) dropped
 from 0.3 m above the fluid.  Further lines in `config`: the 8,388,608-particle
 step with four rigid boxes (`also_8M`) and without them
 (`also_8M_fluid_only`); the 33,554,432-particle step with 64 rigid bodies and
-a 0.5 m/s inflow (`also_33M`); and the batched form over 256 independent
+a 0.5 m/s inflow (`also_33M`); the batched form over 256 independent
 scenes, fixed and autotuned, beside the same scenes stepped one at a time
-(`batched`).  Every single-fluid line carries a `drift` report from the FP64
-conservation sums (mass, momentum against the gravity impulse, energies).
+(`batched`); and the mesh sampler's winding-number kernel (`mesh_sampler`).
+Every single-fluid line carries a `drift` report from the FP64 conservation
+sums (mass, momentum against the gravity impulse, energies).
 
 Roofline: algorithmic bytes per particle (hash 24, one sort pass 16, cells +
 reorder 72, density 24, forces 60, integrate 48) plus 4 B per bucket for the
@@ -190,6 +191,44 @@ def measure_batched(ens, steps, warmup, tune_steps=5):
     return out
 
 
+def measure_mesh(nu=200, nv=100, points_across=128, reps=5):
+    """The mesh sampler's winding-number kernel on a 40,000-triangle torus
+    mesh over its 128 x 32 x 128 lattice, inputs resident on the device;
+    HIP events on the library's stream around `reps` launches."""
+    import ctypes as C
+    from fluidgrid_amd import mesh as fm
+    L = lib.load()
+    v, t = fm.torus_mesh(0.3, 0.1, nu, nv)
+    spacing = 0.8 / points_across
+    origin, dims = fm.lattice(v, spacing)
+    npts = dims[0] * dims[1] * dims[2]
+    dv = lib.DeviceArray.from_numpy(np.ascontiguousarray(v))
+    dt = lib.DeviceArray.from_numpy(np.ascontiguousarray(t))
+    dw = lib.DeviceArray((npts,), np.float32)
+    args = (dv.ptr, len(v), dt.ptr, len(t), abi.Float3(*origin), spacing, *dims, dw.ptr)
+    lib.check(L.fg_mesh_winding_lattice(*args), "mesh warm-up")
+    lib.check(L.fg_synchronize(), "sync")
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    lib.check(L.fg_event_create(C.byref(e0)), "event")
+    lib.check(L.fg_event_create(C.byref(e1)), "event")
+    lib.check(L.fg_event_record(e0), "event")
+    for _ in range(reps):
+        L.fg_mesh_winding_lattice(*args)
+    lib.check(L.fg_event_record(e1), "event")
+    lib.check(L.fg_synchronize(), "sync")
+    ms = C.c_float()
+    lib.check(L.fg_event_elapsed_ms(e0, e1, C.byref(ms)), "elapsed")
+    L.fg_event_destroy(e0)
+    L.fg_event_destroy(e1)
+    per = ms.value / reps
+    w = dw.numpy()
+    return {"workload": f"winding numbers of a {len(t):,}-triangle torus mesh on its {dims[0]}x{dims[1]}x{dims[2]} "
+                        "sampling lattice (fg_mesh_winding_lattice)",
+            "triangles": int(len(t)), "points": int(npts), "ms": per,
+            "point_triangle_pairs_per_s": npts * len(t) / (per * 1e-3),
+            "inside_points": int((w > 0.5).sum()), "finite": bool(np.isfinite(w).all())}
+
+
 def step_bytes(scene, c_occ):
     bodies = scene.info.get("bodies")
     m = bodies.m if bodies is not None else 0
@@ -248,6 +287,7 @@ def main():
     ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-sampler line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB ten times (reference for HBM byte counters)")
     a = ap.parse_args()
@@ -387,6 +427,9 @@ def main():
             "speedup_vs_plain_per_scene": rb["tuned"]["value"] / rb["plain_per_scene"]["value"],
             "speedup_vs_plain_1M": rb["tuned"]["value"] / plain,
         }
+    if not a.no_mesh:
+        log("mesh sampler ...")
+        out["config"]["mesh_sampler"] = measure_mesh()
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(scene)

@@ -5,9 +5,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
+
+#include "fluidgrid/MeshSampler.h"
 
 namespace Simulation {
 namespace RigidBody {
@@ -72,24 +76,55 @@ RigidBody::RigidBody(bool isStatic, float mass, float particleRadius, std::vecto
 RigidBody* RigidBody::loadFromFile(const std::string& filename, float3 origin) {
     std::ifstream f(filename);
     if (!f) throw std::runtime_error("RigidBody::loadFromFile: cannot open " + filename);
-    std::string tag;
-    f >> tag;
-    bool isStatic = (tag == "static");
-    float mass = 0.0f;
-    if (!isStatic) f >> mass;
-    float radius = 0.0f;
-    std::string mesh;
-    int count = 0;
-    f >> radius >> mesh >> count;
-    if (!f || count < 0) throw std::runtime_error("RigidBody::loadFromFile: bad header in " + filename);
+    auto next = [&](const char* what) {
+        std::string line;
+        if (!std::getline(f, line))
+            throw std::runtime_error(std::string("RigidBody::loadFromFile: no ") + what + " line in " + filename);
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        return line;
+    };
+    const bool isStatic = next("type") == "static";
+    const float mass = isStatic ? 0.0f : (float)std::atof(next("mass").c_str());
+    const float radius = (float)std::atof(next("radius").c_str());
+    const std::string mesh = next("mesh");
+    const double count = std::atof(next("count").c_str());
+    if (!(count >= 0.0) || count > 2147483647.0)
+        throw std::runtime_error("RigidBody::loadFromFile: bad sphere count in " + filename);
     std::vector<float4> s;
-    s.reserve(count);
-    for (int i = 0; i < count; i++) {
-        float x, y, z;
-        if (!(f >> x >> y >> z)) throw std::runtime_error("RigidBody::loadFromFile: short file " + filename);
-        s.push_back(make_float4(x + origin.x, y + origin.y, z + origin.z, radius));
+    s.reserve((size_t)count);
+    for (int i = 0; i < (int)count; i++) {
+        std::istringstream line(next("sphere"));
+        std::string x, y, z;
+        line >> x >> y >> z;
+        s.push_back(make_float4((float)std::atof(x.c_str()) + origin.x, (float)std::atof(y.c_str()) + origin.y,
+                                (float)std::atof(z.c_str()) + origin.z, radius));
     }
-    return new RigidBody(isStatic, mass, radius, std::move(s));
+    RigidBody* rb = new RigidBody(isStatic, mass, radius, std::move(s));
+    rb->setMeshName(mesh);
+    return rb;
+}
+
+void RigidBody::saveToFile(const std::string& filename, float3 origin) const {
+    FILE* f = std::fopen(filename.c_str(), "w");
+    if (!f) throw std::runtime_error("RigidBody::saveToFile: cannot open " + filename);
+    if (mIsStatic)
+        std::fprintf(f, "static\n");
+    else
+        std::fprintf(f, "dynamic\n%.9g\n", (double)mMass);
+    std::fprintf(f, "%.9g\n%s\n%zu\n", (double)mParticleRadius, mMeshName.c_str(), mSpheres.size());
+    for (const float4& p : mSpheres)
+        std::fprintf(f, "%.9g %.9g %.9g\n", (double)(p.x - origin.x), (double)(p.y - origin.y),
+                     (double)(p.z - origin.z));
+    if (std::fclose(f) != 0) throw std::runtime_error("RigidBody::saveToFile: write failed for " + filename);
+}
+
+RigidBody* RigidBody::fromMesh(const std::string& objFile, bool isStatic, float mass, float particleRadius,
+                               float3 origin, bool shellOnly) {
+    std::vector<float4> s = sampleMeshSpheres(loadObj(objFile), 2.0f * particleRadius, shellOnly);
+    for (float4& p : s) p = make_float4(p.x + origin.x, p.y + origin.y, p.z + origin.z, particleRadius);
+    RigidBody* rb = new RigidBody(isStatic, isStatic ? 0.0f : mass, particleRadius, std::move(s));
+    rb->setMeshName(objFile);
+    return rb;
 }
 
 RigidBodySimulator* RigidBodySimulator::instance() {

@@ -115,6 +115,15 @@ SIGNATURES = {
     "fg_body_reduce_f64": (i32, [vp, vp, vp, u32, vp, vp, vp, vp]),
     "fg_body_mass_properties_f64": (i32, [vp, vp, u32, vp, vp, vp]),
     "fg_sphere_bounds": (i32, [vp, u32, f32, vp]),
+    # meshes and body files (fluidgrid_amd_mesh.h)
+    "fg_mesh_winding_lattice": (i32, [vp, u32, vp, u32, abi.Float3, f32, u32, u32, u32, vp]),
+    "fg_mesh_load_obj": (i32, [C.c_char_p, C.POINTER(C.c_uint), C.POINTER(C.c_uint), vp, vp]),
+    "fg_mesh_lattice": (i32, [vp, u32, f32, C.POINTER(abi.Float3), vp]),
+    "fg_mesh_winding_numbers": (i32, [vp, u32, vp, u32, f32, vp]),
+    "fg_mesh_sample_spheres": (i32, [vp, u32, vp, u32, f32, i32, vp, u32, C.POINTER(C.c_uint)]),
+    "fg_body_file_write": (i32, [C.c_char_p, i32, f32, f32, C.c_char_p, vp, u32]),
+    "fg_body_file_read": (i32, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                C.POINTER(C.c_uint), vp, u32, C.c_char_p, sz]),
     # tuning and batched scenes
     "fg_set_tuning": (i32, [i32, i32]),
     "fg_get_tuning": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),

@@ -36,22 +36,35 @@ public:
     // Spheres in world space; w is ignored.
     RigidBody(bool isStatic, float mass, float particleRadius, std::vector<float4> worldSpheres);
 
-    // RigidBody.cpp:55-108 text format: "static" or "dynamic <mass>", then the
-    // sphere radius, a mesh name, the sphere count and one "x y z" line per
-    // sphere (positions relative to the body origin, placed at `origin`).
+    // RigidBody.cpp:55-108 text format, one item per line: "static", or any
+    // other first line followed by the mass; the sphere radius; the name of
+    // the source mesh; the sphere count; one "x y z" line per sphere
+    // (positions relative to the body origin, placed at `origin`).  Numbers
+    // are read as atof does (a malformed number reads as 0); a file that
+    // ends early throws std::runtime_error.
     static RigidBody* loadFromFile(const std::string& filename, float3 origin);
+    // The same format, written so that loadFromFile reads back the same
+    // floats (9 significant digits); positions relative to `origin`.
+    void saveToFile(const std::string& filename, float3 origin) const;
+    // A body sampled from a Wavefront OBJ mesh (sampleMeshSpheres with
+    // spacing 2 * particleRadius, solid or shell), translated by `origin`.
+    static RigidBody* fromMesh(const std::string& objFile, bool isStatic, float mass, float particleRadius,
+                               float3 origin, bool shellOnly);
 
     const std::vector<float4>& getWorldSpheres() const { return mSpheres; }
     int getNumParticles() const { return (int)mSpheres.size(); }
     float getParticleRadius() const { return mParticleRadius; }
     float getMass() const { return mMass; }
     bool isStatic() const { return mIsStatic; }
+    const std::string& getMeshName() const { return mMeshName; }
+    void setMeshName(const std::string& name) { mMeshName = name; }
 
 private:
     bool mIsStatic;
     float mMass;
     float mParticleRadius;
     std::vector<float4> mSpheres;
+    std::string mMeshName;
 };
 
 class RigidBodySimulator {

@@ -450,6 +450,21 @@ int fg_conservation_f64(const fg_float4* positions, const fg_float4* velocities,
                         const fg_float2* density_pressure, unsigned int numParticles, float mass, float restDensity,
                         float gasConstant, double* out8, void* workspace, size_t workspace_bytes);
 
+/* Generalized winding numbers of a triangle mesh on a point lattice, for
+ * sampling meshes into body sphere sets (the reference's body particle
+ * files, Simulators/RigidBody.cpp:55-108, name the mesh they came from).
+ * Point (ix, iy, iz), 0 <= i < n per axis, is origin + i * spacing per axis
+ * (product and sum each rounded to FP32); winding[ix + nx (iy + ny iz)]
+ * receives sum_t Omega_t / (4 pi), Omega_t the signed solid angle of
+ * triangle t (corners vertices[triangles[3t..3t+2]], counter-clockwise seen
+ * from outside), each term 2 atan2(a.(b x c), |a||b||c| + (a.b)|c| +
+ * (b.c)|a| + (c.a)|b|) in FP32 with a, b, c the corners minus the point,
+ * summed in FP64.  A triangle with a corner index >= numVertices adds 0.
+ * Device pointers; at most 2^34 points. */
+int fg_mesh_winding_lattice(const fg_float3* vertices, unsigned int numVertices, const unsigned int* triangles,
+                            unsigned int numTriangles, fg_float3 origin, float spacing, unsigned int nx,
+                            unsigned int ny, unsigned int nz, float* winding);
+
 /* ------------------------------------------------------------------------ */
 /* 3. Batched scenes (no counterpart in the reference)                       */
 /* ------------------------------------------------------------------------ */

@@ -876,6 +876,35 @@ void orc_body_mass_props_f64(const fg_float4* world, uint32_t count, double mass
 
 void orc_set_reverse_order(int r) { g_reverse = r; }
 
+/* Generalized winding numbers in FP64 (Van Oosterom & Strackee 1983 solid
+ * angle per triangle, summed over triangles / 4 pi): the checker of
+ * fg_mesh_winding_lattice.  The reference has no counterpart (its body files,
+ * Simulators/RigidBody.cpp:55-108, come from meshes sampled elsewhere).
+ * Triangles naming a vertex >= nverts add 0. */
+void orc_winding(const float* pts, uint32_t npts, const float* verts, uint32_t nverts, const uint32_t* tris,
+                 uint32_t ntris, double* out) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)npts; i++) {
+        const double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
+        double acc = 0.0;
+        for (uint32_t t = 0; t < ntris; t++) {
+            const uint32_t i0 = tris[3 * t], i1 = tris[3 * t + 1], i2 = tris[3 * t + 2];
+            if (i0 >= nverts || i1 >= nverts || i2 >= nverts) continue;
+            const double ax = verts[3 * i0] - px, ay = verts[3 * i0 + 1] - py, az = verts[3 * i0 + 2] - pz;
+            const double bx = verts[3 * i1] - px, by = verts[3 * i1 + 1] - py, bz = verts[3 * i1 + 2] - pz;
+            const double cx = verts[3 * i2] - px, cy = verts[3 * i2 + 1] - py, cz = verts[3 * i2 + 2] - pz;
+            const double la = std::sqrt(ax * ax + ay * ay + az * az);
+            const double lb = std::sqrt(bx * bx + by * by + bz * bz);
+            const double lc = std::sqrt(cx * cx + cy * cy + cz * cz);
+            const double det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+            const double den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (bx * cx + by * cy + bz * cz) * la +
+                               (cx * ax + cy * ay + cz * az) * lb;
+            acc += 2.0 * std::atan2(det, den);
+        }
+        out[i] = acc / (4.0 * 3.14159265358979323846);
+    }
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -53,6 +53,7 @@ def lib():
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         L.orc_set_reverse_order.argtypes = [C.c_int]
+        L.orc_winding.argtypes = [_f32p, C.c_uint32, _f32p, C.c_uint32, _u32p, C.c_uint32, _f64p]
         _lib = L
     return _lib
 

@@ -2,6 +2,7 @@
 include/*.h, and its POD layouts agree between the C header, the C++ mirror
 and the ctypes mirror (no compute calls are made here)."""
 import ctypes as C
+import glob
 import os
 import re
 import subprocess
@@ -12,8 +13,7 @@ import pytest
 from fluidgrid_amd import abi, lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("fluidgrid_amd.h", "fluidgrid_amd_sim.h",
-                                                      "fluidgrid_amd_batch.h")]
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 DECL = re.compile(r"^\s*(?:const\s+)?(?:void|int|size_t|char|unsigned int)\s*\*?\s*(\w+)\s*\(", re.M)
 
 
@@ -37,6 +37,12 @@ def test_library_exports_every_declared_symbol():
     L = lib.load()
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert missing == []
+
+
+def test_every_c_header_is_checked():
+    names = [os.path.basename(h) for h in HEADERS]
+    for h in ("fluidgrid_amd.h", "fluidgrid_amd_sim.h", "fluidgrid_amd_batch.h", "fluidgrid_amd_mesh.h"):
+        assert h in names
 
 
 def test_bindings_cover_every_declared_symbol():
