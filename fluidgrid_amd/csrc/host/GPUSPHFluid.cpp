@@ -126,6 +126,37 @@ void GPUSPHFluid::setInflow(const fg_inflow_desc& desc) {
         throw std::runtime_error("GPUSPHFluid::setInflow: hipMemset failed");
 }
 
+unsigned int GPUSPHFluid::readEmitStep() const {
+    unsigned int k = 0;
+    if (!d_emitStep) return 0;
+    hip_check(hipDeviceSynchronize(), "readEmitStep sync");
+    hip_check(hipMemcpy(&k, d_emitStep, sizeof(k), hipMemcpyDeviceToHost), "readEmitStep");
+    return k;
+}
+
+void GPUSPHFluid::writeEmitStep(unsigned int step) {
+    if (!d_emitStep) {
+        if (step) throw std::invalid_argument("GPUSPHFluid::writeEmitStep: the fluid has no inflow");
+        return;
+    }
+    hip_check(hipDeviceSynchronize(), "writeEmitStep sync");
+    hip_check(hipMemcpy(d_emitStep, &step, sizeof(step), hipMemcpyHostToDevice), "writeEmitStep");
+}
+
+void GPUSPHFluid::downloadDensity(float2* rho_p) const {
+    const int n = mFluidParams.max_particles;
+    if (n <= 0 || !rho_p) return;
+    hip_check(hipDeviceSynchronize(), "downloadDensity sync");
+    hip_check(hipMemcpy(rho_p, d_mass_pressure, sizeof(float2) * n, hipMemcpyDeviceToHost), "downloadDensity");
+}
+
+void GPUSPHFluid::uploadDensity(const float2* rho_p) {
+    const int n = mFluidParams.max_particles;
+    if (n <= 0 || !rho_p) return;
+    hip_check(hipDeviceSynchronize(), "uploadDensity sync");
+    hip_check(hipMemcpy(d_mass_pressure, rho_p, sizeof(float2) * n, hipMemcpyHostToDevice), "uploadDensity");
+}
+
 void GPUSPHFluid::emitInflow(float dt) {
     if (d_emitStep)
         fg_emit_inflow(reinterpret_cast<fg_float4*>(d_position), reinterpret_cast<fg_float4*>(d_velocity), d_emitStep,

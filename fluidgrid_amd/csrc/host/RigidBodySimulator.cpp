@@ -466,6 +466,16 @@ void RigidBodySimulator::downloadBodies(float4* pos, float4* vel, float4* orient
     if (L) hip_check(hipMemcpy(L, d_L, sizeof(float4) * B, hipMemcpyDeviceToHost), "download");
 }
 
+void RigidBodySimulator::uploadBodies(const float4* pos, const float4* vel, const float4* orient, const float4* L) {
+    const size_t B = mMass.size();
+    if (!B) return;
+    hip_check(hipDeviceSynchronize(), "sync");
+    if (pos) hip_check(hipMemcpy(d_cm, pos, sizeof(float4) * B, hipMemcpyHostToDevice), "upload");
+    if (vel) hip_check(hipMemcpy(d_vel, vel, sizeof(float4) * B, hipMemcpyHostToDevice), "upload");
+    if (orient) hip_check(hipMemcpy(d_orient, orient, sizeof(float4) * B, hipMemcpyHostToDevice), "upload");
+    if (L) hip_check(hipMemcpy(d_L, L, sizeof(float4) * B, hipMemcpyHostToDevice), "upload");
+}
+
 void RigidBodySimulator::downloadSums(double* f4, double* t4) const {
     const size_t B = mMass.size();
     if (!B) return;

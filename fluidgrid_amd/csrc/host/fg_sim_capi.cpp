@@ -3,6 +3,7 @@
 // through it).  One process-wide simulator, as in the reference.
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "fluidgrid/HeterogeneousSimulator.h"
@@ -111,6 +112,36 @@ int fg_sim_step(int steps) {
     try {
         Simulators::HeterogeneousSimulator::instance()->doTimeSteps(steps);
         return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_save_state(const char* path) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    if (!path) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        Simulators::HeterogeneousSimulator::instance()->saveState(path);
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_sim_load_state(const char* path) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    if (!path) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        Simulators::HeterogeneousSimulator::instance()->loadState(path);
+        return FG_OK;
+    } catch (const std::runtime_error& e) {
+        // a file that does not fit the scene, or a damaged one
+        const std::string msg = e.what();
+        if (msg.rfind("loadState: ", 0) == 0) {
+            fg_record_error(FG_ERR_INVALID_ARGUMENT, msg.c_str());
+            return FG_ERR_INVALID_ARGUMENT;
+        }
+        return fail(e);
     } catch (const std::exception& e) {
         return fail(e);
     }

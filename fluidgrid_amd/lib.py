@@ -89,6 +89,8 @@ SIGNATURES = {
     "fg_conservation_workspace_size": (sz, []),
     "fg_conservation_f64": (i32, [vp, vp, vp, vp, u32, f32, f32, f32, vp, vp, sz]),
     "fg_sim_shutdown": (i32, []),
+    "fg_sim_save_state": (i32, [C.c_char_p]),
+    "fg_sim_load_state": (i32, [C.c_char_p]),
     "fg_sim_add_bodies": (i32, [vp, vp, u32, vp, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
     "fg_sim_add_static_spheres": (i32, [vp, u32, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
     "fg_sim_static_grid": (i32, [C.POINTER(abi.BodyGrid)]),
@@ -289,6 +291,14 @@ class Simulator:
 
     def set_overlap(self, on=True):
         check(load().fg_sim_set_overlap(int(on)), "fg_sim_set_overlap")
+
+    def save_state(self, path):
+        """Checkpoint of the whole simulator (fg_sim_save_state)."""
+        check(load().fg_sim_save_state(str(path).encode()), "fg_sim_save_state")
+
+    def load_state(self, path):
+        """Resume from a checkpoint of the same scene (fg_sim_load_state)."""
+        check(load().fg_sim_load_state(str(path).encode()), "fg_sim_load_state")
 
     def step(self, steps=1):
         check(load().fg_sim_step(int(steps)), "fg_sim_step")

@@ -96,6 +96,15 @@ public:
     void setInflow(const fg_inflow_desc& desc);
     bool hasInflow() const { return d_emitStep != nullptr; }
     void emitInflow(float dt);
+    const fg_inflow_desc& getInflow() const { return mInflow; }
+    // Checkpoint support: the inflow step counter (0 without an inflow) and
+    // the sorted (rho, p) buffer, whose entries at inactive particles carry
+    // over from step to step (the density pass leaves them as they are and
+    // neighbours read them).  Both synchronise the device.
+    unsigned int readEmitStep() const;
+    void writeEmitStep(unsigned int step);
+    void downloadDensity(float2* rho_p) const;
+    void uploadDensity(const float2* rho_p);
     void* getSortWorkspace() { return d_sortWorkspace; }
     unsigned long long* getNeighborMasks() { return d_neighborMasks; }
     size_t getSortWorkspaceBytes() const { return mSortWorkspaceBytes; }

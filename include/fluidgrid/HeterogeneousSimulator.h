@@ -14,6 +14,7 @@
 //    fluid's hash, sort and density (same dependencies, same results).
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "ApplicationData.h"
@@ -51,6 +52,19 @@ public:
     hipStream_t stream() const { return mStream; }
     const StageTimes& lastStageTimes() const { return mTimes; }
     void invalidateGraph();
+    // Checkpoint / resume (no counterpart in the reference).  saveState
+    // writes everything a step reads from the one before: per fluid the
+    // positions and velocities (particle order), the sorted (rho, p) buffer
+    // and the inflow step counter; per dynamic body the centre of mass,
+    // velocity, orientation and angular momentum.  The parameter blocks go
+    // along to check the scene.  loadState restores such a file into a
+    // simulator set up with the same scene (same parameter blocks, fluids,
+    // particle counts, inflows and bodies); continuing from it gives the
+    // same bits as not having stopped.  Both throw std::runtime_error on a
+    // mismatch, a damaged file (checksum) or an I/O error; loadState
+    // changes nothing unless the whole file checks.
+    void saveState(const std::string& path);
+    void loadState(const std::string& path);
 
 private:
     HeterogeneousSimulator();

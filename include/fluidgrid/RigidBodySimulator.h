@@ -104,6 +104,8 @@ public:
     // Extensions: state readback (positions, velocities, orientations,
     // angular momenta: float4 per body) and the FP64 sums of the last step.
     void downloadBodies(float4* pos, float4* vel, float4* orient, float4* L) const;
+    // The inverse: restores the per-body state (each array B float4s).
+    void uploadBodies(const float4* pos, const float4* vel, const float4* orient, const float4* L);
     void downloadSums(double* forces4, double* torques4) const;
     void downloadMassProperties(double* cm3, double* inertia9) const;
     void setStream(hipStream_t s) { mStream = s; }

@@ -87,6 +87,17 @@ int fg_sim_body_state(fg_float4* positions, fg_float4* velocities, fg_float4* or
 int fg_sim_body_sums(double* forces4, double* torques4);
 /* FP64 centre of mass (3 per body) and inertia tensor (9 per body). */
 int fg_sim_body_mass_properties(double* centreOfMass3, double* inertia9);
+
+/* Checkpoint / resume (HeterogeneousSimulator::saveState / loadState; the
+ * reference has none).  save writes the state a step reads from the step
+ * before (particles, the sorted density buffer, inflow step counters, body
+ * state) with the parameter blocks and a checksum.  load restores it into a
+ * simulator set up with the same scene; stepping on from there gives the
+ * same bits as an uninterrupted run.  load changes nothing and returns
+ * FG_ERR_INVALID_ARGUMENT if the file does not match the scene or fails its
+ * checksum.  Both synchronise the device. */
+int fg_sim_save_state(const char* path);
+int fg_sim_load_state(const char* path);
 int fg_sim_shutdown(void);
 
 #ifdef __cplusplus

@@ -118,3 +118,21 @@ def test_new_entry_points_reject_bad_arguments_before_any_launch():
         assert call() == bad, k
         assert L.fg_last_error() == bad, k
         assert L.fg_last_error() == 0
+
+
+def test_checkpoint_and_mesh_entry_points_check_arguments():
+    """Without a simulator, save/load report FG_ERR_NOT_INITIALIZED; the mesh
+    entry points reject null pointers and empty meshes; nothing touches a
+    device."""
+    L = lib.load()
+    L.fg_sim_shutdown()
+    not_init = 4  # FG_ERR_NOT_INITIALIZED
+    assert L.fg_sim_save_state(b"/nonexistent/x.fgs") == not_init
+    assert L.fg_sim_load_state(b"/nonexistent/x.fgs") == not_init
+    bad = 1
+    for call in (lambda: L.fg_mesh_winding_lattice(None, 0, None, 3, abi.Float3(0, 0, 0), 0.1, 2, 2, 2, None),
+                 lambda: L.fg_mesh_sample_spheres(None, 0, None, 0, 0.1, 0, None, 0, None),
+                 lambda: L.fg_mesh_lattice(None, 0, 0.1, None, None),
+                 lambda: L.fg_body_file_read(None, None, None, None, None, None, 0, None, 0)):
+        assert call() == bad
+        assert L.fg_last_error() == bad
