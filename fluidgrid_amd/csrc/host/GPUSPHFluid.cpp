@@ -1,6 +1,7 @@
 // GPUSPHFluid.cpp -- device buffers of one fluid (GPUSPHFluid.cpp:9-56).
 #include "fluidgrid/GPUSPHFluid.h"
 
+#include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
@@ -155,6 +156,42 @@ void GPUSPHFluid::uploadDensity(const float2* rho_p) {
     if (n <= 0 || !rho_p) return;
     hip_check(hipDeviceSynchronize(), "uploadDensity sync");
     hip_check(hipMemcpy(d_mass_pressure, rho_p, sizeof(float2) * n, hipMemcpyHostToDevice), "uploadDensity");
+}
+
+int GPUSPHFluid::writePly(const std::string& path) const {
+    const int n = mFluidParams.max_particles;
+    std::vector<float4> pos(n > 0 ? n : 0), vel(pos.size());
+    std::vector<float2> rp(pos.size());
+    std::vector<unsigned int> id(pos.size());
+    if (n > 0) {
+        hip_check(hipDeviceSynchronize(), "writePly sync");
+        hip_check(hipMemcpy(pos.data(), d_position, sizeof(float4) * n, hipMemcpyDeviceToHost), "writePly");
+        hip_check(hipMemcpy(vel.data(), d_velocity, sizeof(float4) * n, hipMemcpyDeviceToHost), "writePly");
+        hip_check(hipMemcpy(rp.data(), d_mass_pressure, sizeof(float2) * n, hipMemcpyDeviceToHost), "writePly");
+        hip_check(hipMemcpy(id.data(), d_particleId, sizeof(unsigned int) * n, hipMemcpyDeviceToHost), "writePly");
+    }
+    // (rho, p) are stored in sorted order: slot k belongs to particle id[k].
+    std::vector<float2> byParticle(pos.size(), make_float2(0.0f, 0.0f));
+    for (int k = 0; k < n; k++)
+        if (id[k] < (unsigned)n) byParticle[id[k]] = rp[k];
+    int count = 0;
+    for (int i = 0; i < n; i++) count += pos[i].w > 0.0f;
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("GPUSPHFluid::writePly: cannot open " + path);
+    std::fprintf(f,
+                 "ply\nformat binary_little_endian 1.0\ncomment fluidgrid_amd particles\nelement vertex %d\n"
+                 "property float x\nproperty float y\nproperty float z\nproperty float vx\nproperty float vy\n"
+                 "property float vz\nproperty float density\nproperty float pressure\nend_header\n",
+                 count);
+    bool ok = true;
+    for (int i = 0; i < n && ok; i++) {
+        if (!(pos[i].w > 0.0f)) continue;
+        const float rec[8] = {pos[i].x, pos[i].y, pos[i].z, vel[i].x, vel[i].y, vel[i].z, byParticle[i].x,
+                              byParticle[i].y};
+        ok = std::fwrite(rec, sizeof(rec), 1, f) == 1;
+    }
+    if (std::fclose(f) != 0 || !ok) throw std::runtime_error("GPUSPHFluid::writePly: write failed for " + path);
+    return count;
 }
 
 void GPUSPHFluid::emitInflow(float dt) {

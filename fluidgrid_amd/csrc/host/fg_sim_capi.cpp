@@ -147,6 +147,18 @@ int fg_sim_load_state(const char* path) {
     }
 }
 
+int fg_sim_write_ply(int idx, const char* path, unsigned int* count) {
+    Fluid::GPUSPHFluid* f = fluid_at(idx);
+    if (!f || !path) return FG_ERR_INVALID_ARGUMENT;
+    try {
+        const int n = f->writePly(path);
+        if (count) *count = (unsigned)n;
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 int fg_sim_synchronize(void) {
     if (!sim()) return FG_ERR_NOT_INITIALIZED;
     hipError_t e = hipStreamSynchronize(Simulators::HeterogeneousSimulator::instance()->stream());

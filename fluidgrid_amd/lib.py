@@ -91,6 +91,7 @@ SIGNATURES = {
     "fg_sim_shutdown": (i32, []),
     "fg_sim_save_state": (i32, [C.c_char_p]),
     "fg_sim_load_state": (i32, [C.c_char_p]),
+    "fg_sim_write_ply": (i32, [i32, C.c_char_p, C.POINTER(C.c_uint)]),
     "fg_sim_add_bodies": (i32, [vp, vp, u32, vp, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
     "fg_sim_add_static_spheres": (i32, [vp, u32, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
     "fg_sim_static_grid": (i32, [C.POINTER(abi.BodyGrid)]),
@@ -295,6 +296,12 @@ class Simulator:
     def save_state(self, path):
         """Checkpoint of the whole simulator (fg_sim_save_state)."""
         check(load().fg_sim_save_state(str(path).encode()), "fg_sim_save_state")
+
+    def write_ply(self, path):
+        """Active particles as a binary PLY point cloud; returns the count."""
+        n = C.c_uint(0)
+        check(load().fg_sim_write_ply(self.idx, str(path).encode(), C.byref(n)), "fg_sim_write_ply")
+        return n.value
 
     def load_state(self, path):
         """Resume from a checkpoint of the same scene (fg_sim_load_state)."""

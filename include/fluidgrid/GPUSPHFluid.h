@@ -4,6 +4,8 @@
 // instead of a mapped VBO (GPUSPHFluid.cpp:17-41, GLCudaBuffer.h:32-78).
 #pragma once
 
+#include <string>
+
 #include <vector>
 
 #include "ApplicationData.h"
@@ -105,6 +107,12 @@ public:
     void writeEmitStep(unsigned int step);
     void downloadDensity(float2* rho_p) const;
     void uploadDensity(const float2* rho_p);
+    // Point cloud for visualisation (the reference draws the positions
+    // from a GL buffer, GPUSPHFluid.cpp:66-89): a binary little-endian PLY
+    // file with one vertex per active particle (w > 0), properties x y z
+    // (particle order), vx vy vz, and density and pressure of the last step
+    // (looked up through the sorted particle ids).  Returns the vertex count.
+    int writePly(const std::string& path) const;
     void* getSortWorkspace() { return d_sortWorkspace; }
     unsigned long long* getNeighborMasks() { return d_neighborMasks; }
     size_t getSortWorkspaceBytes() const { return mSortWorkspaceBytes; }

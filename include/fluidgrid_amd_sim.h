@@ -98,6 +98,11 @@ int fg_sim_body_mass_properties(double* centreOfMass3, double* inertia9);
  * checksum.  Both synchronise the device. */
 int fg_sim_save_state(const char* path);
 int fg_sim_load_state(const char* path);
+/* Point cloud of fluid idx for visualisation (GPUSPHFluid::writePly): a
+ * binary little-endian PLY file of the active particles with position,
+ * velocity, density and pressure.  *count (may be NULL) receives the number
+ * of vertices written. */
+int fg_sim_write_ply(int idx, const char* path, unsigned int* count);
 int fg_sim_shutdown(void);
 
 #ifdef __cplusplus

@@ -1,4 +1,5 @@
-"""Checkpoint / resume through the C-ABI (fg_sim_save_state, fg_sim_load_state).
+"""Checkpoint / resume and point-cloud output through the C-ABI
+(fg_sim_save_state, fg_sim_load_state, fg_sim_write_ply).
 
   resume            a run stopped after k steps, saved, restored into a new
                     simulator of the same scene and stepped on gives the same
@@ -10,6 +11,9 @@
   refusals          a file of another scene, a damaged byte (checksum) and a
                     truncated file are refused with FG_ERR_INVALID_ARGUMENT
                     and leave the simulator's state as it was
+  PLY output        the point cloud holds exactly the active particles in
+                    particle order, their velocities and the (rho, p) of the
+                    last step looked up through the sorted ids (bit-exact)
 """
 import numpy as np
 import pytest
@@ -138,3 +142,40 @@ def test_refusals_leave_state_unchanged(fglib, tmp_path):
         _same(before, _snapshot(sim))
     finally:
         sim.close()
+
+
+def _read_ply(path):
+    data = open(path, "rb").read()
+    end = data.index(b"end_header\n") + len(b"end_header\n")
+    head = data[:end].decode().splitlines()
+    assert head[0] == "ply" and head[1] == "format binary_little_endian 1.0"
+    n = int(next(h for h in head if h.startswith("element vertex")).split()[-1])
+    props = [h.split()[-1] for h in head if h.startswith("property float")]
+    rec = np.frombuffer(data[end:], np.float32).reshape(n, len(props))
+    return props, rec
+
+
+def test_ply_point_cloud(fglib, tmp_path):
+    """fg_sim_write_ply: the active particles in particle order with their
+    velocities and the density/pressure of the last step."""
+    s, b = _scene()
+    sim = _sim(fglib, s, b)
+    try:
+        sim.step(8)
+        p = tmp_path / "cloud.ply"
+        n = sim.write_ply(p)
+        pos, vel = sim.state()
+        bufs = sim.buffers
+        ids = fglib.download_raw(bufs.particleId, (s.n,), np.uint32)
+        rp = fglib.download_raw(bufs.densityPressure, (s.n, 2), np.float32)
+    finally:
+        sim.close()
+    props, rec = _read_ply(p)
+    assert props == ["x", "y", "z", "vx", "vy", "vz", "density", "pressure"]
+    act = pos[:, 3] > 0
+    assert n == act.sum() == len(rec) and n < s.n  # reserve slots are left out
+    by_particle = np.zeros((s.n, 2), np.float32)
+    by_particle[ids] = rp
+    assert np.array_equal(rec[:, :3], pos[act, :3]) and np.array_equal(rec[:, 3:6], vel[act, :3])
+    assert np.array_equal(rec[:, 6:], by_particle[act])
+    assert np.all(rec[:, 6] > 0)
