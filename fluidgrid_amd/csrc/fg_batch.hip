@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(BS)
                    const unsigned long long* __restrict__ masks, float4* __restrict__ vout,
                    float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n) {
     __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
-    __shared__ unsigned long long s_mm[FLAT ? 9 * BS : 1];
+    __shared__ uint32_t s_mm[FLAT ? 9 * BS : 1];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
     const Contacts none{};

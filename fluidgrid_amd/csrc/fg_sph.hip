@@ -220,15 +220,16 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8)))
 }
 
 // Flat schedule (row table in LDS); same results as k_forces_fast<true, COUPLED>.
+// The occupancy hint is the one of k_forces_fast, for the same reason.
 template <bool COUPLED, int BS>
-__global__ void __launch_bounds__(BS)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8)))
     k_forces_flat(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, float4* __restrict__ vout,
                   float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
                   fg_common_fluid_params cp, fg_kernels_cache kc, const unsigned long long* __restrict__ masks,
                   Contacts bc) {
     __shared__ uint32_t s_rb[9 * BS];
-    __shared__ unsigned long long s_mm[9 * BS];
+    __shared__ uint32_t s_mm[9 * BS];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
     FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};

@@ -103,21 +103,20 @@ __device__ __forceinline__ bool row_range(const RowWalker& w, int dy, int dz, fl
     const float gz = dz < 0 ? w.gz_lo : (dz > 0 ? w.gz_hi : 0.0f);
     const float gyz = gy + gz;
     if (gyz >= cut) return false;
-    int x0 = w.cx - 1, x1 = w.cx + 1;
-    if (x0 < 0 || gyz + w.gx_lo >= cut) x0 = w.cx;
-    if (x1 >= cp.bucket_size.x || gyz + w.gx_hi >= cut) x1 = w.cx;
-    const uint32_t kb = key_base + linear_key(0, cy, cz, cp);
-    uint32_t b = kEmpty, e = 0;
-    for (int x = x0; x <= x1; x++) {
-        const uint32_t s0 = start[kb + (uint32_t)x];
-        if (s0 != kEmpty) {
-            if (b == kEmpty) b = s0;
-            e = end[kb + (uint32_t)x];
-        }
-    }
-    if (b == kEmpty) return false;
-    rb = b;
-    re = min(e, n);
+    const bool useL = w.cx > 0 && gyz + w.gx_lo < cut;                      // cell cx-1
+    const bool useR = w.cx + 1 < cp.bucket_size.x && gyz + w.gx_hi < cut;  // cell cx+1
+    const uint32_t kb = key_base + linear_key(0, cy, cz, cp) + (uint32_t)w.cx;
+    // All six loads at once (one memory round trip); a cell outside the row
+    // reads its neighbour instead and is not used.  end[] of an empty cell
+    // is stale and is never used either.
+    const uint32_t kl = useL ? kb - 1u : kb, kr = useR ? kb + 1u : kb;
+    const uint32_t sL = start[kl], sC = start[kb], sR = start[kr];
+    const uint32_t eL = end[kl], eC = end[kb], eR = end[kr];
+    // first non-empty start and last non-empty end of the row's cells
+    const bool nL = useL && sL != kEmpty, nC = sC != kEmpty, nR = useR && sR != kEmpty;
+    if (!(nL || nC || nR)) return false;
+    rb = nL ? sL : (nC ? sC : sR);
+    re = min(nR ? eR : (nC ? eC : eL), n);
     return rb < re;
 }
 
@@ -361,13 +360,16 @@ __device__ __forceinline__ void pair_term_rp(PairAcc& A, uint32_t i, uint32_t j,
 
 // Per-lane row table of the flat force schedule, in LDS: for each of the 9
 // stencil rows, the sorted index of its first particle and its neighbour
-// mask (0 for a culled row).
+// mask (0 for a culled row).  The flat schedule takes rows of at most 32
+// candidates, so a mask is one 32-bit word: 72 bytes per lane, which keeps
+// eight 256-lane workgroups' tables within a CU's LDS.
+constexpr uint32_t kFlatRowBits = 32;
 struct FlatRows {
-    uint32_t* rb;              // [9][lanes]
-    unsigned long long* mm;    // [9][lanes]
+    uint32_t* rb;  // [9][lanes]
+    uint32_t* mm;  // [9][lanes]
     uint32_t lanes, lane;
     __device__ __forceinline__ uint32_t& b(int r) { return rb[r * lanes + lane]; }
-    __device__ __forceinline__ unsigned long long& m(int r) { return mm[r * lanes + lane]; }
+    __device__ __forceinline__ uint32_t& m(int r) { return mm[r * lanes + lane]; }
 };
 
 // Pressure + viscosity + walls (+ body contact) of sorted particle i (i < n),
@@ -378,10 +380,11 @@ struct FlatRows {
 //  * row loop: the 9 rows in turn; a wavefront waits at every row for its
 //    lane with the most neighbours in that row;
 //  * flat (MASKS with a row table): the row masks are gathered first, then
-//    each lane walks its own neighbour stream across rows, one neighbour per
-//    iteration, so a wavefront runs for its lane with the most neighbours
-//    overall.  A particle with a row longer than the mask width takes the
-//    row loop.
+//    each lane walks its own neighbour stream across rows, two neighbours
+//    per iteration with their loads issued together, so a wavefront runs
+//    for its lane with the most neighbours overall rather than, row by row,
+//    for the lane with the most neighbours in that row.  A particle with a
+//    row longer than kFlatRowBits takes the row loop.
 template <bool MASKS, bool COUPLED = false>
 __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __restrict__ sp,
                                                 const float4* __restrict__ sv, const float2* __restrict__ rho_p,
@@ -410,12 +413,13 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
             for (int row = 0; row < 9; row++) {
                 const int dz = row / 3 - 1, dy = row % 3 - 1;
                 uint32_t rb = 0, re = 0;
-                unsigned long long mm = 0ull;
+                uint32_t mm = 0u;
                 if (row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) {
-                    if (re - rb <= kRowMaskBits) {
-                        mm = masks[(size_t)row * n + i];
+                    if (re - rb <= kFlatRowBits) {
+                        unsigned long long m64 = masks[(size_t)row * n + i];
                         // the particle itself is in its own row's mask; it adds nothing
-                        if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
+                        if (i >= rb && i < re) m64 &= ~(1ull << (i - rb));
+                        mm = (uint32_t)m64;
                     } else {
                         fits = false;
                     }
@@ -425,49 +429,80 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
             }
             if (fits) {
                 rowloop = false;
-                int row = -1;
-                uint32_t rb = 0;
-                unsigned long long mm = 0ull;
-                for (;;) {
-                    while (mm == 0ull && row < 8) {
+                int row = 0;
+                uint32_t cm = flat->m(0), cb = flat->b(0);
+                // The next neighbour of the stream (rows in order, ascending
+                // within a row), or kEmpty at its end.
+                auto next = [&]() -> uint32_t {
+                    while (cm == 0u && row < 8) {
                         row++;
-                        mm = flat->m(row);
-                        rb = flat->b(row);
+                        cm = flat->m(row);
+                        cb = flat->b(row);
                     }
-                    if (mm == 0ull) break;
-                    const uint32_t k = (uint32_t)__builtin_ctzll(mm);
-                    mm &= mm - 1ull;
-                    const uint32_t j = rb + k;
-                    const float2 ab = inv[j];
-                    pair_term_masked(A, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), ab.x, ab.y, h, s);
+                    if (cm == 0u) return kEmpty;
+                    const uint32_t k = (uint32_t)__builtin_ctz(cm);
+                    cm &= cm - 1u;
+                    return cb + k;
+                };
+                for (;;) {
+                    const uint32_t j1 = next();
+                    if (j1 == kEmpty) break;
+                    const uint32_t jn = next();
+                    const bool two = jn != kEmpty;
+                    const uint32_t j2 = two ? jn : j1;
+                    const float3 q1 = ld_xyz(sp + j1), q2 = ld_xyz(sp + j2);
+                    const float3 v1 = ld_xyz(sv + j1), v2 = ld_xyz(sv + j2);
+                    const float2 d1 = inv[j1], d2 = inv[j2];
+                    asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(v2.x), "v"(v2.y), "v"(v2.z), "v"(d2.x),
+                                 "v"(d2.y));
+                    pair_term_masked(A, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s);
+                    if (two) pair_term_masked(A, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s);
+                    if (!two) break;
                 }
             }
         }
         for (int row = 0; rowloop && row < 9; row++) {
             const int dz = row / 3 - 1, dy = row % 3 - 1;
+            // The row's mask is read before its range is known, so the two
+            // reads share one memory round trip (the asm keeps the compiler
+            // from sinking the load behind the range test).
+            unsigned long long mrow = 0ull;
+            if (MASKS) {
+                mrow = masks[(size_t)row * n + i];
+                asm volatile("" ::"v"(mrow));
+            }
             uint32_t rb, re;
             if (!row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) continue;
             if (MASKS && re - rb <= kRowMaskBits) {
-                unsigned long long mm = masks[(size_t)row * n + i];
+                unsigned long long mm = mrow;
                 // the particle itself is in its own row's mask; it adds nothing
                 if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
-                while (mm) {
-                    const uint32_t k1 = (uint32_t)__builtin_ctzll(mm);
-                    mm &= mm - 1ull;
-                    const bool two = mm != 0ull;
-                    const uint32_t k2 = two ? (uint32_t)__builtin_ctzll(mm) : k1;
-                    if (two) mm &= mm - 1ull;
-                    const uint32_t j1 = rb + k1, j2 = rb + k2;
-                    const float3 q1 = ld_xyz(sp + j1), q2 = ld_xyz(sp + j2);
-                    const float3 v1 = ld_xyz(sv + j1), v2 = ld_xyz(sv + j2);
-                    const float2 d1 = inv[j1], d2 = inv[j2];
-                    // Keeps the second pair's loads beside the first pair's (one
-                    // memory round trip per iteration): without a use here the
-                    // compiler sinks them into the `two` branch below.
-                    asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(v2.x), "v"(v2.y), "v"(v2.z), "v"(d2.x),
-                                 "v"(d2.y));
-                    pair_term_masked(A, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s);
-                    if (two) pair_term_masked(A, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s);
+                // The mask as two 32-bit words, low word first (the same
+                // ascending order): 32-bit bit scans and clears, and rows of
+                // at most 32 candidates never enter the second loop.
+                uint32_t word = (uint32_t)mm, base = rb;
+                for (int half = 0; half < 2; half++) {
+                    while (word) {
+                        const uint32_t k1 = (uint32_t)__builtin_ctz(word);
+                        word &= word - 1u;
+                        const bool two = word != 0u;
+                        const uint32_t k2 = two ? (uint32_t)__builtin_ctz(word) : k1;
+                        word &= word - 1u;  // (0 stays 0)
+                        const uint32_t j1 = base + k1, j2 = base + k2;
+                        const float3 q1 = ld_xyz(sp + j1), q2 = ld_xyz(sp + j2);
+                        const float3 v1 = ld_xyz(sv + j1), v2 = ld_xyz(sv + j2);
+                        const float2 d1 = inv[j1], d2 = inv[j2];
+                        // Keeps the second pair's loads beside the first pair's (one
+                        // memory round trip per iteration): without a use here the
+                        // compiler sinks them into the `two` branch below.
+                        asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(v2.x), "v"(v2.y), "v"(v2.z),
+                                     "v"(d2.x), "v"(d2.y));
+                        pair_term_masked(A, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s);
+                        if (two) pair_term_masked(A, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s);
+                    }
+                    word = (uint32_t)(mm >> 32);  // then the high word
+                    base = rb + 32u;
+                    if (!word) break;
                 }
             } else {
                 for (uint32_t j = rb; j < re; j++)
