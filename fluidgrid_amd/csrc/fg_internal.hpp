@@ -139,6 +139,17 @@ __device__ __forceinline__ int cell_coord(float p, float vmin, float cs, int n) 
 
 struct CellXYZ { int x, y, z; };
 
+// Stencil reach along one axis: the number of cells on each side of a
+// particle's own cell that can hold a neighbour within h, ceil(h / cs) (h in
+// world units, kernel_radius / simulation_scale), at least 1.  The 27-cell
+// stencil of the density and force passes (the reference's, and the rows of
+// the fast path) needs 1: cells at least h on every axis.
+__host__ __device__ inline int linear_reach(float kernel_radius, float simulation_scale, float cell) {
+    const float q = (kernel_radius / simulation_scale) / cell;
+    const int r = (int)ceilf(q);
+    return r < 1 ? 1 : r;
+}
+
 __device__ __forceinline__ CellXYZ grid_cell(float px, float py, float pz, const GridDesc& g) {
     return {cell_coord(px, g.vmin.x, g.cs.x, g.dims.x), cell_coord(py, g.vmin.y, g.cs.y, g.dims.y),
             cell_coord(pz, g.vmin.z, g.cs.z, g.dims.z)};

@@ -249,6 +249,22 @@ static bool have_params(const char* who) {
     return true;
 }
 
+// The density and force passes walk the 27 cells around a particle's own,
+// which holds every neighbour only when the cells are at least h on every
+// axis; smaller cells are refused rather than summed short.
+static bool stencil_covers_h(const char* who) {
+    const fg_common_fluid_params& cp = state().common;
+    const float s = cp.simulation_scale;
+    if (linear_reach(cp.kernel_radius, s, cp.cell_size.x) > 1 ||
+        linear_reach(cp.kernel_radius, s, cp.cell_size.y) > 1 ||
+        linear_reach(cp.kernel_radius, s, cp.cell_size.z) > 1) {
+        set_error(FG_ERR_UNSUPPORTED, "%s: cell_size is smaller than the kernel radius (the 27-cell stencil "
+                  "would miss neighbours)", who);
+        return false;
+    }
+    return true;
+}
+
 static bool use_fast() { return state().key_mode == FG_KEY_LINEAR && state().math_mode == FG_MATH_FAST; }
 
 #define FG_SPH_LAUNCH_BS(KERN, BS, n, ...) \
@@ -277,6 +293,7 @@ static bool use_fast() { return state().key_mode == FG_KEY_LINEAR && state().mat
 
 int launch_density(float4* sp, float2* rho_p, uint32_t* start, uint32_t* end, uint32_t n, float k, float rho0,
                    float m, unsigned long long* masks = nullptr) {
+    if (!stencil_covers_h("density pass")) return FG_ERR_UNSUPPORTED;
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
     if (use_fast() && masks && st.flat_density)
@@ -298,6 +315,7 @@ int launch_density(float4* sp, float2* rho_p, uint32_t* start, uint32_t* end, ui
 int launch_forces(float4* sp, float4* sv, float2* rho_p, uint32_t* start, uint32_t* end, float4* vout,
                   float4* pout, uint32_t* id, uint32_t n, float m, float mu, const unsigned long long* masks = nullptr,
                   const Contacts* bcp = nullptr) {
+    if (!stencil_covers_h("force pass")) return FG_ERR_UNSUPPORTED;
     Contacts bc{};
     if (bcp) bc = *bcp;
     const bool coupled = bc.coupled != 0;

@@ -106,17 +106,28 @@ __device__ __forceinline__ bool row_range(const RowWalker& w, int dy, int dz, fl
     const bool useL = w.cx > 0 && gyz + w.gx_lo < cut;                      // cell cx-1
     const bool useR = w.cx + 1 < cp.bucket_size.x && gyz + w.gx_hi < cut;  // cell cx+1
     const uint32_t kb = key_base + linear_key(0, cy, cz, cp) + (uint32_t)w.cx;
-    // All six loads at once (one memory round trip); a cell outside the row
-    // reads its neighbour instead and is not used.  end[] of an empty cell
-    // is stale and is never used either.
-    const uint32_t kl = useL ? kb - 1u : kb, kr = useR ? kb + 1u : kb;
-    const uint32_t sL = start[kl], sC = start[kb], sR = start[kr];
-    const uint32_t eL = end[kl], eC = end[kb], eR = end[kr];
-    // first non-empty start and last non-empty end of the row's cells
-    const bool nL = useL && sL != kEmpty, nC = sC != kEmpty, nR = useR && sR != kEmpty;
-    if (!(nL || nC || nR)) return false;
-    rb = nL ? sL : (nC ? sC : sR);
-    re = min(nR ? eR : (nC ? eC : eL), n);
+    const uint32_t k0 = useL ? kb - 1u : kb, k1 = useR ? kb + 1u : kb;
+    // The row's particles are sorted[start[k0], end[k1]) when its end cells
+    // are occupied, as in the bulk of the fluid: three loads in one round
+    // trip.  Otherwise the first occupied cell from the left and the last
+    // from the right bound it (end[] of an empty cell is stale and unused).
+    const uint32_t s0 = start[k0], s1 = start[k1], e1 = end[k1];
+    if (s0 != kEmpty && s1 != kEmpty) {
+        rb = s0;
+        re = min(e1, n);
+        return rb < re;
+    }
+    uint32_t b = kEmpty, e = 0;
+    for (uint32_t k = k0; k <= k1; k++) {
+        const uint32_t sk = start[k];
+        if (sk != kEmpty) {
+            if (b == kEmpty) b = sk;
+            e = end[k];
+        }
+    }
+    if (b == kEmpty) return false;
+    rb = b;
+    re = min(e, n);
     return rb < re;
 }
 

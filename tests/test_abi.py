@@ -136,3 +136,27 @@ def test_checkpoint_and_mesh_entry_points_check_arguments():
                  lambda: L.fg_body_file_read(None, None, None, None, None, None, 0, None, 0)):
         assert call() == bad
         assert L.fg_last_error() == bad
+
+
+def test_cells_smaller_than_h_are_refused():
+    """The density and force passes walk 27 cells: with a cell edge below
+    the kernel radius they record FG_ERR_UNSUPPORTED before any launch."""
+    import ctypes as C
+    from fluidgrid_amd import scenes
+    L = lib.load()
+    s = scenes.small_dam_break(lattice=(4, 4, 4), h=0.04)
+    cp = abi.CommonFluidParams.from_buffer_copy(s.common)
+    cp.cell_size = abi.f3(0.02, 0.04, 0.04)  # h/2 along x
+    L.gInitializeCommonFluidParams(cp)
+    L.gInitializeKernelParams(s.kernels)
+    p = C.c_void_p(256)
+    try:
+        rc = L.fg_sph_density_nbr(p, p, p, p, p, 64, 3.0, 1000.0, 1.0)
+        assert rc == 5  # FG_ERR_UNSUPPORTED
+        assert b"smaller than the kernel radius" in L.fg_last_error_string()
+        assert L.fg_last_error() == 5
+        rc = L.fg_sph_forces_integrate_nbr(p, p, p, p, p, p, p, p, p, 64, 1.0, 1.0)
+        assert rc == 5
+        assert L.fg_last_error() == 5
+    finally:
+        L.gInitializeCommonFluidParams(s.common)
