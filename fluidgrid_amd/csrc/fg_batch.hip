@@ -6,6 +6,7 @@
 // scenes.  Each lane handles one particle; a wavefront whose particles belong
 // to several scenes runs once per distinct scene (readfirstlane loop), so the
 // scene's parameters are wave-uniform and read through the scalar cache.
+#include "fg_batch_bodies.hpp"
 #include "fg_internal.hpp"
 #include "fg_sph_fast.hpp"
 
@@ -32,6 +33,7 @@ __device__ __forceinline__ GridDesc scene_grid(const fg_common_fluid_params& cp)
     g.dims = make_int3(cp.bucket_size.x, cp.bucket_size.y, cp.bucket_size.z);
     g.nb = dev_num_buckets(cp);
     g.mode = FG_KEY_LINEAR;
+    g.key_base = 0;
     return g;
 }
 
@@ -91,6 +93,32 @@ __global__ void __launch_bounds__(BS)
     });
 }
 
+// The same with each scene's bodies: scenes with bodies take the coupled
+// force expression and contact against their body grid.
+template <int BS, bool FLAT>
+__global__ void __launch_bounds__(BS)
+    k_batch_forces_coupled(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes,
+                           const uint32_t* __restrict__ scene_of, const float4* __restrict__ sp,
+                           const float4* __restrict__ sv, const float2* __restrict__ rho_p,
+                           const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                           const unsigned long long* __restrict__ masks, float4* __restrict__ vout,
+                           float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n,
+                           const BatchBodyScene* __restrict__ bodies) {
+    __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
+    __shared__ uint32_t s_mm[FLAT ? 9 * BS : 1];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (i >= n) return;
+    FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
+    for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
+        const fg_scene_desc& d = scenes[s];
+        Contacts c{};
+        c.dyn = bodies[s].dyn;
+        c.coupled = c.dyn.enabled;
+        forces_particle<true, true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, d.mass, d.viscosity, d.common,
+                                    d.kernels, d.keyBase, masks, c, FLAT ? &fr : nullptr);
+    });
+}
+
 #define FG_BATCH_LAUNCH_BS(KERN, BS, n, ...) \
     hipLaunchKernelGGL(KERN, dim3(grid_for(n, BS)), dim3(BS), 0, state().stream, __VA_ARGS__)
 
@@ -104,6 +132,24 @@ __global__ void __launch_bounds__(BS)
             default: FG_BATCH_LAUNCH_BS((KERN<256, FLAT>), 256, n, __VA_ARGS__);                           \
         }                                                                                                   \
     } while (0)
+
+int batch_forces_coupled(const fg_scene_desc* scenes, uint32_t nscenes, const uint32_t* scene_of, float4* sp,
+                         float4* sv, float2* rho_p, uint32_t* start, uint32_t* end, const unsigned long long* masks,
+                         float4* vel, float4* pos, uint32_t* id, uint32_t n, const BatchBodyScene* bodies) {
+    if (n == 0) return FG_OK;
+    if (!scenes || !scene_of || !sp || !sv || !rho_p || !start || !end || !masks || !vel || !id || !bodies ||
+        nscenes == 0 || (void*)vel == (void*)sv) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "batch_forces_coupled: null or aliased pointer, or no scenes");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    if (state().flat_forces)
+        FG_BATCH_LAUNCH2(k_batch_forces_coupled, true, n, scenes, nscenes, scene_of, sp, sv, rho_p, start, end, masks,
+                         vel, pos, id, n, bodies);
+    else
+        FG_BATCH_LAUNCH2(k_batch_forces_coupled, false, n, scenes, nscenes, scene_of, sp, sv, rho_p, start, end,
+                         masks, vel, pos, id, n, bodies);
+    return check_launch("batch_forces_coupled") ? FG_OK : fg_peek_error();
+}
 
 }  // namespace fg
 

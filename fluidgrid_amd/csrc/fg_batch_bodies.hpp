@@ -1,0 +1,35 @@
+// fg_batch_bodies.hpp -- launchers of the rigid-body stages of the batched
+// step (library-internal: called by host/BatchedSimulator.cpp).  Every
+// per-element computation is the single-fluid one (fg_rigid.hip,
+// fg_sph_fast.hpp) with the parameters of the element's scene, so each
+// scene's bodies and fluid advance exactly as in a single-fluid simulator.
+#pragma once
+#include "fg_internal.hpp"
+
+namespace fg {
+
+// World positions and body-grid keys of all body spheres (gCalcHashKeyWithOffset per scene).
+int batch_body_transform(uint32_t* key, uint32_t* id, const float4* cm, const float4* orient, const float4* rel,
+                         float4* world, uint32_t m, const uint32_t* body_scene, const BatchBodyScene* scenes);
+// Per scene: bounds of its sphere centres grown by bounds_margin (fg_sphere_bounds), into out[2s], out[2s+1].
+int batch_scene_bounds(const float4* world, const BatchBodyScene* scenes, uint32_t nscenes, float4* out);
+// Per body: bounds grown by its scene's pair_margin (fg_body_bounds).
+int batch_body_bounds(const float4* world, const uint32_t* offsets, const uint32_t* body_scene,
+                      const BatchBodyScene* scenes, uint32_t nbodies, float4* out);
+// Per body sphere: penalty force and torque from its scene's fluid (fg_body_forces_from_fluid), then, in
+// scenes with two or more bodies, from the other bodies of the scene added on (fg_body_forces_from_bodies).
+int batch_body_forces(const float4* world, const float4* vel, const float4* cm, float4* out_f, float4* out_t,
+                      uint32_t m, uint32_t nbodies, const uint32_t* body_scene, const BatchBodyScene* scenes,
+                      const float4* fluid_pos, const float4* fluid_vel, const uint32_t* fluid_start,
+                      const uint32_t* fluid_end, uint32_t nfluid, const float4* body_bounds);
+// Per body: IntegrateRBK with its scene's dt, gravity, box and velocity limit (gIntegrateRB).
+int batch_integrate_bodies(float4* cm, const float* mass, float4* vel, const float4* force, uint32_t nbodies,
+                           const uint32_t* body_scene, const BatchBodyScene* scenes, float4* L, const float4* torque,
+                           const float4* inv_inertia, float4* orient);
+// fg_batch_forces_integrate_nbr with each scene's bodies (fg_sph_forces_coupled per scene; scenes
+// without bodies take the uncoupled expression).
+int batch_forces_coupled(const fg_scene_desc* scenes, uint32_t nscenes, const uint32_t* scene_of, float4* sp,
+                         float4* sv, float2* rho_p, uint32_t* start, uint32_t* end, const unsigned long long* masks,
+                         float4* vel, float4* pos, uint32_t* id, uint32_t n, const BatchBodyScene* bodies);
+
+}  // namespace fg

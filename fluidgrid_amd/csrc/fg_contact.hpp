@@ -46,11 +46,11 @@ __device__ __forceinline__ uint32_t contact_key(const float4& sp, const CellXYZ&
                                                 const GridDesc& g) {
     if (g.mode == FG_KEY_REF_XOR) {
         const float ox = g.cs.x * (float)x, oy = g.cs.y * (float)y, oz = g.cs.z * (float)z;
-        return hash_ref_xor(sp.x + ox, sp.y + oy, sp.z + oz, g.cs.x, g.cs.y, g.cs.z, g.nb);
+        return g.key_base + hash_ref_xor(sp.x + ox, sp.y + oy, sp.z + oz, g.cs.x, g.cs.y, g.cs.z, g.nb);
     }
     const int a = c.x + x, b = c.y + y, d = c.z + z;
     if (a < 0 || b < 0 || d < 0 || a >= g.dims.x || b >= g.dims.y || d >= g.dims.z) return kEmpty;
-    return grid_linear_key(a, b, d, g);
+    return g.key_base + grid_linear_key(a, b, d, g);
 }
 
 // ProcessCollisionForcesD, K_DeviceFunctions.cu:113-159: the (2r+1)^3

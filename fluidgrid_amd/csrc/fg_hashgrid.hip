@@ -15,6 +15,7 @@ GridDesc fluid_grid_desc(const fg_common_fluid_params& cp, fg_float3 cs, uint32_
     g.dims = make_int3(cp.bucket_size.x, cp.bucket_size.y, cp.bucket_size.z);
     g.nb = nb;
     g.mode = mode;
+    g.key_base = 0;
     return g;
 }
 
@@ -56,8 +57,7 @@ __global__ void __launch_bounds__(256) k_hash_with_offset(uint32_t* __restrict__
     if (i >= n) return;
     const float4 p = rel[i];
     const int b = f2i_sat(p.w);
-    float4 a = quat_add(quat_transform_point(p, orient[b]), offset[b]);
-    a.w = p.w;
+    const float4 a = world_sphere(p, orient[b], offset[b]);
     id[i] = i;
     out[i] = a;
     key[i] = grid_key(a.x, a.y, a.z, g);

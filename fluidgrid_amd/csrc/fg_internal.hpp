@@ -55,6 +55,7 @@ struct GridDesc {
     int3 dims;         // LINEAR cells per axis
     uint32_t nb;       // number of buckets (key range)
     int mode;          // FG_KEY_REF_XOR or FG_KEY_LINEAR
+    uint32_t key_base; // first bucket of this grid in a shared key space (batched scenes); 0 otherwise
 };
 
 // Penalty contact of fluid particles against a body-sphere grid
@@ -79,6 +80,30 @@ struct BodyContact {
 struct Contacts {
     BodyContact st, dyn;
     int coupled;
+};
+
+// Rigid bodies of one scene of a batch.  All scenes' bodies share global
+// arrays (bodies scene-major, spheres in body order, w = global body index);
+// each scene's body grid occupies its own bucket range (grid.key_base) of
+// one shared body key space, as the scenes' fluid grids do, so one stable
+// sort and one cell-range pass serve every scene and every walk stays in
+// its scene.  Each field is what the single-fluid RigidBodySimulator
+// derives from that scene's parameter blocks.
+struct BatchBodyScene {
+    BodyContact dyn;        // the body grid as the scene's fluid sees it (aabb: the scene's sphere bounds)
+    GridDesc fluid;         // the scene's fluid grid (LINEAR, key_base = the scene's fluid key base)
+    int3 rb_to_fluid;       // stencil of the body spheres in the fluid grid
+    float sphere_radius;    // body sphere radius
+    float fluid_radius;     // 2 * pointRadius of the scene's fluid
+    float ks, kd;           // -springForce, -dampingForce
+    float bounds_margin;    // (2 pointRadius + r + 0.003) * 1.01: sphere bounds of the fluid-side cull
+    float pair_margin;      // (2 r + 0.001) * 1.01: per-body bounds of the body-body broad phase
+    uint32_t body_begin, body_end;      // global body range
+    uint32_t sphere_begin, sphere_end;  // global sphere range
+    // body integration (IntegrateRBK parameters of the scene)
+    float dt, vlimit;
+    float4 ext;
+    float3 vmin, vmax;
 };
 
 inline uint32_t num_buckets(const fg_common_fluid_params& p) {
@@ -161,9 +186,9 @@ __device__ __forceinline__ uint32_t grid_linear_key(int cx, int cy, int cz, cons
 
 // Key of a position in either mode.
 __device__ __forceinline__ uint32_t grid_key(float px, float py, float pz, const GridDesc& g) {
-    if (g.mode == FG_KEY_REF_XOR) return hash_ref_xor(px, py, pz, g.cs.x, g.cs.y, g.cs.z, g.nb);
+    if (g.mode == FG_KEY_REF_XOR) return g.key_base + hash_ref_xor(px, py, pz, g.cs.x, g.cs.y, g.cs.z, g.nb);
     CellXYZ c = grid_cell(px, py, pz, g);
-    return grid_linear_key(c.x, c.y, c.z, g);
+    return g.key_base + grid_linear_key(c.x, c.y, c.z, g);
 }
 
 __device__ __forceinline__ CellXYZ linear_cell(float px, float py, float pz, const fg_common_fluid_params& cp) {

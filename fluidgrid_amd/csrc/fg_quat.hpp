@@ -37,6 +37,14 @@ __device__ __forceinline__ float4 quat_add(float4 a, float4 b) {
     return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 
+// K_HashGrid.cu:52-57: a body sphere in world space, conj(q) (p,0) q + cm,
+// keeping the body index in w.
+__device__ __forceinline__ float4 world_sphere(float4 rel, float4 q, float4 cm) {
+    float4 a = quat_add(quat_transform_point(rel, q), cm);
+    a.w = rel.w;
+    return a;
+}
+
 __device__ __forceinline__ float4 f4_scale(float4 a, float s) { return make_float4(a.x * s, a.y * s, a.z * s, a.w * s); }
 
 // K_MathUtils.cu:72-82

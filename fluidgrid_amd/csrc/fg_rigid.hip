@@ -8,6 +8,7 @@
 //    scan), one workgroup per body with a fixed reduction order;
 //  * body centre of mass and inertia tensor in FP64 from the sphere set;
 //  * body integration (IntegrateRBK) and the reference's reduction helpers.
+#include "fg_batch_bodies.hpp"
 #include "fg_contact.hpp"
 #include "fg_internal.hpp"
 #include "fg_quat.hpp"
@@ -20,16 +21,17 @@ GridDesc fluid_grid_desc(const fg_common_fluid_params& cp, fg_float3 cs, uint32_
 
 #pragma clang fp contract(off)
 
-// K_RigidBodySim.cu:66-99
-__global__ void __launch_bounds__(256)
-    k_body_forces_from_fluid(const float4* __restrict__ rb_pos, const float4* __restrict__ rb_vel,
-                             const float4* __restrict__ rb_cm, float4* __restrict__ out_f, float4* __restrict__ out_t,
-                             uint32_t m, uint32_t nbodies, float radius, const float4* __restrict__ sph_pos,
-                             const float4* __restrict__ sph_vel, const uint32_t* __restrict__ sph_start,
-                             const uint32_t* __restrict__ sph_end, uint32_t nsph, float sph_radius, GridDesc g,
-                             int3 relation, float ks, float kd) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+// K_RigidBodySim.cu:66-99, for body sphere i.
+__device__ __forceinline__ void body_sphere_from_fluid(uint32_t i, const float4* __restrict__ rb_pos,
+                                                       const float4* __restrict__ rb_vel,
+                                                       const float4* __restrict__ rb_cm, float4* __restrict__ out_f,
+                                                       float4* __restrict__ out_t, uint32_t nbodies, float radius,
+                                                       const float4* __restrict__ sph_pos,
+                                                       const float4* __restrict__ sph_vel,
+                                                       const uint32_t* __restrict__ sph_start,
+                                                       const uint32_t* __restrict__ sph_end, uint32_t nsph,
+                                                       float sph_radius, const GridDesc& g, int3 relation, float ks,
+                                                       float kd) {
     const float4 pos = rb_pos[i];
     const int b = f2i_sat(pos.w);
     const bool ok = b >= 0 && (uint32_t)b < nbodies;
@@ -42,6 +44,19 @@ __global__ void __launch_bounds__(256)
     float4 t = vec3_cross(dir, make_float4(F.x, F.y, F.z, 0.0f));
     t.w = (float)b;
     out_t[i] = t;
+}
+
+__global__ void __launch_bounds__(256)
+    k_body_forces_from_fluid(const float4* __restrict__ rb_pos, const float4* __restrict__ rb_vel,
+                             const float4* __restrict__ rb_cm, float4* __restrict__ out_f, float4* __restrict__ out_t,
+                             uint32_t m, uint32_t nbodies, float radius, const float4* __restrict__ sph_pos,
+                             const float4* __restrict__ sph_vel, const uint32_t* __restrict__ sph_start,
+                             const uint32_t* __restrict__ sph_end, uint32_t nsph, float sph_radius, GridDesc g,
+                             int3 relation, float ks, float kd) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    body_sphere_from_fluid(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, radius, sph_pos, sph_vel, sph_start,
+                           sph_end, nsph, sph_radius, g, relation, ks, kd);
 }
 
 __device__ __forceinline__ bool inside_box(const float4& p, const float4& lo, const float4& hi) {
@@ -57,14 +72,16 @@ __device__ __forceinline__ bool inside_box(const float4& p, const float4& lo, co
 // body's box, and the static walk for a sphere outside the static box; a
 // skipped walk would only have added +0.  ACCUM adds the force and torque to
 // what out_f/out_t hold (the fluid's part), else they are written.
+// For body sphere i; the broad phase looks at the bounds of bodies
+// [k0, k1) other than the sphere's own.
 template <bool ACCUM>
-__global__ void __launch_bounds__(256)
-    k_body_forces_from_bodies(const float4* __restrict__ rb_pos, const float4* __restrict__ rb_vel,
-                              const float4* __restrict__ rb_cm, float4* __restrict__ out_f,
-                              float4* __restrict__ out_t, uint32_t m, uint32_t nbodies, float radius, BodyContact dyn,
-                              BodyContact st, const float4* __restrict__ body_bounds, uint32_t nbounds) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+__device__ __forceinline__ void body_sphere_from_bodies(uint32_t i, const float4* __restrict__ rb_pos,
+                                                        const float4* __restrict__ rb_vel,
+                                                        const float4* __restrict__ rb_cm, float4* __restrict__ out_f,
+                                                        float4* __restrict__ out_t, uint32_t nbodies, float radius,
+                                                        const BodyContact& dyn, const BodyContact& st,
+                                                        const float4* __restrict__ body_bounds, uint32_t k0,
+                                                        uint32_t k1) {
     const float4 pos = rb_pos[i];
     const int b = f2i_sat(pos.w);
     const bool ok = b >= 0 && (uint32_t)b < nbodies;
@@ -73,7 +90,7 @@ __global__ void __launch_bounds__(256)
     F3 F = f3(0.0f, 0.0f, 0.0f);
     if (dyn.enabled) {
         bool near = body_bounds == nullptr;
-        for (uint32_t k = 0; !near && k < nbounds; k++)
+        for (uint32_t k = k0; !near && k < k1; k++)
             if ((int)k != b) near = inside_box(pos, body_bounds[2 * k], body_bounds[2 * k + 1]);
         if (near)
             F = add(F, contact_stencil<false>(pos, vel, radius, dyn.pos, dyn.vel, dyn.body_radius, dyn.ks, dyn.kd,
@@ -94,6 +111,18 @@ __global__ void __launch_bounds__(256)
         out_f[i] = make_float4(F.x, F.y, F.z, (float)b);
         out_t[i] = make_float4(t.x, t.y, t.z, (float)b);
     }
+}
+
+template <bool ACCUM>
+__global__ void __launch_bounds__(256)
+    k_body_forces_from_bodies(const float4* __restrict__ rb_pos, const float4* __restrict__ rb_vel,
+                              const float4* __restrict__ rb_cm, float4* __restrict__ out_f,
+                              float4* __restrict__ out_t, uint32_t m, uint32_t nbodies, float radius, BodyContact dyn,
+                              BodyContact st, const float4* __restrict__ body_bounds, uint32_t nbounds) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    body_sphere_from_bodies<ACCUM>(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, radius, dyn, st, body_bounds,
+                                   0u, nbounds);
 }
 
 // K_RigidBodySim.cu:102-111
@@ -128,13 +157,13 @@ __global__ void k_add_force_torque(const float4* __restrict__ sum_f, const float
     to_t[b] = quat_add(to_t[b], sum_t[o]);
 }
 
-// K_RigidBodySim.cu:160-262
-__global__ void k_integrate_rb(float4* __restrict__ pos, const float* __restrict__ mass, float4* __restrict__ vel,
-                               const float4* __restrict__ forces, uint64_t nb, float dt, float4 ext, float3 vmin,
-                               float3 vmax, float4* __restrict__ L, const float4* __restrict__ torque,
-                               const float4* __restrict__ inv_inertia, float4* __restrict__ orient, float vlimit) {
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nb) return;
+// K_RigidBodySim.cu:160-262, for body idx.
+__device__ __forceinline__ void integrate_body(uint64_t idx, float4* __restrict__ pos, const float* __restrict__ mass,
+                                               float4* __restrict__ vel, const float4* __restrict__ forces, float dt,
+                                               float4 ext, float3 vmin, float3 vmax, float4* __restrict__ L,
+                                               const float4* __restrict__ torque,
+                                               const float4* __restrict__ inv_inertia, float4* __restrict__ orient,
+                                               float vlimit) {
     float4 F = forces[idx];
     const float spd = sqrtf(F.x * F.x + F.y * F.y + F.z * F.z + F.w * F.w);
     if (spd > vlimit) F = f4_scale(F, vlimit / spd);
@@ -162,6 +191,15 @@ __global__ void k_integrate_rb(float4* __restrict__ pos, const float* __restrict
     else if (p.z >= vmax.z) { p.z = vmax.z; v.z *= -0.7f; }
     pos[idx] = p;
     vel[idx] = v;
+}
+
+__global__ void k_integrate_rb(float4* __restrict__ pos, const float* __restrict__ mass, float4* __restrict__ vel,
+                               const float4* __restrict__ forces, uint64_t nb, float dt, float4 ext, float3 vmin,
+                               float3 vmax, float4* __restrict__ L, const float4* __restrict__ torque,
+                               const float4* __restrict__ inv_inertia, float4* __restrict__ orient, float vlimit) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    integrate_body(idx, pos, mass, vel, forces, dt, ext, vmin, vmax, L, torque, inv_inertia, orient, vlimit);
 }
 
 #pragma clang fp contract(fast)
@@ -311,6 +349,190 @@ __global__ void __launch_bounds__(256) k_body_bounds(const float4* __restrict__ 
         out[2 * b] = make_float4(s[0][0] - margin, s[1][0] - margin, s[2][0] - margin, 0.0f);
         out[2 * b + 1] = make_float4(s[3][0] + margin, s[4][0] + margin, s[5][0] + margin, 0.0f);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched scenes: the kernels above, element by element, with the
+// parameters of the element's scene (BatchBodyScene).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_bbody_transform(uint32_t* __restrict__ key, uint32_t* __restrict__ id, const float4* __restrict__ cm,
+                      const float4* __restrict__ orient, const float4* __restrict__ rel, float4* __restrict__ world,
+                      uint32_t m, const uint32_t* __restrict__ body_scene, const BatchBodyScene* __restrict__ scenes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const float4 p = rel[i];
+    const int b = f2i_sat(p.w);
+    const float4 a = world_sphere(p, orient[b], cm[b]);
+    id[i] = i;
+    world[i] = a;
+    key[i] = grid_key(a.x, a.y, a.z, scenes[body_scene[b]].dyn.grid);
+}
+
+// One workgroup per scene over its spheres (k_sphere_bounds per scene; min
+// and max do not depend on the reduction order).
+__global__ void __launch_bounds__(256) k_bscene_bounds(const float4* __restrict__ p,
+                                                       const BatchBodyScene* __restrict__ scenes, uint32_t nscenes,
+                                                       float4* __restrict__ out) {
+    __shared__ float s[6][256];
+    const uint32_t sc = blockIdx.x;
+    if (sc >= nscenes) return;
+    const BatchBodyScene& d = scenes[sc];
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (uint32_t i = d.sphere_begin + threadIdx.x; i < d.sphere_end; i += 256) {
+        const float4 q = p[i];
+        lo[0] = fminf(lo[0], q.x); lo[1] = fminf(lo[1], q.y); lo[2] = fminf(lo[2], q.z);
+        hi[0] = fmaxf(hi[0], q.x); hi[1] = fmaxf(hi[1], q.y); hi[2] = fmaxf(hi[2], q.z);
+    }
+    for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = lo[k];
+        s[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float mg = d.bounds_margin;
+        out[2 * sc] = make_float4(s[0][0] - mg, s[1][0] - mg, s[2][0] - mg, 0.0f);
+        out[2 * sc + 1] = make_float4(s[3][0] + mg, s[4][0] + mg, s[5][0] + mg, 0.0f);
+    }
+}
+
+// k_body_bounds with the margin of each body's scene.
+__global__ void __launch_bounds__(256) k_bbody_bounds(const float4* __restrict__ p, const uint32_t* __restrict__ offset,
+                                                      const uint32_t* __restrict__ body_scene,
+                                                      const BatchBodyScene* __restrict__ scenes, uint32_t nbodies,
+                                                      float4* __restrict__ out) {
+    __shared__ float s[6][256];
+    const uint32_t b = blockIdx.x;
+    if (b >= nbodies) return;
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (uint32_t i = offset[b] + threadIdx.x; i < offset[b + 1]; i += 256) {
+        const float4 q = p[i];
+        lo[0] = fminf(lo[0], q.x); lo[1] = fminf(lo[1], q.y); lo[2] = fminf(lo[2], q.z);
+        hi[0] = fmaxf(hi[0], q.x); hi[1] = fmaxf(hi[1], q.y); hi[2] = fmaxf(hi[2], q.z);
+    }
+    for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = lo[k];
+        s[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float mg = scenes[body_scene[b]].pair_margin;
+        out[2 * b] = make_float4(s[0][0] - mg, s[1][0] - mg, s[2][0] - mg, 0.0f);
+        out[2 * b + 1] = make_float4(s[3][0] + mg, s[4][0] + mg, s[5][0] + mg, 0.0f);
+    }
+}
+
+#pragma clang fp contract(off)
+__global__ void __launch_bounds__(256)
+    k_bbody_from_fluid(const float4* __restrict__ rb_pos, const float4* __restrict__ rb_vel,
+                       const float4* __restrict__ rb_cm, float4* __restrict__ out_f, float4* __restrict__ out_t,
+                       uint32_t m, uint32_t nbodies, const uint32_t* __restrict__ body_scene,
+                       const BatchBodyScene* __restrict__ scenes, const float4* __restrict__ sph_pos,
+                       const float4* __restrict__ sph_vel, const uint32_t* __restrict__ sph_start,
+                       const uint32_t* __restrict__ sph_end, uint32_t nsph) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int b = f2i_sat(rb_pos[i].w);
+    if (b < 0 || (uint32_t)b >= nbodies) return;
+    const BatchBodyScene& d = scenes[body_scene[b]];
+    body_sphere_from_fluid(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, d.sphere_radius, sph_pos, sph_vel,
+                           sph_start, sph_end, nsph, d.fluid_radius, d.fluid, d.rb_to_fluid, d.ks, d.kd);
+}
+
+// Scenes with one body have no body-body pass (the single-fluid simulator
+// does not launch it), so their spheres keep exactly the fluid's part.
+__global__ void __launch_bounds__(256)
+    k_bbody_from_bodies(const float4* __restrict__ rb_pos, const float4* __restrict__ rb_vel,
+                        const float4* __restrict__ rb_cm, float4* __restrict__ out_f, float4* __restrict__ out_t,
+                        uint32_t m, uint32_t nbodies, const uint32_t* __restrict__ body_scene,
+                        const BatchBodyScene* __restrict__ scenes, const float4* __restrict__ body_bounds) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int b = f2i_sat(rb_pos[i].w);
+    if (b < 0 || (uint32_t)b >= nbodies) return;
+    const BatchBodyScene& d = scenes[body_scene[b]];
+    if (d.body_end - d.body_begin < 2) return;
+    BodyContact dyn = d.dyn;  // as fg_body_forces_from_bodies sets it up
+    dyn.relation = make_int3(1, 1, 1);
+    dyn.fluid_radius = 0.0f;
+    dyn.eps = 0.001f;
+    dyn.aabb = nullptr;
+    const BodyContact st{};
+    body_sphere_from_bodies<true>(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, d.sphere_radius, dyn, st,
+                                  body_bounds, d.body_begin, d.body_end);
+}
+
+__global__ void k_bintegrate_rb(float4* __restrict__ pos, const float* __restrict__ mass, float4* __restrict__ vel,
+                                const float4* __restrict__ forces, uint32_t nb, const uint32_t* __restrict__ body_scene,
+                                const BatchBodyScene* __restrict__ scenes, float4* __restrict__ L,
+                                const float4* __restrict__ torque, const float4* __restrict__ inv_inertia,
+                                float4* __restrict__ orient) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb) return;
+    const BatchBodyScene& d = scenes[body_scene[idx]];
+    integrate_body(idx, pos, mass, vel, forces, d.dt, d.ext, d.vmin, d.vmax, L, torque, inv_inertia, orient,
+                   d.vlimit);
+}
+#pragma clang fp contract(fast)
+
+int batch_body_transform(uint32_t* key, uint32_t* id, const float4* cm, const float4* orient, const float4* rel,
+                         float4* world, uint32_t m, const uint32_t* body_scene, const BatchBodyScene* scenes) {
+    if (m == 0) return FG_OK;
+    hipLaunchKernelGGL(k_bbody_transform, dim3(grid_for(m, 256)), dim3(256), 0, state().stream, key, id, cm, orient,
+                       rel, world, m, body_scene, scenes);
+    return check_launch("batch_body_transform") ? FG_OK : fg_peek_error();
+}
+
+int batch_scene_bounds(const float4* world, const BatchBodyScene* scenes, uint32_t nscenes, float4* out) {
+    if (nscenes == 0) return FG_OK;
+    hipLaunchKernelGGL(k_bscene_bounds, dim3(nscenes), dim3(256), 0, state().stream, world, scenes, nscenes, out);
+    return check_launch("batch_scene_bounds") ? FG_OK : fg_peek_error();
+}
+
+int batch_body_bounds(const float4* world, const uint32_t* offsets, const uint32_t* body_scene,
+                      const BatchBodyScene* scenes, uint32_t nbodies, float4* out) {
+    if (nbodies == 0) return FG_OK;
+    hipLaunchKernelGGL(k_bbody_bounds, dim3(nbodies), dim3(256), 0, state().stream, world, offsets, body_scene,
+                       scenes, nbodies, out);
+    return check_launch("batch_body_bounds") ? FG_OK : fg_peek_error();
+}
+
+int batch_body_forces(const float4* world, const float4* vel, const float4* cm, float4* out_f, float4* out_t,
+                      uint32_t m, uint32_t nbodies, const uint32_t* body_scene, const BatchBodyScene* scenes,
+                      const float4* fluid_pos, const float4* fluid_vel, const uint32_t* fluid_start,
+                      const uint32_t* fluid_end, uint32_t nfluid, const float4* body_bounds) {
+    if (m == 0) return FG_OK;
+    hipLaunchKernelGGL(k_bbody_from_fluid, dim3(grid_for(m, 256)), dim3(256), 0, state().stream, world, vel, cm,
+                       out_f, out_t, m, nbodies, body_scene, scenes, fluid_pos, fluid_vel, fluid_start, fluid_end,
+                       nfluid);
+    hipLaunchKernelGGL(k_bbody_from_bodies, dim3(grid_for(m, 256)), dim3(256), 0, state().stream, world, vel, cm,
+                       out_f, out_t, m, nbodies, body_scene, scenes, body_bounds);
+    return check_launch("batch_body_forces") ? FG_OK : fg_peek_error();
+}
+
+int batch_integrate_bodies(float4* cm, const float* mass, float4* vel, const float4* force, uint32_t nbodies,
+                           const uint32_t* body_scene, const BatchBodyScene* scenes, float4* L, const float4* torque,
+                           const float4* inv_inertia, float4* orient) {
+    if (nbodies == 0) return FG_OK;
+    hipLaunchKernelGGL(k_bintegrate_rb, dim3(grid_for(nbodies, 64)), dim3(64), 0, state().stream, cm, mass, vel,
+                       force, nbodies, body_scene, scenes, L, torque, inv_inertia, orient);
+    return check_launch("batch_integrate_bodies") ? FG_OK : fg_peek_error();
 }
 
 static GridDesc body_grid(fg_float3 cs, uint32_t nb) {

@@ -1,9 +1,14 @@
 // BatchedSimulator.cpp -- host side of the batched-scene step (fg_batch.hip).
 #include "fluidgrid/BatchedSimulator.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 #include <string>
+
+#include "fg_batch_bodies.hpp"
+#include "rigid_math.hpp"
 
 namespace Simulation {
 namespace Simulators {
@@ -125,6 +130,7 @@ BatchedSimulator::BatchedSimulator(const std::vector<BatchScene>& scenes) {
     mTuning.sortDigitBits = bits;
     mTuning.flatDensity = fd != 0;
     mTuning.flatForces = ff != 0;
+    mSceneBodies.resize(scenes.size());
 }
 
 BatchedSimulator::~BatchedSimulator() {
@@ -136,6 +142,199 @@ BatchedSimulator::~BatchedSimulator() {
     dfree(d_descs); dfree(d_sceneOf); dfree(d_pos); dfree(d_vel); dfree(d_spos); dfree(d_svel); dfree(d_rho);
     dfree(d_key); dfree(d_id); dfree(d_start); dfree(d_end); dfree(d_masks);
     if (d_sortWs) (void)hipFree(d_sortWs);
+    releaseBodies();
+}
+
+void BatchedSimulator::releaseBodies() {
+    if (d_bscenes) (void)hipFree(d_bscenes);
+    d_bscenes = nullptr;
+    dfree(d_bodyScene); dfree(d_boffsets); dfree(d_bkey); dfree(d_bid); dfree(d_bstart); dfree(d_bend);
+    dfree(d_brel); dfree(d_bworld); dfree(d_bsorted); dfree(d_bpf); dfree(d_bpt); dfree(d_bcm); dfree(d_bvel);
+    dfree(d_borient); dfree(d_bL); dfree(d_bforce); dfree(d_btorque); dfree(d_binv); dfree(d_sceneBounds);
+    dfree(d_bodyBounds); dfree(d_bmass); dfree(d_bsumF); dfree(d_bsumT);
+    if (d_bsortWs) (void)hipFree(d_bsortWs);
+    d_bsortWs = nullptr;
+    mBodiesReady = false;
+}
+
+void BatchedSimulator::addBodies(unsigned scene, const BatchBodies& b) {
+    if (scene >= mDescs.size()) throw std::invalid_argument("BatchedSimulator::addBodies: no such scene");
+    if (mBodiesReady) throw std::logic_error("BatchedSimulator::addBodies: call before the first step");
+    if (!mSceneBodies[scene].masses.empty())
+        throw std::logic_error("BatchedSimulator::addBodies: scene " + std::to_string(scene) + " already has bodies");
+    const size_t B = b.masses.size();
+    if (B == 0 || b.offsets.size() != B + 1 || b.offsets[0] != 0 || b.offsets[B] != b.worldSpheres.size())
+        throw std::invalid_argument("BatchedSimulator::addBodies: offsets do not match the bodies and spheres");
+    for (size_t k = 0; k < B; k++)
+        if (b.offsets[k + 1] <= b.offsets[k] || !(b.masses[k] > 0.0))
+            throw std::invalid_argument("BatchedSimulator::addBodies: a body without spheres or mass");
+    if (b.rigid.getDnNumBuckets() <= 0 || !(b.sphereRadius > 0.0f))
+        throw std::invalid_argument("BatchedSimulator::addBodies: empty body grid or sphere radius");
+    mSceneBodies[scene] = b;
+    invalidateGraph();
+}
+
+// The single-fluid RigidBodySimulator::reset, for every scene at once.
+void BatchedSimulator::setupBodies() {
+    if (mBodiesReady) return;
+    const unsigned S = (unsigned)mDescs.size();
+    std::vector<float4> world;
+    std::vector<unsigned> offsets(1, 0), bodyScene;
+    std::vector<float> mass;
+    std::vector<fg::BatchBodyScene> bs(S);
+    uint64_t nbB = 0;
+    for (unsigned s = 0; s < S; s++) {
+        const BatchBodies& in = mSceneBodies[s];
+        fg::BatchBodyScene& d = bs[s];
+        std::memset(static_cast<void*>(&d), 0, sizeof(d));
+        d.body_begin = (unsigned)mass.size();
+        d.sphere_begin = (unsigned)world.size();
+        for (size_t b = 0; b < in.masses.size(); b++) {
+            const unsigned gb = (unsigned)mass.size();
+            for (unsigned i = in.offsets[b]; i < in.offsets[b + 1]; i++)
+                world.push_back(make_float4(in.worldSpheres[i].x, in.worldSpheres[i].y, in.worldSpheres[i].z, (float)gb));
+            offsets.push_back((unsigned)world.size());
+            mass.push_back((float)in.masses[b]);  // RigidBody keeps a float mass
+            bodyScene.push_back(s);
+        }
+        d.body_end = (unsigned)mass.size();
+        d.sphere_end = (unsigned)world.size();
+        if (d.body_end == d.body_begin) continue;
+        const auto& rp = in.rigid;
+        const fg_scene_desc& fd = mDescs[s];
+        const auto& cp = fd.common;
+        const float r = in.sphereRadius;
+        d.dyn.grid.cs = make_float3(rp.particleRadius, rp.particleRadius, rp.particleRadius);
+        d.dyn.grid.nb = (unsigned)rp.getDnNumBuckets();
+        d.dyn.grid.mode = FG_KEY_REF_XOR;
+        d.dyn.grid.key_base = (unsigned)nbB;
+        nbB += (unsigned)rp.getDnNumBuckets();
+        d.dyn.relation = make_int3(cp.fluid_to_srb_relation.x, cp.fluid_to_srb_relation.y, cp.fluid_to_srb_relation.z);
+        d.dyn.fluid_radius = 2.0f * cp.pointRadius;
+        d.dyn.body_radius = r;
+        d.dyn.ks = -in.springForce;
+        d.dyn.kd = -in.dampingForce;
+        d.dyn.eps = 0.003f;
+        d.dyn.enabled = 1;
+        d.fluid.vmin = make_float3(cp.volume_min.x, cp.volume_min.y, cp.volume_min.z);
+        d.fluid.cs = make_float3(cp.cell_size.x, cp.cell_size.y, cp.cell_size.z);
+        d.fluid.dims = make_int3(cp.bucket_size.x, cp.bucket_size.y, cp.bucket_size.z);
+        d.fluid.nb = (unsigned)cp.bucket_size.x * cp.bucket_size.y * cp.bucket_size.z;
+        d.fluid.mode = FG_KEY_LINEAR;
+        d.fluid.key_base = fd.keyBase;
+        d.rb_to_fluid = make_int3(rp.rb_to_fluid_relation.x, rp.rb_to_fluid_relation.y, rp.rb_to_fluid_relation.z);
+        d.sphere_radius = r;
+        d.fluid_radius = 2.0f * cp.pointRadius;
+        d.ks = -in.springForce;
+        d.kd = -in.dampingForce;
+        const float margin = 2.0f * cp.pointRadius + r + 0.003f;
+        d.bounds_margin = margin * 1.01f;
+        d.pair_margin = (2.0f * r + 0.001f) * 1.01f;
+        d.dt = rp.deltaTimeIntegration;
+        d.vlimit = rp.velocity_limit;
+        d.ext = make_float4(rp.global_external_force.x, rp.global_external_force.y, rp.global_external_force.z,
+                            rp.global_external_force.w);
+        d.vmin = make_float3(rp.volume_min.x, rp.volume_min.y, rp.volume_min.z);
+        d.vmax = make_float3(rp.volume_max.x, rp.volume_max.y, rp.volume_max.z);
+    }
+    const unsigned B = (unsigned)mass.size(), M = (unsigned)world.size();
+    if (B == 0) return;
+    if (nbB >= 0xFFFFFFFFull) throw std::invalid_argument("BatchedSimulator: too many body buckets");
+    mBodyCount = B;
+    mSphereCount = M;
+    mBodyBuckets = (unsigned)nbB;
+
+    d_bodyScene = dalloc<unsigned>(B);
+    d_boffsets = dalloc<unsigned>(B + 1);
+    d_bkey = dalloc<unsigned>(M);
+    d_bid = dalloc<unsigned>(M);
+    d_bstart = dalloc<unsigned>(mBodyBuckets);
+    d_bend = dalloc<unsigned>(mBodyBuckets);
+    d_brel = dalloc<float4>(M);
+    d_bworld = dalloc<float4>(M);
+    d_bsorted = dalloc<float4>(M);
+    d_bpf = dalloc<float4>(M);
+    d_bpt = dalloc<float4>(M);
+    d_bcm = dalloc<float4>(B);
+    d_bvel = dalloc<float4>(B);
+    d_borient = dalloc<float4>(B);
+    d_bL = dalloc<float4>(B);
+    d_bforce = dalloc<float4>(B);
+    d_btorque = dalloc<float4>(B);
+    d_binv = dalloc<float4>(3 * B);
+    d_sceneBounds = dalloc<float4>(2 * S);
+    d_bodyBounds = dalloc<float4>(2 * B);
+    d_bmass = dalloc<float>(B);
+    d_bsumF = dalloc<double>(4 * B);
+    d_bsumT = dalloc<double>(4 * B);
+    mBSortWsBytes = fg_sort_workspace_size(M, key_bits_for(mBodyBuckets));
+    hip_check(hipMalloc(&d_bsortWs, mBSortWsBytes ? mBSortWsBytes : 256), "hipMalloc");
+    hip_check(hipMalloc(&d_bscenes, sizeof(fg::BatchBodyScene) * S), "hipMalloc");
+
+    hip_check(hipMemcpy(d_bworld, world.data(), sizeof(float4) * M, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(d_boffsets, offsets.data(), sizeof(unsigned) * (B + 1), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(d_bodyScene, bodyScene.data(), sizeof(unsigned) * B, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(d_bmass, mass.data(), sizeof(float) * B, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemset(d_bend, 0, sizeof(unsigned) * mBodyBuckets), "memset");
+
+    // Centre of mass and inertia in FP64 on the GPU (per body, as in the
+    // single-fluid set-up), then the same host arithmetic.
+    std::vector<double> md(mass.begin(), mass.end()), cmd(3 * B), inertia(9 * B);
+    double *d_md = dalloc<double>(B), *d_cmd = dalloc<double>(3 * B), *d_in = dalloc<double>(9 * B);
+    hip_check(hipMemcpy(d_md, md.data(), sizeof(double) * B, hipMemcpyHostToDevice), "upload");
+    {
+        StreamScope scope(mStream);
+        fg_body_mass_properties_f64(reinterpret_cast<fg_float4*>(d_bworld), d_boffsets, B, d_md, d_cmd, d_in);
+    }
+    fg_check("fg_body_mass_properties_f64");
+    hip_check(hipStreamSynchronize(mStream), "sync");
+    hip_check(hipMemcpy(cmd.data(), d_cmd, sizeof(double) * 3 * B, hipMemcpyDeviceToHost), "download");
+    hip_check(hipMemcpy(inertia.data(), d_in, sizeof(double) * 9 * B, hipMemcpyDeviceToHost), "download");
+    dfree(d_md);
+    dfree(d_cmd);
+    dfree(d_in);
+    std::vector<float4> cm(B), inv(3 * B), zero(B, make_float4(0, 0, 0, 0)), orient(B, make_float4(0, 0, 0, 1));
+    for (unsigned b = 0; b < B; b++) {
+        cm[b] = make_float4((float)cmd[3 * b], (float)cmd[3 * b + 1], (float)cmd[3 * b + 2], 0.0f);
+        double iv[9];
+        fg_host::invert3(&inertia[9 * b], iv);
+        for (int r = 0; r < 3; r++)
+            inv[3 * b + r] = make_float4((float)iv[3 * r], (float)iv[3 * r + 1], (float)iv[3 * r + 2], 0.0f);
+    }
+    std::vector<float4> rel(M);
+    for (unsigned b = 0; b < B; b++)
+        for (unsigned i = offsets[b]; i < offsets[b + 1]; i++)
+            rel[i] = make_float4((float)((double)world[i].x - cmd[3 * b]), (float)((double)world[i].y - cmd[3 * b + 1]),
+                                 (float)((double)world[i].z - cmd[3 * b + 2]), (float)b);
+    hip_check(hipMemcpy(d_brel, rel.data(), sizeof(float4) * M, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(d_bcm, cm.data(), sizeof(float4) * B, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(d_binv, inv.data(), sizeof(float4) * 3 * B, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(d_borient, orient.data(), sizeof(float4) * B, hipMemcpyHostToDevice), "upload");
+    for (float4* p : {d_bvel, d_bL, d_bforce, d_btorque})
+        hip_check(hipMemcpy(p, zero.data(), sizeof(float4) * B, hipMemcpyHostToDevice), "upload");
+    // the shared arrays each scene's contact walks
+    for (unsigned s = 0; s < S; s++) {
+        fg::BatchBodyScene& d = bs[s];
+        d.dyn.pos = d_bsorted;
+        d.dyn.vel = d_bvel;
+        d.dyn.start = d_bstart;
+        d.dyn.end = d_bend;
+        d.dyn.aabb = d_sceneBounds + 2 * s;
+        d.dyn.count = M;
+        d.dyn.nvel = B;
+    }
+    hip_check(hipMemcpy(d_bscenes, bs.data(), sizeof(fg::BatchBodyScene) * S, hipMemcpyHostToDevice), "upload");
+    mBodiesReady = true;
+}
+
+void BatchedSimulator::downloadBodies(float4* pos, float4* vel, float4* orient, float4* L) const {
+    const size_t B = mBodyCount;
+    if (!B) return;
+    hip_check(hipStreamSynchronize(mStream), "sync");
+    if (pos) hip_check(hipMemcpy(pos, d_bcm, sizeof(float4) * B, hipMemcpyDeviceToHost), "download");
+    if (vel) hip_check(hipMemcpy(vel, d_bvel, sizeof(float4) * B, hipMemcpyDeviceToHost), "download");
+    if (orient) hip_check(hipMemcpy(orient, d_borient, sizeof(float4) * B, hipMemcpyDeviceToHost), "download");
+    if (L) hip_check(hipMemcpy(L, d_bL, sizeof(float4) * B, hipMemcpyDeviceToHost), "download");
 }
 
 void BatchedSimulator::invalidateGraph() {
@@ -166,18 +365,52 @@ void BatchedSimulator::enqueueStep() {
     gFindCellOffsetOrderingPosVel(d_start, d_end, d_key, d_id, reinterpret_cast<fg_float4*>(d_pos),
                                   reinterpret_cast<fg_float4*>(d_spos), reinterpret_cast<fg_float4*>(d_vel),
                                   reinterpret_cast<fg_float4*>(d_svel), mN, mBuckets);
+    if (mBodyCount) enqueueBodyStages(true);
     if (mTiming) (void)hipEventRecord(mEv[3], mStream);
     fg_batch_density_nbr(d_descs, S, d_sceneOf, reinterpret_cast<fg_float4*>(d_spos), reinterpret_cast<fg_float2*>(d_rho),
                          d_start, d_end, d_masks, mN);
     if (mTiming) (void)hipEventRecord(mEv[4], mStream);
-    fg_batch_forces_integrate_nbr(d_descs, S, d_sceneOf, reinterpret_cast<fg_float4*>(d_spos),
-                                  reinterpret_cast<fg_float4*>(d_svel), reinterpret_cast<fg_float2*>(d_rho), d_start,
-                                  d_end, d_masks, reinterpret_cast<fg_float4*>(d_vel),
-                                  reinterpret_cast<fg_float4*>(d_pos), d_id, mN);
+    if (mBodyCount) {
+        fg::batch_body_forces(d_bworld, d_bvel, d_bcm, d_bpf, d_bpt, mSphereCount, mBodyCount, d_bodyScene,
+                              static_cast<const fg::BatchBodyScene*>(d_bscenes), d_spos, d_svel, d_start, d_end, mN,
+                              d_bodyBounds);
+        fg::batch_forces_coupled(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, d_vel, d_pos,
+                                 d_id, mN, static_cast<const fg::BatchBodyScene*>(d_bscenes));
+        enqueueBodyStages(false);
+    } else {
+        fg_batch_forces_integrate_nbr(d_descs, S, d_sceneOf, reinterpret_cast<fg_float4*>(d_spos),
+                                      reinterpret_cast<fg_float4*>(d_svel), reinterpret_cast<fg_float2*>(d_rho),
+                                      d_start, d_end, d_masks, reinterpret_cast<fg_float4*>(d_vel),
+                                      reinterpret_cast<fg_float4*>(d_pos), d_id, mN);
+    }
     if (mTiming) (void)hipEventRecord(mEv[5], mStream);
 }
 
+// Body stages of the coupled step (RigidBodySimulator::generateCollisionData
+// before the fluid density; applyForces and integrate after the fluid
+// forces), for every scene at once.
+void BatchedSimulator::enqueueBodyStages(bool before_density) {
+    const fg::BatchBodyScene* bs = static_cast<const fg::BatchBodyScene*>(d_bscenes);
+    if (before_density) {
+        fg::batch_body_transform(d_bkey, d_bid, d_bcm, d_borient, d_brel, d_bworld, mSphereCount, d_bodyScene, bs);
+        fg_sort_pairs_ws(d_bkey, d_bid, mSphereCount, key_bits_for(mBodyBuckets), d_bsortWs, mBSortWsBytes, 1);
+        gFindCellOffsetOrderingPos(d_bstart, d_bend, d_bkey, d_bid, reinterpret_cast<fg_float4*>(d_bworld),
+                                   reinterpret_cast<fg_float4*>(d_bsorted), mSphereCount, mBodyBuckets);
+        fg::batch_scene_bounds(d_bworld, bs, (unsigned)mDescs.size(), d_sceneBounds);
+        fg::batch_body_bounds(d_bworld, d_boffsets, d_bodyScene, bs, mBodyCount, d_bodyBounds);
+        return;
+    }
+    fg_body_reduce_f64(reinterpret_cast<fg_float4*>(d_bpf), reinterpret_cast<fg_float4*>(d_bpt), d_boffsets,
+                       mBodyCount, reinterpret_cast<fg_double4*>(d_bsumF), reinterpret_cast<fg_double4*>(d_bsumT),
+                       reinterpret_cast<fg_float4*>(d_bforce), reinterpret_cast<fg_float4*>(d_btorque));
+    fg::batch_integrate_bodies(d_bcm, d_bmass, d_bvel, d_bforce, mBodyCount, d_bodyScene, bs, d_bL, d_btorque, d_binv,
+                               d_borient);
+    fg_memset_u32(reinterpret_cast<unsigned*>(d_bforce), 0u, 4 * (size_t)mBodyCount);
+    fg_memset_u32(reinterpret_cast<unsigned*>(d_btorque), 0u, 4 * (size_t)mBodyCount);
+}
+
 void BatchedSimulator::doTimeStep() {
+    setupBodies();
     StreamScope scope(mStream);
     enqueueStep();
     fg_check("BatchedSimulator::doTimeStep");
@@ -189,6 +422,7 @@ void BatchedSimulator::doTimeStep() {
 
 void BatchedSimulator::doTimeSteps(int steps) {
     if (steps <= 0) return;
+    setupBodies();
     if (!mUseGraph || mTiming) {
         for (int s = 0; s < steps; s++) doTimeStep();
         return;
@@ -210,11 +444,22 @@ void BatchedSimulator::synchronize() { hip_check(hipStreamSynchronize(mStream), 
 
 BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<BatchTuning, float>>* trials) {
     if (trialSteps < 1) trialSteps = 1;
+    setupBodies();
     synchronize();
     // Save the state so that the trials leave the simulation where it was.
     float4 *bp = dalloc<float4>(mN), *bv = dalloc<float4>(mN);
     hip_check(hipMemcpy(bp, d_pos, sizeof(float4) * mN, hipMemcpyDeviceToDevice), "save");
     hip_check(hipMemcpy(bv, d_vel, sizeof(float4) * mN, hipMemcpyDeviceToDevice), "save");
+    const size_t B = mBodyCount;
+    float4* const bodyState[4] = {d_bcm, d_bvel, d_borient, d_bL};
+    float4* bb = B ? dalloc<float4>(4 * B) : nullptr;
+    for (int k = 0; k < 4 && B; k++)
+        hip_check(hipMemcpy(bb + k * B, bodyState[k], sizeof(float4) * B, hipMemcpyDeviceToDevice), "save");
+    auto restore_bodies = [&]() {
+        for (int k = 0; k < 4 && B; k++)
+            hip_check(hipMemcpyAsync(bodyState[k], bb + k * B, sizeof(float4) * B, hipMemcpyDeviceToDevice, mStream),
+                      "restore");
+    };
     const bool timing = mTiming, graph = mUseGraph;
     mTiming = false;
     mUseGraph = true;
@@ -227,6 +472,7 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
         setTuning(t);
         hip_check(hipMemcpyAsync(d_pos, bp, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
         hip_check(hipMemcpyAsync(d_vel, bv, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
+        restore_bodies();
         doTimeSteps(2);  // capture + warm-up
         hip_check(hipEventRecord(e0, mStream), "event");
         doTimeSteps(trialSteps);
@@ -269,9 +515,14 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
     setTuning(best);
     hip_check(hipMemcpyAsync(d_pos, bp, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
     hip_check(hipMemcpyAsync(d_vel, bv, sizeof(float4) * mN, hipMemcpyDeviceToDevice, mStream), "restore");
+    restore_bodies();
     hip_check(hipStreamSynchronize(mStream), "restore");
     dfree(bp);
     dfree(bv);
+    if (bb) {
+        hip_check(hipStreamSynchronize(mStream), "sync");
+        dfree(bb);
+    }
     mTiming = timing;
     mUseGraph = graph;
     return best;

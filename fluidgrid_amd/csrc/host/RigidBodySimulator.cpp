@@ -12,6 +12,7 @@
 #include <stdexcept>
 
 #include "fluidgrid/MeshSampler.h"
+#include "rigid_math.hpp"
 
 namespace Simulation {
 namespace RigidBody {
@@ -50,22 +51,6 @@ int key_bits_for(unsigned int nb) {
     return b;
 }
 
-// 3x3 inverse in FP64 (row-major); throws when singular.
-void invert3(const double* a, double* inv) {
-    const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
-    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
-    if (!(std::fabs(det) > 0.0)) throw std::runtime_error("RigidBodySimulator: singular inertia tensor");
-    const double id = 1.0 / det;
-    inv[0] = c00 * id;
-    inv[1] = (a[2] * a[7] - a[1] * a[8]) * id;
-    inv[2] = (a[1] * a[5] - a[2] * a[4]) * id;
-    inv[3] = c01 * id;
-    inv[4] = (a[0] * a[8] - a[2] * a[6]) * id;
-    inv[5] = (a[2] * a[3] - a[0] * a[5]) * id;
-    inv[6] = c02 * id;
-    inv[7] = (a[1] * a[6] - a[0] * a[7]) * id;
-    inv[8] = (a[0] * a[4] - a[1] * a[3]) * id;
-}
 
 fg_float3 abi3(float3 v) { return fg_float3{v.x, v.y, v.z}; }
 }  // namespace
@@ -310,7 +295,7 @@ void RigidBodySimulator::reset() {
     for (size_t b = 0; b < B; b++) {
         cm[b] = make_float4((float)mCM[3 * b], (float)mCM[3 * b + 1], (float)mCM[3 * b + 2], 0.0f);
         double iv[9];
-        invert3(&mInertia[9 * b], iv);
+        fg_host::invert3(&mInertia[9 * b], iv);
         for (int r = 0; r < 3; r++) {
             inv[3 * b + r] = make_float4((float)iv[3 * r], (float)iv[3 * r + 1], (float)iv[3 * r + 2], 0.0f);
             const double* I = &mInertia[9 * b];

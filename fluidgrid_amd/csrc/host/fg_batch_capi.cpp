@@ -48,6 +48,50 @@ int fg_bsim_create(unsigned int numScenes, const fg_common_fluid_params* commons
     }
 }
 
+int fg_bsim_add_bodies(unsigned int scene, const fg_float4* worldSpheres, const unsigned int* bodyOffsets,
+                       unsigned int numBodies, const double* masses, float sphereRadius,
+                       const fg_common_rigid_body_params* rigidParams, float springForce, float dampingForce) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    if (!worldSpheres || !bodyOffsets || !masses || !rigidParams || numBodies == 0) {
+        fg_record_error(FG_ERR_INVALID_ARGUMENT, "fg_bsim_add_bodies: null pointer or no bodies");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    try {
+        Simulators::BatchBodies b;
+        b.offsets.assign(bodyOffsets, bodyOffsets + numBodies + 1);
+        const float4* w = reinterpret_cast<const float4*>(worldSpheres);
+        b.worldSpheres.assign(w, w + bodyOffsets[numBodies]);
+        b.masses.assign(masses, masses + numBodies);
+        b.sphereRadius = sphereRadius;
+        b.rigid = Application::abi_cast<Application::CommonRigidBodyParams>(*rigidParams);
+        b.springForce = springForce;
+        b.dampingForce = dampingForce;
+        batch()->addBodies(scene, b);
+        return FG_OK;
+    } catch (const std::invalid_argument& e) {
+        fg_record_error(FG_ERR_INVALID_ARGUMENT, e.what());
+        return FG_ERR_INVALID_ARGUMENT;
+    } catch (const std::logic_error& e) {
+        fg_record_error(FG_ERR_INVALID_ARGUMENT, e.what());
+        return FG_ERR_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int fg_bsim_body_state(unsigned int* numBodies, fg_float4* positions, fg_float4* velocities, fg_float4* orientations,
+                       fg_float4* angularMomenta) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        if (numBodies) *numBodies = batch()->numBodies();
+        batch()->downloadBodies(reinterpret_cast<float4*>(positions), reinterpret_cast<float4*>(velocities),
+                                reinterpret_cast<float4*>(orientations), reinterpret_cast<float4*>(angularMomenta));
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 int fg_bsim_set_options(int use_graph, int stage_timing) {
     if (!batch()) return FG_ERR_NOT_INITIALIZED;
     batch()->setUseGraph(use_graph != 0);

@@ -136,6 +136,8 @@ SIGNATURES = {
     "fg_batch_density_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, u32]),
     "fg_batch_forces_integrate_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
     "fg_bsim_create": (i32, [u32, vp, vp, vp, vp, vp]),
+    "fg_bsim_add_bodies": (i32, [u32, vp, vp, u32, vp, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
+    "fg_bsim_body_state": (i32, [C.POINTER(C.c_uint), vp, vp, vp, vp]),
     "fg_bsim_set_options": (i32, [i32, i32]),
     "fg_bsim_step": (i32, [i32]),
     "fg_bsim_synchronize": (i32, []),
@@ -423,6 +425,24 @@ class BatchSimulator:
 
     def set_options(self, use_graph=True, stage_timing=False):
         check(load().fg_bsim_set_options(int(use_graph), int(stage_timing)), "fg_bsim_set_options")
+
+    def add_bodies(self, scene, bodies):
+        """Couple a scenes.BodySet to the fluid of scene `scene` (before the first step)."""
+        world = np.ascontiguousarray(bodies.world, np.float32)
+        offs = np.ascontiguousarray(bodies.offsets, np.uint32)
+        mass = np.ascontiguousarray(bodies.mass, np.float64)
+        check(load().fg_bsim_add_bodies(int(scene), world.ctypes.data, offs.ctypes.data, bodies.b, mass.ctypes.data,
+                                        bodies.sphere_radius, C.byref(bodies.rb), bodies.spring, bodies.damping),
+              "fg_bsim_add_bodies")
+
+    def body_state(self):
+        """Per body (scenes in order): dict of pos, vel, orient, L (float32 (B, 4))."""
+        L = load()
+        nb = C.c_uint(0)
+        check(L.fg_bsim_body_state(C.byref(nb), None, None, None, None), "fg_bsim_body_state")
+        out = [np.empty((nb.value, 4), np.float32) for _ in range(4)]
+        check(L.fg_bsim_body_state(C.byref(nb), *[o.ctypes.data for o in out]), "fg_bsim_body_state")
+        return dict(zip(("pos", "vel", "orient", "L"), out))
 
     def step(self, steps=1):
         check(load().fg_bsim_step(int(steps)), "fg_bsim_step")

@@ -288,6 +288,16 @@ def torus_spheres(R, r, d):
     return _surface_voxels(sdf, (-ext, -(r + d), -ext), (ext, r + d, ext), d)
 
 
+def ball_spheres(radius, d):
+    """Surface-voxel sample of a sphere of the given radius centred at the origin."""
+    ext = radius + d
+
+    def sdf(x, y, z):
+        return np.sqrt(x * x + y * y + z * z) - radius
+
+    return _surface_voxels(sdf, (-ext,) * 3, (ext,) * 3, d)
+
+
 def box_spheres(edge, d):
     """Surface-voxel sample of a cube of the given edge centred at the origin."""
     a = 0.5 * edge
@@ -469,9 +479,11 @@ def config4_scene(seed=None, num_bodies=64):
 
 @dataclass
 class Ensemble:
-    """Independent dam-break scenes advanced together (scene-major)."""
+    """Independent dam-break scenes advanced together (scene-major);
+    bodies[s] is scene s's BodySet or None."""
     scenes: list
     seed: int
+    bodies: list = None
 
     @property
     def n(self):
@@ -482,7 +494,8 @@ class Ensemble:
         return sum(s.num_buckets for s in self.scenes)
 
 
-def ensemble(num_scenes=256, seed=SEED_BASE + 3, n_range=(4096, 65536), edge_range=(0.5, 1.5), fill=0.3):
+def ensemble(num_scenes=256, seed=SEED_BASE + 3, n_range=(4096, 65536), edge_range=(0.5, 1.5), fill=0.3,
+             bodies=False, max_bodies=4):
     """Scenes with log-uniform particle counts in n_range and box edges L
     uniform in edge_range; h is chosen so that the block of N particles at
     spacing h/2 fills `fill` of the box: N (h/2)^3 = fill L^3.  The block is
@@ -507,4 +520,43 @@ def ensemble(num_scenes=256, seed=SEED_BASE + 3, n_range=(4096, 65536), edge_ran
                        name=f"ensemble_{s}")
         sc.info["scene_index"] = s
         scenes.append(sc)
-    return Ensemble(scenes, seed)
+    return Ensemble(scenes, seed, [_ensemble_bodies(sc, seed * 1000 + k, max_bodies) for k, sc in enumerate(scenes)]
+                    if bodies else None)
+
+
+def _ensemble_bodies(sc, seed, max_bodies=4):
+    """0..max_bodies rigid spheres or boxes for one ensemble scene: size
+    (sphere diameter or box edge) uniform in [0.12, 0.2] of the box edge,
+    density uniform in [300, 900] kg/m^3, floating half-submerged at the
+    fluid surface in a row along x, each touching the next (so neighbours
+    are in body-body contact), at a seeded depth-wise position, boxes at
+    seeded orientations; boundary spheres of radius h/2 on a 16^3-bucket
+    body grid.  None for a scene drawn with no body."""
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(0, max_bodies + 1))
+    if k == 0:
+        return None
+    h = sc.info["h"]
+    edge = sc.info["box"][0]
+    top = float(sc.pos[:, 1].max())
+    sets, masses, names = [], [], []
+    cz = rng.uniform(0.3 * edge, 0.7 * edge)
+    x = 0.1 * edge
+    for b in range(k):
+        size = edge * rng.uniform(0.12, 0.2)
+        rho = rng.uniform(300.0, 900.0)
+        cx = x + 0.5 * size
+        x += size
+        y = top
+        if rng.random() < 0.5:
+            pts = ball_spheres(0.5 * size, 0.5 * h)
+            masses.append(rho * np.pi * size ** 3 / 6.0)
+            names.append("sphere")
+        else:
+            q = rng.normal(size=4)
+            q /= np.linalg.norm(q)
+            pts = quat_rotate(q, box_spheres(size, 0.5 * h))
+            masses.append(rho * size ** 3)
+            names.append("box")
+        sets.append(pts + np.array([cx, y, cz]))
+    return make_bodies(sets, masses, h, sc.common, names=names, buckets=(16, 16, 16))

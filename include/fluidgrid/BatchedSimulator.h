@@ -24,6 +24,17 @@ struct BatchScene {
     std::vector<float4> velocities;
 };
 
+// Dynamic rigid bodies of one scene: the inputs a single-fluid simulator
+// takes for them (fg_sim_add_bodies).
+struct BatchBodies {
+    std::vector<float4> worldSpheres;  // body b owns [offsets[b], offsets[b+1]); w is ignored
+    std::vector<unsigned> offsets;     // numBodies + 1 entries, offsets[0] = 0
+    std::vector<double> masses;        // kg per body
+    float sphereRadius = 0.0f;
+    Application::CommonRigidBodyParams rigid{};
+    float springForce = 0.0f, dampingForce = 0.0f;
+};
+
 struct BatchTuning {
     int sphBlock = 256;       // workgroup size of density and force kernels
     int sortDigitBits = 8;    // widest radix digit
@@ -59,6 +70,16 @@ public:
     // hash, sort, cells, density, forces+integrate (ms) of the last timed step
     const float* stageTimes() const { return mTimes; }
 
+    // Dynamic rigid bodies of scene s, coupled two-way with that scene's
+    // fluid with the stages and arithmetic of the single-fluid coupled step
+    // (HeterogeneousSimulator with RigidBodySimulator); bodies of different
+    // scenes never interact.  Before the first step, at most once per scene.
+    void addBodies(unsigned scene, const BatchBodies& bodies);
+    unsigned numBodies() const { return mBodyCount; }
+    // Per body, scenes in order: centre of mass, velocity, orientation
+    // (x, y, z, w), angular momentum.  Any pointer may be null.
+    void downloadBodies(float4* pos, float4* vel, float4* orient, float4* L) const;
+
     unsigned numScenes() const { return (unsigned)mDescs.size(); }
     unsigned numParticles() const { return mN; }
     unsigned numBuckets() const { return mBuckets; }
@@ -67,7 +88,10 @@ public:
 
 private:
     void enqueueStep();
+    void enqueueBodyStages(bool before_density);
     void invalidateGraph();
+    void setupBodies();
+    void releaseBodies();
 
     std::vector<fg_scene_desc> mDescs;
     unsigned mN = 0, mBuckets = 0;
@@ -88,6 +112,21 @@ private:
     unsigned long long* d_masks = nullptr;
     void* d_sortWs = nullptr;
     size_t mSortWsBytes = 0;
+
+    // rigid bodies (all scenes; bodies scene-major, spheres in body order)
+    std::vector<BatchBodies> mSceneBodies;
+    bool mBodiesReady = false;
+    unsigned mBodyCount = 0, mSphereCount = 0, mBodyBuckets = 0;
+    void* d_bscenes = nullptr;  // per-scene body descriptors (device)
+    unsigned *d_bodyScene = nullptr, *d_boffsets = nullptr, *d_bkey = nullptr, *d_bid = nullptr;
+    unsigned *d_bstart = nullptr, *d_bend = nullptr;
+    float4 *d_brel = nullptr, *d_bworld = nullptr, *d_bsorted = nullptr, *d_bpf = nullptr, *d_bpt = nullptr;
+    float4 *d_bcm = nullptr, *d_bvel = nullptr, *d_borient = nullptr, *d_bL = nullptr, *d_bforce = nullptr;
+    float4 *d_btorque = nullptr, *d_binv = nullptr, *d_sceneBounds = nullptr, *d_bodyBounds = nullptr;
+    float* d_bmass = nullptr;
+    double *d_bsumF = nullptr, *d_bsumT = nullptr;
+    void* d_bsortWs = nullptr;
+    size_t mBSortWsBytes = 0;
 };
 
 }  // namespace Simulators

@@ -17,6 +17,20 @@ extern "C" {
  * volume_min) and the scene's box, gravity, dt and scale. */
 int fg_bsim_create(unsigned int numScenes, const fg_common_fluid_params* commons, const fg_fluid_params* fluids,
                    const unsigned int* counts, const fg_float4* positions, const fg_float4* velocities);
+/* Dynamic rigid bodies of one scene (the arguments of fg_sim_add_bodies),
+ * coupled two-way with that scene's fluid by the stages and arithmetic of
+ * the single-fluid coupled step: each scene with bodies advances exactly as
+ * a single-fluid simulator given the same fluid and bodies.  Bodies of
+ * different scenes never interact.  Before the first step, once per scene. */
+int fg_bsim_add_bodies(unsigned int scene, const fg_float4* worldSpheres, const unsigned int* bodyOffsets,
+                       unsigned int numBodies, const double* masses, float sphereRadius,
+                       const fg_common_rigid_body_params* rigidParams, float springForce, float dampingForce);
+/* Body count (numBodies, may be NULL) and, for every body with the scenes in
+ * order (the bodies of the first scene given bodies first), centre of mass,
+ * velocity, orientation (x, y, z, w) and angular momentum; arrays may be NULL.
+ * Bodies become part of the batch at its first step: before it the count is 0. */
+int fg_bsim_body_state(unsigned int* numBodies, fg_float4* positions, fg_float4* velocities, fg_float4* orientations,
+                       fg_float4* angularMomenta);
 int fg_bsim_set_options(int use_graph, int stage_timing);
 int fg_bsim_step(int steps);                      /* asynchronous */
 int fg_bsim_synchronize(void);
