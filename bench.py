@@ -138,10 +138,14 @@ def measure_batched(ens, steps, warmup, tune_steps=5):
     time through the single-fluid simulator (the plain form's way to run
     them)."""
     out = {}
+    bodies = ens.bodies or [None] * len(ens.scenes)
     try:
         for label in ("fixed", "tuned"):
             b = lib.BatchSimulator(ens)
             try:
+                for s, bs in enumerate(bodies):
+                    if bs is not None:
+                        b.add_bodies(s, bs)
                 if label == "fixed":
                     b.set_tuning(256, 8, False, False)
                 else:
@@ -175,9 +179,11 @@ def measure_batched(ens, steps, warmup, tune_steps=5):
     # plain form: every scene alone through the single-fluid simulator
     k = max(5, steps // 4)
     total_ms = 0.0
-    for sc in ens.scenes:
+    for sc, bs in zip(ens.scenes, bodies):
         sim = lib.Simulator(sc, fused=True, use_graph=True)
         try:
+            if bs is not None:
+                sim.add_bodies(bs)
             sim.step(2)
             sim.synchronize()
             t0 = time.perf_counter()
@@ -409,7 +415,9 @@ def main():
         }
         del s4
     if not a.no_batch:
-        ens = scenes.ensemble(num_scenes=a.batch_scenes)
+        ens = scenes.ensemble(num_scenes=a.batch_scenes, bodies=True)
+        nbod = sum(0 if x is None else x.b for x in ens.bodies)
+        nsph = sum(0 if x is None else x.m for x in ens.bodies)
         log(f"ensemble: {len(ens.scenes)} scenes, n={ens.n:,}")
         kb = max(20, a.steps // 2)
         rb = measure_batched(ens, kb, max(5, a.warmup // 2))
@@ -417,7 +425,8 @@ def main():
         out["config"]["batched"] = {
             "workload": f"{len(ens.scenes)} independent FP32 dam-break scenes, {ens.n:,} particles, "
                         f"{ens.num_buckets:,} buckets; counts log-uniform in [4096, 65536], edges in [0.5, 1.5] m, "
-                        "fluid only",
+                        f"0-4 floating rigid spheres or boxes per scene ({nbod} bodies, {nsph:,} boundary spheres), "
+                        "each scene coupled two-way with its own bodies",
             "steps": kb,
             "value_fixed": rb["fixed"]["value"], "value_tuned": rb["tuned"]["value"],
             "ms_per_step_fixed": rb["fixed"]["ms_per_step"], "ms_per_step_tuned": rb["tuned"]["ms_per_step"],
