@@ -95,8 +95,10 @@ __global__ void __launch_bounds__(BS)
 
 // The same with each scene's bodies: scenes with bodies take the coupled
 // force expression and contact against their body grid.
+// amdgpu_waves_per_eu(5): 96 VGPRs without scratch (100 and occupancy 4
+// without the hint).
 template <int BS, bool FLAT>
-__global__ void __launch_bounds__(BS)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5)))
     k_batch_forces_coupled(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes,
                            const uint32_t* __restrict__ scene_of, const float4* __restrict__ sp,
                            const float4* __restrict__ sv, const float2* __restrict__ rho_p,

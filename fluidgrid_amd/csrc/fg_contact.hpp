@@ -53,6 +53,35 @@ __device__ __forceinline__ uint32_t contact_key(const float4& sp, const CellXYZ&
     return g.key_base + grid_linear_key(a, b, d, g);
 }
 
+// The reference hash is an XOR of per-axis products, and each axis term
+// depends only on that coordinate and its stencil offset.  For the stencil
+// (1,1,1) the three terms per axis are computed once (9 IEEE divisions
+// instead of 81) and combined per cell; every key is the one hash_ref_xor
+// gives for that cell.
+struct XorAxisTerms {
+    uint32_t x[3], y[3], z[3];
+};
+__device__ __forceinline__ uint32_t xor_axis_term(float p, float cs, float off, uint32_t prime) {
+    return (uint32_t)f2i_sat(rn_div(rn_add(p + cs * off, 5000.0f), cs)) * prime;
+}
+__device__ __forceinline__ XorAxisTerms xor_axis_terms(const float4& sp, const GridDesc& g) {
+    XorAxisTerms t;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float o = (float)(k - 1);
+        t.x[k] = xor_axis_term(sp.x, g.cs.x, o, 73856093u);
+        t.y[k] = xor_axis_term(sp.y, g.cs.y, o, 19349663u);
+        t.z[k] = xor_axis_term(sp.z, g.cs.z, o, 83492791u);
+    }
+    return t;
+}
+__device__ __forceinline__ uint32_t sel3(const uint32_t (&v)[3], int k) {
+    return k < 0 ? v[0] : (k > 0 ? v[2] : v[1]);
+}
+__device__ __forceinline__ uint32_t xor_key(const XorAxisTerms& t, int x, int y, int z, const GridDesc& g) {
+    return g.key_base + ((sel3(t.x, x) ^ sel3(t.y, y) ^ sel3(t.z, z)) % g.nb);
+}
+
 // ProcessCollisionForcesD, K_DeviceFunctions.cu:113-159: the (2r+1)^3
 // stencil in z, y, x order; REF_XOR grids rehash x + k*cs, LINEAR grids
 // step cell coordinates and skip cells outside the grid.  For the usual
@@ -71,10 +100,16 @@ __device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv
     GridDesc gg = g;
     if (XOR_ONLY) gg.mode = FG_KEY_REF_XOR;  // body grids: only the hash is compiled in
     if (!XOR_ONLY && g.mode == FG_KEY_LINEAR) c = grid_cell(sp.x, sp.y, sp.z, g);
-    if (PREFETCH && rel.x == 1 && rel.y == 1 && rel.z == 1) {
+    const bool unit = rel.x == 1 && rel.y == 1 && rel.z == 1;
+    const bool xor_terms = unit && gg.mode == FG_KEY_REF_XOR;
+    XorAxisTerms t{};
+    if (xor_terms) t = xor_axis_terms(sp, gg);
+    if (PREFETCH && unit) {
         uint32_t keys[27], s0[27];
 #pragma unroll
-        for (int k = 0; k < 27; k++) keys[k] = contact_key(sp, c, k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1, gg);
+        for (int k = 0; k < 27; k++)
+            keys[k] = xor_terms ? xor_key(t, k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1, gg)
+                                : contact_key(sp, c, k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1, gg);
 #pragma unroll
         for (int k = 0; k < 27; k++) s0[k] = keys[k] != kEmpty ? start[keys[k]] : kEmpty;
 #pragma unroll
@@ -87,7 +122,7 @@ __device__ __forceinline__ F3 contact_stencil(const float4& sp, const float4& sv
     for (int z = -rel.z; z <= rel.z; z++)
         for (int y = -rel.y; y <= rel.y; y++)
             for (int x = -rel.x; x <= rel.x; x++) {
-                const uint32_t key = contact_key(sp, c, x, y, z, gg);
+                const uint32_t key = xor_terms ? xor_key(t, x, y, z, gg) : contact_key(sp, c, x, y, z, gg);
                 if (key == kEmpty) continue;
                 const uint32_t s0 = start[key];
                 F = add(F, contact_range(sp, sv, sr, cr, cpos, cvel, s0, s0 != kEmpty ? end[key] : 0u, ks, kd, w_index,
