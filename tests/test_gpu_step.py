@@ -133,14 +133,21 @@ def test_trajectory_100_steps_appmain_ref_xor(fglib, orc):
     assert err <= tol
 
 
-@pytest.mark.parametrize("which", ["cfg0", "compressed"])
+@pytest.mark.parametrize("which", ["cfg0", "compressed", "coupled"])
 def test_schedules_and_block_sizes_bit_identical(fglib, which):
     """The flat and row-by-row schedules of the density and force passes,
     at every workgroup size, give the same bits after 10 graph steps
     ('compressed' has rows longer than the mask width, which the flat force
-    schedule hands to the row loop)."""
+    schedule hands to the row loop; 'coupled' runs the coupled force kernels
+    with a torus resting in the surface)."""
+    bodies = None
     if which == "cfg0":
         s = scenes.config_scene(0)
+    elif which == "coupled":
+        s = scenes.small_dam_break(lattice=(24, 10, 24), h=0.02, box=(0.5, 0.5, 0.5))
+        top = float(s.pos[:, 1].max())
+        tor = scenes.torus_spheres(0.06, 0.02, 0.01) + np.array([0.12, top, 0.12])
+        bodies = scenes.make_bodies([tor], [0.5], 0.02, s.common)
     else:
         s = scenes.small_dam_break(lattice=(16, 16, 16), h=0.04)
         s.pos[:, :3] = 0.3 + (s.pos[:, :3] - 0.3) * 0.55
@@ -152,6 +159,8 @@ def test_schedules_and_block_sizes_bit_identical(fglib, which):
             L.fg_set_tuning(blk, 8)
             sim = fglib.Simulator(s, fused=True, use_graph=True)
             try:
+                if bodies is not None:
+                    sim.add_bodies(bodies)
                 sim.step(10)
                 pos, vel = sim.state()
             finally:
