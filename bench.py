@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--fluid-only] [--no-cpu-baseline]
                     [--no-8m] [--no-33m] [--no-batch] [--batch-scenes S] [--no-mesh]
+                    [--only {33m,batched,mesh}]
 
 A step is one pass of the per-timestep pipeline over every particle of a
 synthetic dam-break scene, driven through the C++ HeterogeneousSimulator and
@@ -280,6 +281,53 @@ def cpu_model():
     return None
 
 
+def line_33m(a, fused):
+    """Configuration 4: 33.5M particles, 64 bodies, inflow; the configuration's 100 steps."""
+    s4 = scenes.config4_scene()
+    b4 = s4.info["bodies"]
+    log(f"scene {s4.name}: n={s4.n:,} ({s4.info['active']:,} active), {b4.b} bodies, {b4.m:,} spheres")
+    k4 = min(a.steps, s4.steps)  # the configuration's 100 steps: two inflow layers are emitted
+    r4 = measure(s4, k4, 5, fused)
+    act = s4.info["active"]
+    return {
+        "workload": f"plain FP32 dam break, {act:,} active particles (+{s4.n - act:,} inflow reserve slots), "
+                    f"{s4.num_buckets:,} LINEAR buckets, h=0.003, dt=8e-5, 0.5 m/s inflow through the x-min "
+                    f"face, coupled two-way with {b4.b} rigid bodies (tori and boxes, {b4.m:,} spheres)",
+        "particles": act, "slots": s4.n, "steps": k4, "value": act * k4 / r4["seconds"],
+        "ms_per_step": r4["ms_per_step"],
+        "step_hbm_frac": step_bytes(s4, r4["c_occ"]) / (r4["ms_per_step"] * 1e-3) / HBM_PEAK,
+        "stage_ms": {k: round(v, 5) for k, v in r4["stage_ms"].items()},
+        "finite": r4["finite"], "drift": r4["drift"],
+    }
+
+
+def line_batched(a, plain=None):
+    """Configuration 3: the 256-scene ensemble with its bodies, fixed and tuned, beside the
+    scenes stepped one at a time."""
+    ens = scenes.ensemble(num_scenes=a.batch_scenes, bodies=True)
+    nbod = sum(0 if x is None else x.b for x in ens.bodies)
+    nsph = sum(0 if x is None else x.m for x in ens.bodies)
+    log(f"ensemble: {len(ens.scenes)} scenes, n={ens.n:,}")
+    kb = max(20, a.steps // 2)
+    rb = measure_batched(ens, kb, max(5, a.warmup // 2))
+    out = {
+        "workload": f"{len(ens.scenes)} independent FP32 dam-break scenes, {ens.n:,} particles, "
+                    f"{ens.num_buckets:,} buckets; counts log-uniform in [4096, 65536], edges in [0.5, 1.5] m, "
+                    f"0-4 floating rigid spheres or boxes per scene ({nbod} bodies, {nsph:,} boundary spheres), "
+                    "each scene coupled two-way with its own bodies",
+        "steps": kb,
+        "value_fixed": rb["fixed"]["value"], "value_tuned": rb["tuned"]["value"],
+        "ms_per_step_fixed": rb["fixed"]["ms_per_step"], "ms_per_step_tuned": rb["tuned"]["ms_per_step"],
+        "tuning": rb["tuning"], "stage_ms": rb["stage_ms"], "finite": rb["finite"],
+        "plain_per_scene_value": rb["plain_per_scene"]["value"],
+        "plain_per_scene_ms_per_step": rb["plain_per_scene"]["ms_per_step"],
+        "speedup_vs_plain_per_scene": rb["tuned"]["value"] / rb["plain_per_scene"]["value"],
+    }
+    if plain:
+        out["speedup_vs_plain_1M"] = rb["tuned"]["value"] / plain
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,6 +342,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-sampler line")
+    ap.add_argument("--only", choices=("33m", "batched", "mesh"),
+                    help="run only that line and print it alone (for profiling one workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB ten times (reference for HBM byte counters)")
     a = ap.parse_args()
@@ -314,6 +364,12 @@ def main():
             lib.check(L.fg_copy_bytes(dst.ptr, src.ptr, nbytes), "copy")
         lib.check(L.fg_synchronize(), "sync")
         print(json.dumps({"copy_calibration_bytes": nbytes}))
+        return
+
+    if a.only:
+        fn = {"33m": lambda: line_33m(a, not a.unfused), "batched": lambda: line_batched(a),
+              "mesh": measure_mesh}[a.only]
+        print(json.dumps({"only": a.only, a.only: fn()}), flush=True)
         return
 
     coupled = a.config in (1, 2) and not a.fluid_only
@@ -397,45 +453,9 @@ def main():
                 "finite": r8["finite"], "drift": r8["drift"],
             }
     if not a.no_33m and not a.no_8m and a.config == 1:
-        s4 = scenes.config4_scene()
-        b4 = s4.info["bodies"]
-        log(f"scene {s4.name}: n={s4.n:,} ({s4.info['active']:,} active), {b4.b} bodies, {b4.m:,} spheres")
-        k4 = min(a.steps, s4.steps)  # the configuration's 100 steps: two inflow layers are emitted
-        r4 = measure(s4, k4, 5, fused)
-        act = s4.info["active"]
-        out["config"]["also_33M"] = {
-            "workload": f"plain FP32 dam break, {act:,} active particles (+{s4.n - act:,} inflow reserve slots), "
-                        f"{s4.num_buckets:,} LINEAR buckets, h=0.003, dt=8e-5, 0.5 m/s inflow through the x-min "
-                        f"face, coupled two-way with {b4.b} rigid bodies (tori and boxes, {b4.m:,} spheres)",
-            "particles": act, "slots": s4.n, "steps": k4, "value": act * k4 / r4["seconds"],
-            "ms_per_step": r4["ms_per_step"],
-            "step_hbm_frac": step_bytes(s4, r4["c_occ"]) / (r4["ms_per_step"] * 1e-3) / HBM_PEAK,
-            "stage_ms": {k: round(v, 5) for k, v in r4["stage_ms"].items()},
-            "finite": r4["finite"], "drift": r4["drift"],
-        }
-        del s4
+        out["config"]["also_33M"] = line_33m(a, fused)
     if not a.no_batch:
-        ens = scenes.ensemble(num_scenes=a.batch_scenes, bodies=True)
-        nbod = sum(0 if x is None else x.b for x in ens.bodies)
-        nsph = sum(0 if x is None else x.m for x in ens.bodies)
-        log(f"ensemble: {len(ens.scenes)} scenes, n={ens.n:,}")
-        kb = max(20, a.steps // 2)
-        rb = measure_batched(ens, kb, max(5, a.warmup // 2))
-        plain = out["value"]
-        out["config"]["batched"] = {
-            "workload": f"{len(ens.scenes)} independent FP32 dam-break scenes, {ens.n:,} particles, "
-                        f"{ens.num_buckets:,} buckets; counts log-uniform in [4096, 65536], edges in [0.5, 1.5] m, "
-                        f"0-4 floating rigid spheres or boxes per scene ({nbod} bodies, {nsph:,} boundary spheres), "
-                        "each scene coupled two-way with its own bodies",
-            "steps": kb,
-            "value_fixed": rb["fixed"]["value"], "value_tuned": rb["tuned"]["value"],
-            "ms_per_step_fixed": rb["fixed"]["ms_per_step"], "ms_per_step_tuned": rb["tuned"]["ms_per_step"],
-            "tuning": rb["tuning"], "stage_ms": rb["stage_ms"], "finite": rb["finite"],
-            "plain_per_scene_value": rb["plain_per_scene"]["value"],
-            "plain_per_scene_ms_per_step": rb["plain_per_scene"]["ms_per_step"],
-            "speedup_vs_plain_per_scene": rb["tuned"]["value"] / rb["plain_per_scene"]["value"],
-            "speedup_vs_plain_1M": rb["tuned"]["value"] / plain,
-        }
+        out["config"]["batched"] = line_batched(a, out["value"])
     if not a.no_mesh:
         log("mesh sampler ...")
         out["config"]["mesh_sampler"] = measure_mesh()
