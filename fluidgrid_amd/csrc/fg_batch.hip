@@ -59,8 +59,9 @@ __global__ void __launch_bounds__(256) k_batch_hash(const fg_scene_desc* __restr
 template <int BS, bool FLAT>
 __global__ void __launch_bounds__(BS)
     k_batch_density(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes, const uint32_t* __restrict__ scene_of,
-                    const float4* __restrict__ sp, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
-                    const uint32_t* __restrict__ end, unsigned long long* __restrict__ masks, uint32_t n) {
+                    const float4* __restrict__ sp, const float4* __restrict__ sv, float2* __restrict__ rho_p,
+                    const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                    unsigned long long* __restrict__ masks, uint32_t n) {
     __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
     __shared__ uint32_t s_re[FLAT ? 9 * BS : 1];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(BS)
     FlatRanges fr{s_rb, s_re, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
         const fg_scene_desc& d = scenes[s];
-        density_particle<true>(i, sp, rho_p, start, end, n, d.gasConstant, d.restDensity, d.mass, d.common,
+        density_particle<true>(i, sp, sv, rho_p, start, end, n, d.gasConstant, d.restDensity, d.mass, d.common,
                                d.kernels, d.keyBase, masks, FLAT ? &fr : nullptr);
     });
 }
@@ -172,20 +173,21 @@ int fg_batch_hash(const fg_scene_desc* scenes, unsigned int numScenes, const uns
 }
 
 int fg_batch_density_nbr(const fg_scene_desc* scenes, unsigned int numScenes, const unsigned int* sceneOf,
-                         fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,
-                         unsigned int* gridEnd, unsigned long long* neighborMasks, unsigned int numParticles) {
+                         fg_float4* orderedPositions, const fg_float4* orderedVelocity, fg_float2* mass_pressure,
+                         unsigned int* gridStart, unsigned int* gridEnd, unsigned long long* neighborMasks,
+                         unsigned int numParticles) {
     if (numParticles == 0) return FG_OK;
-    if (!scenes || !sceneOf || !orderedPositions || !mass_pressure || !gridStart || !gridEnd || !neighborMasks ||
+    if (!scenes || !sceneOf || !orderedPositions || !orderedVelocity || !mass_pressure || !gridStart || !gridEnd || !neighborMasks ||
         numScenes == 0) {
         set_error(FG_ERR_INVALID_ARGUMENT, "fg_batch_density_nbr: null pointer or no scenes");
         return FG_ERR_INVALID_ARGUMENT;
     }
     if (state().flat_density)
         FG_BATCH_LAUNCH2(k_batch_density, true, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
-                         FG_F2(mass_pressure), gridStart, gridEnd, neighborMasks, numParticles);
+                         reinterpret_cast<const float4*>(orderedVelocity), FG_F2(mass_pressure), gridStart, gridEnd, neighborMasks, numParticles);
     else
         FG_BATCH_LAUNCH2(k_batch_density, false, numParticles, scenes, numScenes, sceneOf, FG_F4(orderedPositions),
-                         FG_F2(mass_pressure), gridStart, gridEnd, neighborMasks, numParticles);
+                         reinterpret_cast<const float4*>(orderedVelocity), FG_F2(mass_pressure), gridStart, gridEnd, neighborMasks, numParticles);
     return check_launch("fg_batch_density_nbr") ? FG_OK : fg_peek_error();
 }
 

@@ -181,18 +181,20 @@ __global__ void __launch_bounds__(256) k_integrate(float4* __restrict__ pos, con
 
 template <bool MASKS, int BS>
 __global__ void __launch_bounds__(BS)
-    k_density_fast(const float4* __restrict__ sp, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
+    k_density_fast(const float4* __restrict__ sp, const float4* __restrict__ sv, float2* __restrict__ rho_p,
+                   const uint32_t* __restrict__ start,
                    const uint32_t* __restrict__ end, uint32_t n, float k, float rho0, float m,
                    fg_common_fluid_params cp, fg_kernels_cache kc, unsigned long long* __restrict__ masks) {
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
-    density_particle<MASKS>(i, sp, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks);
+    density_particle<MASKS>(i, sp, sv, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks);
 }
 
 // Flat schedule (row table in LDS); same results as k_density_fast<MASKS>.
 template <bool MASKS, int BS>
 __global__ void __launch_bounds__(BS)
-    k_density_flat(const float4* __restrict__ sp, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
+    k_density_flat(const float4* __restrict__ sp, const float4* __restrict__ sv, float2* __restrict__ rho_p,
+                   const uint32_t* __restrict__ start,
                    const uint32_t* __restrict__ end, uint32_t n, float k, float rho0, float m,
                    fg_common_fluid_params cp, fg_kernels_cache kc, unsigned long long* __restrict__ masks) {
     __shared__ uint32_t s_rb[9 * BS];
@@ -200,7 +202,7 @@ __global__ void __launch_bounds__(BS)
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
     FlatRanges fr{s_rb, s_re, (uint32_t)BS, (uint32_t)threadIdx.x};
-    density_particle<MASKS>(i, sp, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks, &fr);
+    density_particle<MASKS>(i, sp, sv, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks, &fr);
 }
 
 // amdgpu_waves_per_eu(8): the coupled variant carries two body-grid
@@ -292,16 +294,16 @@ static bool use_fast() { return state().key_mode == FG_KEY_LINEAR && state().mat
     } while (0)
 
 int launch_density(float4* sp, float2* rho_p, uint32_t* start, uint32_t* end, uint32_t n, float k, float rho0,
-                   float m, unsigned long long* masks = nullptr) {
+                   float m, unsigned long long* masks = nullptr, const float4* sv = nullptr) {
     if (!stencil_covers_h("density pass")) return FG_ERR_UNSUPPORTED;
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
     if (use_fast() && masks && st.flat_density)
-        FG_SPH_LAUNCH(k_density_flat, true, n, sp, rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
+        FG_SPH_LAUNCH(k_density_flat, true, n, sp, sv, rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
     else if (use_fast() && masks)
-        FG_SPH_LAUNCH(k_density_fast, true, n, sp, rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
+        FG_SPH_LAUNCH(k_density_fast, true, n, sp, sv, rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
     else if (use_fast())
-        FG_SPH_LAUNCH(k_density_fast, false, n, sp, rho_p, start, end, n, k, rho0, m, st.common, st.kernels,
+        FG_SPH_LAUNCH(k_density_fast, false, n, sp, (const float4*)nullptr, rho_p, start, end, n, k, rho0, m, st.common, st.kernels,
                       (unsigned long long*)nullptr);
     else if (st.key_mode == FG_KEY_LINEAR)
         hipLaunchKernelGGL(k_density_ref<FG_KEY_LINEAR>, grid, block, 0, st.stream, sp, rho_p, start, end, n, k,
@@ -441,21 +443,23 @@ int fg_sph_forces_integrate(fg_float4* orderedPositions, fg_float4* orderedVeloc
 }
 
 size_t fg_neighbor_mask_bytes(unsigned int numParticles) {
-    // 9 row masks, then (1/rho, p/rho) per particle
-    return (size_t)(9 * sizeof(unsigned long long) + 2 * sizeof(float)) * (size_t)numParticles;
+    // 9 row masks, then (from a 256-byte boundary) a 32-byte neighbour record per particle
+    return nbr_record_offset(numParticles) + 2 * sizeof(float4) * (size_t)numParticles;
 }
 
-int fg_sph_density_nbr(fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,
+int fg_sph_density_nbr(fg_float4* orderedPositions, const fg_float4* orderedVelocity, fg_float2* mass_pressure,
+                       unsigned int* gridStart,
                        unsigned int* gridEnd, unsigned long long* neighborMasks, unsigned int numParticles,
                        float gasConstant, float restDensity, float mass) {
     if (numParticles == 0) return FG_OK;
     if (!have_params("fg_sph_density_nbr")) return FG_ERR_NOT_INITIALIZED;
-    if (!orderedPositions || !mass_pressure || !gridStart || !gridEnd || !neighborMasks) {
+    if (!orderedPositions || !orderedVelocity || !mass_pressure || !gridStart || !gridEnd || !neighborMasks) {
         set_error(FG_ERR_INVALID_ARGUMENT, "fg_sph_density_nbr: null pointer");
         return FG_ERR_INVALID_ARGUMENT;
     }
     return launch_density(FG_F4(orderedPositions), FG_F2(mass_pressure), gridStart, gridEnd, numParticles,
-                          gasConstant, restDensity, mass, neighborMasks);
+                          gasConstant, restDensity, mass, neighborMasks,
+                          reinterpret_cast<const float4*>(orderedVelocity));
 }
 
 int fg_sph_forces_integrate_nbr(fg_float4* orderedPositions, fg_float4* orderedVelocity,

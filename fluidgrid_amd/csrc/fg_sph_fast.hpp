@@ -146,6 +146,22 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
 // Rows longer than 64 particles are not encoded and are re-tested.
 constexpr uint32_t kRowMaskBits = 64;
 
+// After the 9 row masks, at the next 256-byte boundary, the neighbour records
+// the force pass reads for every neighbour j: two float4 per particle,
+// (x_j, y_j, z_j, 1/rho_j) and (vx_j, vy_j, vz_j, p_j/rho_j), all from the
+// sorted arrays.  One 32-byte record takes two 16-byte loads from one cache
+// line, where position, velocity and (1/rho, p/rho) read from three arrays
+// took three loads touching three lines.
+__host__ __device__ __forceinline__ size_t nbr_record_offset(uint32_t n) {
+    return ((size_t)9 * sizeof(unsigned long long) * n + 255) & ~(size_t)255;
+}
+__device__ __forceinline__ float4* nbr_records(unsigned long long* masks, uint32_t n) {
+    return reinterpret_cast<float4*>(reinterpret_cast<char*>(masks) + nbr_record_offset(n));
+}
+__device__ __forceinline__ const float4* nbr_records(const unsigned long long* masks, uint32_t n) {
+    return reinterpret_cast<const float4*>(reinterpret_cast<const char*>(masks) + nbr_record_offset(n));
+}
+
 // xyz of a float4 array element as one 12-byte load (the w of a neighbour
 // is not used by the pair terms).
 __device__ __forceinline__ float3 ld_xyz(const float4* p) {
@@ -178,16 +194,20 @@ __device__ __forceinline__ uint32_t density4(const float4& p, const float3& q0, 
 #pragma clang fp contract(fast)
 
 // rho_p[i] = (max(1, rho), k (rho - rho0)) (K_FluidSim.cu:246-249) and, when
-// inv is given, inv[i] = (1/rho_p.x, rho_p.y/rho_p.x).
+// rec is given, particle i's neighbour record from its sorted position p,
+// sorted velocity v and (1/rho_p.x, rho_p.y/rho_p.x).
 #pragma clang fp contract(off)
+__device__ __forceinline__ void store_record(uint32_t i, float2 rp, const float4& p, const float4& v,
+                                             float4* __restrict__ rec) {
+    const float a = 1.0f / rp.x;
+    rec[2 * (size_t)i] = make_float4(p.x, p.y, p.z, a);
+    rec[2 * (size_t)i + 1] = make_float4(v.x, v.y, v.z, rp.y * a);
+}
 __device__ __forceinline__ void store_density(uint32_t i, float rho, float k, float rho0, float2* __restrict__ rho_p,
-                                              float2* __restrict__ inv) {
+                                              float4* __restrict__ rec, const float4& p, const float4& v) {
     const float2 rp = make_float2(fmaxf(1.0f, rho), k * (rho - rho0));
     rho_p[i] = rp;
-    if (inv) {
-        const float a = 1.0f / rp.x;
-        inv[i] = make_float2(a, rp.y * a);
-    }
+    if (rec) store_record(i, rp, p, v, rec);
 }
 #pragma clang fp contract(fast)
 
@@ -202,7 +222,8 @@ struct FlatRanges {
 };
 
 // Density of sorted particle i (i < n): rho_p[i] = (max(1, rho), k (rho - rho0)),
-// and, with MASKS, its row masks.
+// and, with MASKS, its row masks and its neighbour record (sv, the sorted
+// velocities, is read only then).
 //
 // Two schedules give bit-identical sums (rows in order, candidates of a row
 // in groups of four in sorted order, a short last group padded with far
@@ -211,22 +232,18 @@ struct FlatRanges {
 // lane walks its own candidate stream across rows.
 template <bool MASKS>
 __device__ __forceinline__ void density_particle(uint32_t i, const float4* __restrict__ sp,
-                                                 float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
+                                                 const float4* __restrict__ sv, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
                                                  const uint32_t* __restrict__ end, uint32_t n, float k, float rho0,
                                                  float m, const fg_common_fluid_params& cp,
                                                  const fg_kernels_cache& kc, uint32_t key_base,
                                                  unsigned long long* __restrict__ masks,
                                                  FlatRanges* flat = nullptr) {
     const float4 p = sp[i];
-    // Per-particle terms the force pass reads for every neighbour j, stored
-    // after the 9 row masks: (1/rho_j, p_j/rho_j) from the stored (rho, p).
-    float2* inv = MASKS ? reinterpret_cast<float2*>(masks + (size_t)9 * n) : nullptr;
+    float4* rec = MASKS ? nbr_records(masks, n) : nullptr;
+    const float4 v = MASKS ? sv[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (!(p.w > 0.0f)) {
-        if (MASKS) {
-            const float2 rp = rho_p[i];  // left as it is; neighbours still read it
-            const float a = 1.0f / rp.x;
-            inv[i] = make_float2(a, rp.y * a);
-        }
+        // (rho, p) is left as it is; neighbours still read it
+        if (MASKS) store_record(i, rho_p[i], p, v, rec);
         return;
     }
     // Distances are compared unscaled: r_u = r / s, so r < h <=> r_u^2 < (h/s)^2
@@ -271,7 +288,7 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
             if (MASKS) msk |= (unsigned long long)nib << ((j - rb) & 63u);
             j += 4;
         }
-        store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * acc, k, rho0, rho_p, inv);
+        store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * acc, k, rho0, rho_p, rec, p, v);
         return;
     }
 #pragma unroll
@@ -312,7 +329,7 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
         // Rows longer than 64 particles are never read through their mask.
         if (MASKS) masks[(size_t)row * n + i] = msk;
     }
-    store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * acc, k, rho0, rho_p, inv);
+    store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * acc, k, rho0, rho_p, rec, p, v);
 }
 
 // Pressure and viscosity terms of one neighbour j (r < h checked again, so
@@ -416,7 +433,7 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
         const float2 dpi = rho_p[i];
         const RowWalker w = make_rows(pos, cp);
         PairAcc A{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        const float2* inv = MASKS ? reinterpret_cast<const float2*>(masks + (size_t)9 * n) : nullptr;
+        const float4* rec = MASKS ? nbr_records(masks, n) : nullptr;
         bool rowloop = true;
         if (MASKS && flat) {
             bool fits = true;
@@ -461,9 +478,11 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                     const uint32_t jn = next();
                     const bool two = jn != kEmpty;
                     const uint32_t j2 = two ? jn : j1;
-                    const float3 q1 = ld_xyz(sp + j1), q2 = ld_xyz(sp + j2);
-                    const float3 v1 = ld_xyz(sv + j1), v2 = ld_xyz(sv + j2);
-                    const float2 d1 = inv[j1], d2 = inv[j2];
+                    const float4 a1 = rec[2 * (size_t)j1], b1 = rec[2 * (size_t)j1 + 1];
+                    const float4 a2 = rec[2 * (size_t)j2], b2 = rec[2 * (size_t)j2 + 1];
+                    const float3 q1 = make_float3(a1.x, a1.y, a1.z), q2 = make_float3(a2.x, a2.y, a2.z);
+                    const float3 v1 = make_float3(b1.x, b1.y, b1.z), v2 = make_float3(b2.x, b2.y, b2.z);
+                    const float2 d1 = make_float2(a1.w, b1.w), d2 = make_float2(a2.w, b2.w);
                     asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(v2.x), "v"(v2.y), "v"(v2.z), "v"(d2.x),
                                  "v"(d2.y));
                     pair_term_masked(A, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s);
@@ -500,9 +519,11 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                         const uint32_t k2 = two ? (uint32_t)__builtin_ctz(word) : k1;
                         word &= word - 1u;  // (0 stays 0)
                         const uint32_t j1 = base + k1, j2 = base + k2;
-                        const float3 q1 = ld_xyz(sp + j1), q2 = ld_xyz(sp + j2);
-                        const float3 v1 = ld_xyz(sv + j1), v2 = ld_xyz(sv + j2);
-                        const float2 d1 = inv[j1], d2 = inv[j2];
+                        const float4 a1 = rec[2 * (size_t)j1], b1 = rec[2 * (size_t)j1 + 1];
+                        const float4 a2 = rec[2 * (size_t)j2], b2 = rec[2 * (size_t)j2 + 1];
+                        const float3 q1 = make_float3(a1.x, a1.y, a1.z), q2 = make_float3(a2.x, a2.y, a2.z);
+                        const float3 v1 = make_float3(b1.x, b1.y, b1.z), v2 = make_float3(b2.x, b2.y, b2.z);
+                        const float2 d1 = make_float2(a1.w, b1.w), d2 = make_float2(a2.w, b2.w);
                         // Keeps the second pair's loads beside the first pair's (one
                         // memory round trip per iteration): without a use here the
                         // compiler sinks them into the `two` branch below.

@@ -367,8 +367,9 @@ void BatchedSimulator::enqueueStep() {
                                   reinterpret_cast<fg_float4*>(d_svel), mN, mBuckets);
     if (mBodyCount) enqueueBodyStages(true);
     if (mTiming) (void)hipEventRecord(mEv[3], mStream);
-    fg_batch_density_nbr(d_descs, S, d_sceneOf, reinterpret_cast<fg_float4*>(d_spos), reinterpret_cast<fg_float2*>(d_rho),
-                         d_start, d_end, d_masks, mN);
+    fg_batch_density_nbr(d_descs, S, d_sceneOf, reinterpret_cast<fg_float4*>(d_spos),
+                         reinterpret_cast<const fg_float4*>(d_svel), reinterpret_cast<fg_float2*>(d_rho), d_start,
+                         d_end, d_masks, mN);
     if (mTiming) (void)hipEventRecord(mEv[4], mStream);
     if (mBodyCount) {
         fg::batch_body_forces(d_bworld, d_bvel, d_bcm, d_bpf, d_bpt, mSphereCount, mBodyCount, d_bodyScene,

@@ -142,6 +142,7 @@ void HeterogeneousSimulator::calculateFluidDensity(Fluid::GPUSPHFluid* fluid) {
     if (mOptions.fused) {
         // Same density, plus the neighbour bitmasks the force pass reuses.
         fg_sph_density_nbr(reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
+                           reinterpret_cast<const fg_float4*>(fluid->getOrderedVelocities()),
                            reinterpret_cast<fg_float2*>(fluid->getDensityField()), fluid->getHashStart(),
                            fluid->getHashEnd(), fluid->getNeighborMasks(), (unsigned)fluid->getNumParticles(),
                            fluid->getGasConstant(), fluid->getRestDensity(), fluid->getMass());

@@ -55,7 +55,7 @@ SIGNATURES = {
     "fg_calc_hash_key_only": (i32, [vp, vp, u32, u32, abi.Float3]),
     "fg_sph_forces_integrate": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32]),
     "fg_neighbor_mask_bytes": (sz, [u32]),
-    "fg_sph_density_nbr": (i32, [vp, vp, vp, vp, vp, u32, f32, f32, f32]),
+    "fg_sph_density_nbr": (i32, [vp, vp, vp, vp, vp, vp, u32, f32, f32, f32]),
     "fg_sph_forces_integrate_nbr": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32]),
     "fg_device_count": (i32, [C.POINTER(C.c_int)]),
     "fg_device_name": (i32, [i32, C.c_char_p, sz]),
@@ -133,7 +133,7 @@ SIGNATURES = {
     "fg_set_schedule": (i32, [i32, i32]),
     "fg_get_schedule": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fg_batch_hash": (i32, [vp, u32, vp, vp, vp, u32]),
-    "fg_batch_density_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, u32]),
+    "fg_batch_density_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, u32]),
     "fg_batch_forces_integrate_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
     "fg_bsim_create": (i32, [u32, vp, vp, vp, vp, vp]),
     "fg_bsim_add_bodies": (i32, [u32, vp, vp, u32, vp, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),

@@ -316,14 +316,15 @@ int fg_sph_forces_integrate(fg_float4* orderedPositions, fg_float4* orderedVeloc
  * with FG_MATH_FAST; other modes ignore the masks).  The density launch
  * records, for each of a particle's 9 stencil rows, which particles of the
  * row lie within h; the force launch then evaluates only those pairs.
- * neighborMasks holds fg_neighbor_mask_bytes(n) bytes (the 9 row masks of
- * every particle, then (1/rho, p/rho) of every particle, which the force
- * launch reads for each neighbour); the force launch must follow the density
- * launch on the same sorted arrays.  pos may be NULL
- * (no integration). */
+ * neighborMasks holds fg_neighbor_mask_bytes(n) bytes: the 9 row masks of
+ * every particle, then, from the next 256-byte boundary, a 32-byte record of
+ * every particle, (x, y, z, 1/rho) and (vx, vy, vz, p/rho) from the sorted
+ * positions and velocities, which the force launch reads for each neighbour
+ * with two 16-byte loads.  The force launch must follow the density launch on
+ * the same sorted arrays.  pos may be NULL (no integration). */
 size_t fg_neighbor_mask_bytes(unsigned int numParticles);
-int fg_sph_density_nbr(fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,
-                       unsigned int* gridEnd, unsigned long long* neighborMasks, unsigned int numParticles,
+int fg_sph_density_nbr(fg_float4* orderedPositions, const fg_float4* orderedVelocity, fg_float2* mass_pressure,
+                       unsigned int* gridStart, unsigned int* gridEnd, unsigned long long* neighborMasks, unsigned int numParticles,
                        float gasConstant, float restDensity, float mass);
 int fg_sph_forces_integrate_nbr(fg_float4* orderedPositions, fg_float4* orderedVelocity,
                                 fg_float2* density_pressure, unsigned int* gridStart, unsigned int* gridEnd,
@@ -495,8 +496,9 @@ int fg_batch_hash(const fg_scene_desc* scenes, unsigned int numScenes, const uns
                   const fg_float4* positions, unsigned int* hashKey, unsigned int numParticles);
 /* fg_sph_density_nbr per scene (neighbour masks always written). */
 int fg_batch_density_nbr(const fg_scene_desc* scenes, unsigned int numScenes, const unsigned int* sceneOf,
-                         fg_float4* orderedPositions, fg_float2* mass_pressure, unsigned int* gridStart,
-                         unsigned int* gridEnd, unsigned long long* neighborMasks, unsigned int numParticles);
+                         fg_float4* orderedPositions, const fg_float4* orderedVelocity, fg_float2* mass_pressure,
+                         unsigned int* gridStart, unsigned int* gridEnd, unsigned long long* neighborMasks,
+                         unsigned int numParticles);
 /* fg_sph_forces_integrate_nbr per scene (pos may be NULL: no integration). */
 int fg_batch_forces_integrate_nbr(const fg_scene_desc* scenes, unsigned int numScenes,
                                   const unsigned int* sceneOf, fg_float4* orderedPositions, fg_float4* orderedVelocity,

@@ -151,7 +151,7 @@ def test_cells_smaller_than_h_are_refused():
     L.gInitializeKernelParams(s.kernels)
     p = C.c_void_p(256)
     try:
-        rc = L.fg_sph_density_nbr(p, p, p, p, p, 64, 3.0, 1000.0, 1.0)
+        rc = L.fg_sph_density_nbr(p, p, p, p, p, p, 64, 3.0, 1000.0, 1.0)
         assert rc == 5  # FG_ERR_UNSUPPORTED
         assert b"smaller than the kernel radius" in L.fg_last_error_string()
         assert L.fg_last_error() == 5

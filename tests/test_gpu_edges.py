@@ -85,6 +85,6 @@ def test_no_particles(fglib):
     fglib.check(None, "forces")
     L.gSPHIntegrate(one.ptr, one.ptr, None, 0, rp.ptr, start.ptr, end.ptr)
     fglib.check(None, "integrate")
-    fglib.check(L.fg_sph_density_nbr(one.ptr, rp.ptr, start.ptr, end.ptr, None, 0, 3.0, 1000.0, 1.0), "density_nbr")
+    fglib.check(L.fg_sph_density_nbr(one.ptr, one.ptr, rp.ptr, start.ptr, end.ptr, None, 0, 3.0, 1000.0, 1.0), "density_nbr")
     fglib.check(L.fg_sph_forces_integrate_nbr(one.ptr, one.ptr, rp.ptr, start.ptr, end.ptr, None, one.ptr, one.ptr,
                                               u.ptr, 0, 1.0, 1.0), "forces_nbr")

@@ -116,7 +116,7 @@ def test_coupled_fluid_forces_match_oracle(fglib, orc):
                                            rp=np.zeros((s.n, 2), np.float32), bpos=bpos, bvel=bvel, bstart=bstart,
                                            bend=bend).items()}
     masks = fglib.DeviceArray((L.fg_neighbor_mask_bytes(s.n),), np.uint8)
-    fglib.check(L.fg_sph_density_nbr(d["spos"].ptr, d["rp"].ptr, d["start"].ptr, d["end"].ptr, masks.ptr, s.n,
+    fglib.check(L.fg_sph_density_nbr(d["spos"].ptr, d["svel"].ptr, d["rp"].ptr, d["start"].ptr, d["end"].ptr, masks.ptr, s.n,
                                      s.fluid.gasConstant, s.fluid.restDensity, s.fluid.mass), "density")
     rp = d["rp"].numpy()
     args = (start, end, sp, s.fluid, s.common, s.kernels, s.key_mode, b.fluid_radius, (1, 1, 1), b.sphere_radius,

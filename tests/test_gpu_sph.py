@@ -175,7 +175,7 @@ def test_neighbor_mask_path_matches_oracle(fglib, orc, which):
     d = {k: fglib.DeviceArray.from_numpy(v) for k, v in dict(spos=spos, svel=svel, start=start, end=end, sp=sp,
                                                              rp=rho_init, vel=s.vel).items()}
     masks = fglib.DeviceArray((L.fg_neighbor_mask_bytes(s.n),), np.uint8)
-    fglib.check(L.fg_sph_density_nbr(d["spos"].ptr, d["rp"].ptr, d["start"].ptr, d["end"].ptr, masks.ptr, s.n,
+    fglib.check(L.fg_sph_density_nbr(d["spos"].ptr, d["svel"].ptr, d["rp"].ptr, d["start"].ptr, d["end"].ptr, masks.ptr, s.n,
                                      s.fluid.gasConstant, s.fluid.restDensity, s.fluid.mass), "density_nbr")
     rp = d["rp"].numpy()
     active = spos[:, 3] > 0

@@ -95,7 +95,7 @@ def test_static_coupled_forces_match_oracle(fglib, orc, with_dynamic):
                                            rp=np.zeros((s.n, 2), np.float32), bpos=bpos, bvel=bvel, bstart=bstart,
                                            bend=bend, stpos=stpos, ststart=ststart, stend=stend).items()}
     masks = fglib.DeviceArray((L.fg_neighbor_mask_bytes(s.n),), np.uint8)
-    fglib.check(L.fg_sph_density_nbr(d["spos"].ptr, d["rp"].ptr, d["start"].ptr, d["end"].ptr, masks.ptr, s.n,
+    fglib.check(L.fg_sph_density_nbr(d["spos"].ptr, d["svel"].ptr, d["rp"].ptr, d["start"].ptr, d["end"].ptr, masks.ptr, s.n,
                                      s.fluid.gasConstant, s.fluid.restDensity, s.fluid.mass), "density")
     rp = d["rp"].numpy()
     st = (stpos, ststart, stend, stgrid, b.sphere_radius)
