@@ -96,8 +96,8 @@ __global__ void __launch_bounds__(BS)
 
 // The same with each scene's bodies: scenes with bodies take the coupled
 // force expression and contact against their body grid.
-// amdgpu_waves_per_eu(5): 96 VGPRs without scratch (100 and occupancy 4
-// without the hint).
+// amdgpu_waves_per_eu(5): 94-95 VGPRs without scratch (occupancy 4 without
+// the hint; fitted to 6 waves it spills 17 VGPRs to scratch).
 template <int BS, bool FLAT>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5)))
     k_batch_forces_coupled(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes,
