@@ -127,6 +127,14 @@ KEY_LINEAR = 1
 MATH_FAST = 0
 MATH_REFERENCE = 1
 
+# fg_status codes (include/fluidgrid_amd.h)
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_HIP = 2
+ERR_OUT_OF_MEMORY = 3
+ERR_NOT_INITIALIZED = 4
+ERR_UNSUPPORTED = 5
+
 assert C.sizeof(CommonFluidParams) == 104
 assert C.sizeof(FluidParams) == 28
 assert C.sizeof(KernelsCache) == 12

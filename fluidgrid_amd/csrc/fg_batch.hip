@@ -56,6 +56,24 @@ __global__ void __launch_bounds__(256) k_batch_hash(const fg_scene_desc* __restr
     });
 }
 
+// The 27-cell stencil holds every neighbour only when each cell edge is at
+// least the kernel radius.  The scene descriptors live in device memory, so
+// the launchers cannot check them on the host (BatchedSimulator does, when
+// it builds them): a scene with smaller cells gets NaN densities and
+// velocities instead of sums that silently miss neighbours.
+__device__ __forceinline__ bool scene_stencil_covers_h(const fg_common_fluid_params& cp) {
+    const float s = cp.simulation_scale;
+    return linear_reach(cp.kernel_radius, s, cp.cell_size.x) <= 1 &&
+           linear_reach(cp.kernel_radius, s, cp.cell_size.y) <= 1 &&
+           linear_reach(cp.kernel_radius, s, cp.cell_size.z) <= 1;
+}
+
+__device__ __forceinline__ void refuse_scene_particle(uint32_t i, const uint32_t* __restrict__ id, uint32_t n,
+                                                      float4* __restrict__ vout) {
+    const uint32_t o = id[i];
+    if (o < n) vout[o] = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.0f);
+}
+
 template <int BS, bool FLAT>
 __global__ void __launch_bounds__(BS)
     k_batch_density(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes, const uint32_t* __restrict__ scene_of,
@@ -69,6 +87,10 @@ __global__ void __launch_bounds__(BS)
     FlatRanges fr{s_rb, s_re, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
         const fg_scene_desc& d = scenes[s];
+        if (!scene_stencil_covers_h(d.common)) {
+            rho_p[i] = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+            return;
+        }
         density_particle<true>(i, sp, sv, rho_p, start, end, n, d.gasConstant, d.restDensity, d.mass, d.common,
                                d.kernels, d.keyBase, masks, FLAT ? &fr : nullptr);
     });
@@ -89,6 +111,7 @@ __global__ void __launch_bounds__(BS)
     FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
         const fg_scene_desc& d = scenes[s];
+        if (!scene_stencil_covers_h(d.common)) return refuse_scene_particle(i, id, n, vout);
         forces_particle<true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, d.mass, d.viscosity, d.common,
                               d.kernels, d.keyBase, masks, none, FLAT ? &fr : nullptr);
     });
@@ -114,6 +137,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5)))
     FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
         const fg_scene_desc& d = scenes[s];
+        if (!scene_stencil_covers_h(d.common)) return refuse_scene_particle(i, id, n, vout);
         Contacts c{};
         c.dyn = bodies[s].dyn;
         c.coupled = c.dyn.enabled;

@@ -1,9 +1,12 @@
 // fg_emit.hip -- inflow emission of one fluid (IEmitter::update, whose
 // reference implementation is a no-op, Emitter.h:46).  The particle array
-// holds a reserve of inactive slots (w <= 0); every period = spacing / |v|
-// of simulated time the next nu x nv layer of reserve slots is activated on a
-// lattice in the inflow window, with the inflow velocity.  The step index
-// lives on the device, so the emission is part of a captured step graph.
+// holds a reserve of inactive slots (w <= 0); every period of simulated time
+// the next nu x nv layer of reserve slots is activated on a lattice in the
+// inflow window, with the inflow velocity.  Velocities are in simulation
+// units and a step moves a particle by v dt / simulation_scale world units
+// (K_FluidSim.cu:433-438), so a layer has travelled `spacing` world units
+// after period = spacing * simulation_scale / speed.  The step index lives
+// on the device, so the emission is part of a captured step graph.
 #include "fg_internal.hpp"
 
 namespace fg {
@@ -15,12 +18,12 @@ __device__ __forceinline__ uint64_t layers_before(uint64_t k, double dt, double 
 
 #pragma clang fp contract(off)
 __global__ void __launch_bounds__(256) k_emit_inflow(float4* __restrict__ pos, float4* __restrict__ vel,
-                                                     const uint32_t* __restrict__ step, fg_inflow_desc d, float dt) {
+                                                     const uint32_t* __restrict__ step, fg_inflow_desc d, float dt,
+                                                     double period) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t per = d.nu * d.nv;
     if (t >= per) return;
     const uint64_t k = step[0];
-    const double period = (double)d.spacing / (double)d.speed;
     const uint64_t l0 = layers_before(k, dt, period), l1 = layers_before(k + 1, dt, period);
     if (l1 == l0) return;  // no layer this step
     const uint64_t slot = (uint64_t)d.firstSlot + l0 * per + t;
@@ -41,6 +44,11 @@ using namespace fg;
 
 extern "C" {
 
+double fg_inflow_period(const fg_inflow_desc* desc, float simulationScale) {
+    if (!desc || !(desc->speed > 0.0f)) return 0.0;
+    return (double)desc->spacing * (double)simulationScale / (double)desc->speed;
+}
+
 int fg_emit_inflow(fg_float4* positions, fg_float4* velocities, unsigned int* stepCounter, const fg_inflow_desc* desc,
                    float dt) {
     if (!positions || !velocities || !stepCounter || !desc) {
@@ -56,8 +64,22 @@ int fg_emit_inflow(fg_float4* positions, fg_float4* velocities, unsigned int* st
         return FG_ERR_INVALID_ARGUMENT;
     }
     const State& st = state();
+    if (!st.have_common) {
+        set_error(FG_ERR_NOT_INITIALIZED, "fg_emit_inflow: call gInitializeCommonFluidParams first");
+        return FG_ERR_NOT_INITIALIZED;
+    }
+    const double period = fg_inflow_period(desc, st.common.simulation_scale);
+    // One layer at most per step: with dt > period a step would owe several
+    // layers, all at the window, so such a step is refused.
+    if (!(period > 0.0) || (double)dt > period) {
+        set_error(FG_ERR_INVALID_ARGUMENT,
+                  "fg_emit_inflow: dt %g exceeds the layer period %g (spacing * simulation_scale / speed); "
+                  "lower dt or the speed, or widen the spacing",
+                  (double)dt, period);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
     hipLaunchKernelGGL(k_emit_inflow, dim3(grid_for(per, 256)), dim3(256), 0, st.stream, FG_F4(positions),
-                       FG_F4(velocities), stepCounter, d, dt);
+                       FG_F4(velocities), stepCounter, d, dt, period);
     hipLaunchKernelGGL(k_step_counter, dim3(1), dim3(1), 0, st.stream, stepCounter);
     return check_launch("fg_emit_inflow") ? FG_OK : fg_peek_error();
 }

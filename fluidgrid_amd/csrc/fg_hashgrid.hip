@@ -107,6 +107,12 @@ void gCalcHashKey(unsigned int* hashKey, unsigned int* particleID, fg_float4* po
         set_error(FG_ERR_INVALID_ARGUMENT, "gCalcHashKey: null pointer");
         return;
     }
+    if (!(cell_size.x > 0.0f && cell_size.y > 0.0f && cell_size.z > 0.0f)) {
+        set_error(FG_ERR_INVALID_ARGUMENT,
+                  "gCalcHashKey: cell_size (%.9g, %.9g, %.9g) must be positive (numParticles=%u numBuckets=%u)",
+                  (double)cell_size.x, (double)cell_size.y, (double)cell_size.z, numParticles, numBuckets);
+        return;
+    }
     const State& st = state();
     GridDesc g = fluid_grid_desc(st.common, cell_size, numBuckets, st.key_mode);
     if (!validate_grid(g, "gCalcHashKey")) return;

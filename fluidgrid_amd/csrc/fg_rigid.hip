@@ -591,7 +591,18 @@ void gSPHCalculateForcesWithSRB(fg_float4* oldPositions, fg_float4* oldVelocity,
                          _in_srb_num_buckets == 0)) ||
         (useDynamicRB && (!_in_rb_positions || !_in_rb_velocities || !_in_rb_start_offset || !_in_rb_end_offset ||
                           _in_rb_num_buckets == 0))) {
-        set_error(FG_ERR_INVALID_ARGUMENT, "gSPHCalculateForcesWithSRB: body grid missing");
+        // The message echoes every by-value scalar of the call, so a caller
+        // built against the reference prototypes can check what arrived.
+        set_error(FG_ERR_INVALID_ARGUMENT,
+                  "gSPHCalculateForcesWithSRB: body grid missing (numParticles=%u mass=%.9g kViscosity=%.9g "
+                  "sph_particle_radius=%.9g cell_size_relation=(%d,%d,%d) srb_particle_radius=%.9g "
+                  "srb_cell_size=(%.9g,%.9g,%.9g) kspring=%.9g kdamping=%.9g srb_num_buckets=%u rb_num_buckets=%u "
+                  "useStaticRB=%d useDynamicRB=%d)",
+                  numParticles, (double)mass, (double)kViscosity, (double)_in_sph_particle_radius,
+                  _in_cell_size_relation.x, _in_cell_size_relation.y, _in_cell_size_relation.z,
+                  (double)_in_srb_particle_radius, (double)_in_srb_cell_size.x, (double)_in_srb_cell_size.y,
+                  (double)_in_srb_cell_size.z, (double)_in_kspring, (double)_in_kdamping, _in_srb_num_buckets,
+                  _in_rb_num_buckets, (int)useStaticRB, (int)useDynamicRB);
         return;
     }
     Contacts c{};

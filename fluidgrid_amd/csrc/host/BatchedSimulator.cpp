@@ -75,6 +75,15 @@ BatchedSimulator::BatchedSimulator(const std::vector<BatchScene>& scenes) {
             throw std::invalid_argument("BatchedSimulator: scene " + std::to_string(s) + " has an empty grid");
         if (sc.velocities.size() != sc.positions.size())
             throw std::invalid_argument("BatchedSimulator: positions and velocities differ in length");
+        // The density and force passes walk the 27 cells around a particle's
+        // own, which hold every neighbour only when each cell edge is at
+        // least the kernel radius (in world units, h / simulation_scale).
+        const float hw = cp.kernel_radius / cp.simulation_scale;
+        auto reach = [hw](float cell) { return std::ceil(hw / cell); };  // the launchers' linear_reach
+        if (reach(cp.cell_size.x) > 1.0f || reach(cp.cell_size.y) > 1.0f || reach(cp.cell_size.z) > 1.0f)
+            throw UnsupportedScene("BatchedSimulator: scene " + std::to_string(s) +
+                                        " has cells smaller than the kernel radius (the 27-cell stencil would miss "
+                                        "neighbours)");
         const uint64_t cb = (uint64_t)cp.bucket_size.x * cp.bucket_size.y * cp.bucket_size.z;
         fg_scene_desc d{};
         d.common = Application::abi_cast<fg_common_fluid_params>(cp);

@@ -120,6 +120,11 @@ void GPUSPHFluid::setInflow(const fg_inflow_desc& desc) {
         throw std::invalid_argument("GPUSPHFluid::setInflow: reserve slots beyond max_particles");
     if (desc.nu == 0 || desc.nv == 0 || !(desc.spacing > 0.0f) || !(desc.speed > 0.0f))
         throw std::invalid_argument("GPUSPHFluid::setInflow: empty window or non-positive spacing/speed");
+    // At most one layer is due per step (fg_emit_inflow refuses dt > period).
+    const Application::CommonFluidParams& cp = Application::ApplicationData::instance()->getCommonFluidParams();
+    if ((double)cp.deltaTimeIntegration > fg_inflow_period(&desc, cp.simulation_scale))
+        throw std::invalid_argument("GPUSPHFluid::setInflow: the time step exceeds the layer period "
+                                    "spacing * simulation_scale / speed");
     mInflow = desc;
     if (!d_emitStep && hipMalloc(reinterpret_cast<void**>(&d_emitStep), sizeof(unsigned int)) != hipSuccess)
         throw std::runtime_error("GPUSPHFluid::setInflow: hipMalloc failed");

@@ -15,8 +15,13 @@ std::unique_ptr<Simulators::BatchedSimulator>& batch() {
 }
 
 int fail(const std::exception& e) {
-    fg_record_error(FG_ERR_HIP, e.what());
-    return FG_ERR_HIP;
+    int code = FG_ERR_HIP;
+    if (dynamic_cast<const Simulators::UnsupportedScene*>(&e))
+        code = FG_ERR_UNSUPPORTED;
+    else if (dynamic_cast<const std::invalid_argument*>(&e))
+        code = FG_ERR_INVALID_ARGUMENT;
+    fg_record_error(code, e.what());
+    return code;
 }
 }  // namespace
 

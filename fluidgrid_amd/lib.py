@@ -86,6 +86,7 @@ SIGNATURES = {
     "fg_sim_conservation": (i32, [i32, vp]),
     "fg_sim_set_inflow": (i32, [i32, C.POINTER(abi.InflowDesc)]),
     "fg_emit_inflow": (i32, [vp, vp, vp, C.POINTER(abi.InflowDesc), f32]),
+    "fg_inflow_period": (C.c_double, [C.POINTER(abi.InflowDesc), f32]),
     "fg_conservation_workspace_size": (sz, []),
     "fg_conservation_f64": (i32, [vp, vp, vp, vp, u32, f32, f32, f32, vp, vp, sz]),
     "fg_sim_shutdown": (i32, []),

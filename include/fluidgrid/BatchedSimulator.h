@@ -10,12 +10,19 @@
 // no host synchronisation and is replayed as a captured HIP graph.
 #pragma once
 
+#include <stdexcept>
 #include <vector>
 
 #include "ApplicationData.h"
 
 namespace Simulation {
 namespace Simulators {
+
+// A scene the batched passes cannot step correctly (cells smaller than the
+// kernel radius); fg_bsim_create reports it as FG_ERR_UNSUPPORTED.
+struct UnsupportedScene : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
 
 struct BatchScene {
     Application::CommonFluidParams common;  // LINEAR grid of this scene

@@ -417,20 +417,30 @@ typedef struct fg_inflow_desc {
     fg_float3 origin;        /* lattice site (0, 0) of the inflow window */
     fg_float3 axisU, axisV;  /* in-plane lattice axes (unit vectors) */
     fg_float3 normal;        /* inflow direction (unit vector) */
-    float spacing;           /* lattice spacing, and the distance between layers */
-    float speed;             /* inflow speed along normal */
+    float spacing;           /* lattice spacing, and the distance between layers (world units) */
+    float speed;             /* inflow speed along normal (simulation units, as velocities) */
     unsigned int nu, nv;     /* lattice sites per layer */
     unsigned int firstSlot;  /* first reserve slot */
     unsigned int numLayers;  /* the reserve: numLayers * nu * nv slots from firstSlot */
 } fg_inflow_desc;
 
+/* Layer period in seconds: spacing * simulationScale / speed (FP64).  A
+ * step moves a particle by v dt / simulation_scale world units
+ * (K_FluidSim.cu:433-438), so consecutive layers are `spacing` world units
+ * apart.  0 for a null descriptor or a non-positive speed. */
+double fg_inflow_period(const fg_inflow_desc* desc, float simulationScale);
+
 /* Emission at the start of a step.  With k = *stepCounter and period =
- * spacing / speed, layers before step k = floor(k dt / period) (FP64); when
- * that count grows from k to k+1, layer l = floor(k dt / period), if
- * l < numLayers, is emitted: slot firstSlot + l nu nv + iu + nu iv gets
- * position origin + (iu spacing) axisU + (iv spacing) axisV (FP32, left to
- * right), w = 1, velocity speed * normal.  *stepCounter is then incremented.
- * Two launches, no host synchronisation (graph-capturable). */
+ * fg_inflow_period(desc, simulation_scale of gInitializeCommonFluidParams),
+ * layers before step k = floor(k dt / period) (FP64); when that count grows
+ * from k to k+1, layer l = floor(k dt / period), if l < numLayers, is
+ * emitted: slot firstSlot + l nu nv + iu + nu iv gets position
+ * origin + (iu spacing) axisU + (iv spacing) axisV (FP32, left to right),
+ * w = 1, velocity speed * normal.  *stepCounter is then incremented.  At
+ * most one layer is due per step: dt > period is refused
+ * (FG_ERR_INVALID_ARGUMENT), as are missing common parameters
+ * (FG_ERR_NOT_INITIALIZED).  Two launches, no host synchronisation
+ * (graph-capturable). */
 int fg_emit_inflow(fg_float4* positions, fg_float4* velocities, unsigned int* stepCounter, const fg_inflow_desc* desc,
                    float dt);
 
@@ -494,7 +504,12 @@ typedef struct fg_scene_desc {
 /* key[i] = keyBase + LINEAR key of positions[i] in its scene's grid. */
 int fg_batch_hash(const fg_scene_desc* scenes, unsigned int numScenes, const unsigned int* sceneOf,
                   const fg_float4* positions, unsigned int* hashKey, unsigned int numParticles);
-/* fg_sph_density_nbr per scene (neighbour masks always written). */
+/* fg_sph_density_nbr per scene (neighbour masks always written).  `scenes`
+ * is device memory, so the launcher cannot check it on the host: a scene
+ * whose cells are smaller than its kernel radius on any axis (the 27-cell
+ * stencil would miss neighbours) gets NaN (rho, p) for its particles, and the
+ * force pass writes NaN velocities for them, instead of short sums.
+ * fg_bsim_create refuses such scenes before anything runs. */
 int fg_batch_density_nbr(const fg_scene_desc* scenes, unsigned int numScenes, const unsigned int* sceneOf,
                          fg_float4* orderedPositions, const fg_float4* orderedVelocity, fg_float2* mass_pressure,
                          unsigned int* gridStart, unsigned int* gridEnd, unsigned long long* neighborMasks,

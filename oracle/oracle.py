@@ -160,16 +160,26 @@ def step(pos, vel, fluid, common, kernels, mode, rho_p_prev=None, end_init=None,
                 rho_p=rho_p, vel=v_new, pos=p_new, acc=acc, mag=mag)
 
 
-def emit_inflow(pos, vel, step, inflow, dt):
+def inflow_period(inflow, scale):
+    """Layer period of an inflow (fg_inflow_period): spacing * scale / speed
+    in FP64 from the FP32 fields.  A step moves a particle by v dt / scale
+    world units (K_FluidSim.cu:433-438), so after one period a layer has
+    moved `spacing` world units."""
+    return float(np.float32(inflow.spacing)) * float(np.float32(scale)) / float(np.float32(inflow.speed))
+
+
+def emit_inflow(pos, vel, step, inflow, dt, scale):
     """Inflow emission at the start of step `step` (fg_emit_inflow): when
-    floor((k+1) dt / period) > floor(k dt / period), period = spacing / speed
-    (FP64), layer l = floor(k dt / period) of nu x nv reserve slots from
+    floor((k+1) dt / period) > floor(k dt / period), period =
+    inflow_period(inflow, scale), layer l = floor(k dt / period) of nu x nv reserve slots from
     firstSlot is placed on the window lattice origin + iu s U + iv s V
     (FP32, left to right, no fused multiply-add) with velocity speed * normal;
     layers beyond numLayers are not emitted.  Returns the number of particles
     emitted."""
     d = inflow
-    period = float(np.float32(d.spacing)) / float(np.float32(d.speed))
+    period = inflow_period(d, scale)
+    if float(np.float32(dt)) > period:
+        raise ValueError("dt exceeds the inflow layer period")
     l0 = int(np.floor(step * float(np.float32(dt)) / period))
     l1 = int(np.floor((step + 1) * float(np.float32(dt)) / period))
     if l1 == l0 or l0 >= d.numLayers:

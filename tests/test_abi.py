@@ -5,9 +5,11 @@ import ctypes as C
 import glob
 import os
 import re
+import shutil
 import subprocess
 import textwrap
 
+import numpy as np
 import pytest
 
 from fluidgrid_amd import abi, lib
@@ -120,6 +122,28 @@ def test_new_entry_points_reject_bad_arguments_before_any_launch():
         assert L.fg_last_error() == 0
 
 
+def test_inflow_dt_beyond_period_refused_before_launch():
+    """A step longer than the layer period spacing * scale / speed would owe
+    several layers; fg_emit_inflow refuses it on the host."""
+    import ctypes as C
+    from fluidgrid_amd import scenes
+    L = lib.load()
+    s = scenes.with_inflow(scenes.small_dam_break(lattice=(4, 4, 4), h=0.04), (0.3, 0.2), (0.42, 0.3), speed=1.0)
+    d = s.info["inflow"]
+    cp = abi.CommonFluidParams.from_buffer_copy(s.common)
+    cp.simulation_scale = 0.009
+    L.gInitializeCommonFluidParams(cp)
+    try:
+        period = L.fg_inflow_period(C.byref(d), 0.009)
+        assert abs(period - d.spacing * np.float32(0.009) / d.speed) <= 1e-12
+        p = C.c_void_p(256)
+        assert L.fg_emit_inflow(p, p, p, C.byref(d), float(np.float32(2.0 * period))) == 1
+        assert b"exceeds the layer period" in L.fg_last_error_string()
+        assert L.fg_last_error() == 1
+    finally:
+        L.gInitializeCommonFluidParams(s.common)
+
+
 def test_checkpoint_and_mesh_entry_points_check_arguments():
     """Without a simulator, save/load report FG_ERR_NOT_INITIALIZED; the mesh
     entry points reject null pointers and empty meshes; nothing touches a
@@ -160,3 +184,49 @@ def test_cells_smaller_than_h_are_refused():
         assert L.fg_last_error() == 5
     finally:
         L.gInitializeCommonFluidParams(s.common)
+
+
+def test_batched_scene_with_cells_smaller_than_h_is_refused():
+    """fg_bsim_create checks every scene's cells against its kernel radius
+    on the host, before any allocation, and reports FG_ERR_UNSUPPORTED."""
+    import numpy as np
+    from fluidgrid_amd import scenes
+    L = lib.load()
+    a = scenes.small_dam_break(lattice=(4, 4, 4), h=0.04)
+    b = scenes.small_dam_break(lattice=(4, 4, 4), h=0.04)
+    commons = (abi.CommonFluidParams * 2)(a.common, b.common)
+    commons[1].cell_size = abi.f3(0.04, 0.03, 0.04)  # below h along y
+    fluids = (abi.FluidParams * 2)(a.fluid, b.fluid)
+    counts = (C.c_uint * 2)(a.n, b.n)
+    pos = np.ascontiguousarray(np.concatenate([a.pos, b.pos]))
+    vel = np.ascontiguousarray(np.concatenate([a.vel, b.vel]))
+    rc = L.fg_bsim_create(2, commons, fluids, counts, pos.ctypes.data, vel.ctypes.data)
+    assert rc == abi.ERR_UNSUPPORTED
+    assert b"scene 1 has cells smaller than the kernel radius" in L.fg_last_error_string()
+    assert L.fg_last_error() == abi.ERR_UNSUPPORTED
+
+
+def test_reference_prototypes_pass_arguments_intact(tmp_path):
+    """A caller compiled against the reference's own prototypes (HIP
+    float2/float3/float4/int3 and bool by value, its CommonFluidParams
+    declaration, none of this library's headers) links unchanged: the
+    parameter block round-trips bit for bit, and the validation errors echo
+    every by-value scalar of gCalcHashKey (K_HashGrid.cuh:5-8) and of the
+    28-argument gSPHCalculateForcesWithSRB (K_FluidSim.cuh:46-78), whose
+    trailing arguments travel on the stack."""
+    src = os.path.join(ROOT, "tests", "abi_probe", "reference_caller.cpp")
+    exe = tmp_path / "reference_caller"
+    libdir = os.path.join(ROOT, "fluidgrid_amd")
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    subprocess.run([hipcc, "-std=c++17", "-O1", src, "-o", str(exe), "-L", libdir, "-lfluidgrid_amd",
+                    f"-Wl,-rpath,{libdir}"], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    lines = dict(l.split(" ", 1) for l in out.strip().splitlines())
+    assert lines["sizeof"] == "104"
+    assert lines["common_roundtrip"] == "rc=0 equal=1"
+    assert lines["hash"].startswith("err=1 ")
+    assert "cell_size (-0.5, 0.25, 1.5) must be positive (numParticles=4321 numBuckets=98765)" in lines["hash"]
+    assert lines["srb"].startswith("err=1 ")
+    assert ("(numParticles=12345 mass=0.25 kViscosity=3.5 sph_particle_radius=0.125 cell_size_relation=(1,2,3) "
+            "srb_particle_radius=0.0625 srb_cell_size=(0.5,0.75,1.25) kspring=-2000 kdamping=-50 "
+            "srb_num_buckets=4096 rb_num_buckets=777 useStaticRB=1 useDynamicRB=0)") in lines["srb"]

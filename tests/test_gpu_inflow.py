@@ -8,7 +8,11 @@ Tolerances:
                                  reversed bucket walk, at least 1e-4 h;
                                  active count exact
   eager and graph-replayed       bit-identical
+  dt beyond the layer period     refused with FG_ERR_INVALID_ARGUMENT before
+                                 any launch
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -34,14 +38,22 @@ def test_emit_inflow_matches_oracle(fglib, orc):
     v = scenes.quat_rotate(q, np.array([[0.0, 0.0, 1.0]]))[0]
     rot.axisU, rot.axisV = abi.f3(*u), abi.f3(*v)
     dt = np.float32(s.common.deltaTimeIntegration)
-    for desc in (d, rot):
+    # the reference scene's simulation scale: the period carries the factor
+    scaled = abi.InflowDesc.from_buffer_copy(d)
+    cp = abi.CommonFluidParams.from_buffer_copy(s.common)
+    for desc, scale in ((d, 1.0), (rot, 1.0), (scaled, 0.009)):
+        cp.simulation_scale = scale
+        L.gInitializeCommonFluidParams(cp)
+        if scale != 1.0:
+            desc.speed = float(np.float32(d.speed * scale))
+        assert L.fg_inflow_period(C.byref(desc), scale) == orc.inflow_period(desc, scale)
         pos = s.pos.copy()
         vel = rng.normal(size=s.vel.shape).astype(np.float32)
         dp, dv = fglib.DeviceArray.from_numpy(pos), fglib.DeviceArray.from_numpy(vel)
         step = fglib.DeviceArray.from_numpy(np.zeros(1, np.uint32))
         for k in range(24):
             fglib.check(L.fg_emit_inflow(dp.ptr, dv.ptr, step.ptr, desc, float(dt)), "emit")
-            orc.emit_inflow(pos, vel, k, desc, dt)
+            orc.emit_inflow(pos, vel, k, desc, dt, scale)
         assert step.numpy()[0] == 24
         assert np.array_equal(dp.numpy(), pos) and np.array_equal(dv.numpy(), vel)
         assert np.count_nonzero(pos[:, 3] > 0) == s.info["active"] + desc.numLayers * desc.nu * desc.nv
@@ -70,7 +82,7 @@ def test_inflow_trajectory_matches_oracle(fglib, orc):
         try:
             p, v, rho = s.pos.copy(), s.vel.copy(), np.zeros((s.n, 2), np.float32)
             for k in range(steps):
-                orc.emit_inflow(p, v, k, d, np.float32(s.common.deltaTimeIntegration))
+                orc.emit_inflow(p, v, k, d, np.float32(s.common.deltaTimeIntegration), s.common.simulation_scale)
                 r = orc.step(p, v, s.fluid, s.common, s.kernels, s.key_mode, rho_p_prev=rho)
                 p, v, rho = r.pos, r.vel, r.rho_p
         finally:
@@ -92,3 +104,18 @@ def test_inflow_graph_equals_eager(fglib):
     a = _run(fglib, s, 20, False)
     b = _run(fglib, s, 20, True)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_dt_beyond_period_refused(fglib):
+    L = fglib.load()
+    s = _scene()
+    d = s.info["inflow"]
+    L.gInitializeCommonFluidParams(s.common)
+    period = L.fg_inflow_period(C.byref(d), s.common.simulation_scale)
+    pos = fglib.DeviceArray.from_numpy(s.pos)
+    vel = fglib.DeviceArray.from_numpy(s.vel)
+    step = fglib.DeviceArray.from_numpy(np.zeros(1, np.uint32))
+    L.fg_last_error()
+    rc = L.fg_emit_inflow(pos.ptr, vel.ptr, step.ptr, d, float(1.5 * period))
+    assert rc == abi.ERR_INVALID_ARGUMENT and L.fg_last_error() == abi.ERR_INVALID_ARGUMENT
+    assert np.array_equal(pos.numpy(), s.pos) and step.numpy()[0] == 0

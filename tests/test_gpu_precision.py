@@ -6,13 +6,21 @@ From the GPU state S_k (k-1 steps taken), one GPU step and one FP64 oracle
 step (oracle density and forces in double on the same sorted FP32 inputs,
 then v + a dt and x + v dt/s in double) are compared:
 
-  rho   |rho_gpu - rho_64| / rho_64
-  p     |p_gpu - p_64| / (k rho0)          (pressure is ill-conditioned near rho0)
-  v     |v_gpu - v_64| / (dt (|a_64| + S)), S = the FP64 magnitude sum of the pair terms
-  x     |x_gpu - x_64| / h
+  rho       |rho_gpu - rho_64| / rho_64
+  p         |p_gpu - p_64| / (k rho0)          (pressure is ill-conditioned near rho0)
+  v_forces  |v_gpu - v_64'| / (dt (|a_64'| + S)), v_64' the FP64 force pass on
+            the GPU's own (rho, p): the error of the force pass alone; S =
+            the FP64 magnitude sum of the pair terms
+  v         the same against the FP64 step end to end (FP64 density too)
+  x         |x_gpu - x_64| / h
 
-The test asserts the FP32 bound of the plain form (rho within 1e-5) and
-writes the table as JSON to $FG_REPORT_DIR when that is set.
+Bounds asserted: rho <= 1e-5 and v_forces <= 1e-5 (the FP32 targets), and v
+<= 5e-5 end to end.  The end-to-end figure is larger than the force pass's
+own because p = k (rho - rho0) turns the density's relative rounding
+(~5e-7 of rho) into an absolute pressure error k * 5e-7 * rho that does not
+shrink with p: near rho0 it is a large fraction of p, and of the pressure
+pair terms in S.  The table is written as JSON to $FG_REPORT_DIR when that
+is set.
 """
 import json
 import os
@@ -37,22 +45,36 @@ def _f64_step(orc, s, pos, vel):
     x = pos[:, :3].astype(np.float64).copy()
     act = pos[:, 3] > 0
     x[act] += v_out[act, :3] * q
-    return dict(rho_p=rp, spid=spid, vel=v_out[:, :3], pos=x, acc=acc, mag=mag)
+    return dict(rho_p=rp, spid=spid, vel=v_out[:, :3], pos=x, acc=acc, mag=mag, spos=spos, svel=svel,
+                start=start, end=end)
 
 
-def _errors(s, g_rp, g_vel, g_pos, ref):
+def _f64_forces_on(orc, s, ref, rho_p, vel):
+    """The FP64 force pass on a given (rho, p) (the GPU's) and the same
+    sorted inputs."""
+    v_out, acc, mag = orc.forces(ref["spos"], ref["svel"], rho_p.astype(np.float64), ref["start"], ref["end"],
+                                 ref["spid"], s.fluid, s.common, s.kernels, s.key_mode, f64=True,
+                                 v_out_init=vel.astype(np.float64), want_acc=True, want_mag=True)
+    return dict(spid=ref["spid"], vel=v_out[:, :3], acc=acc, mag=mag)
+
+
+def _rel_v(s, g_vel, r):
+    dt = float(s.common.deltaTimeIntegration)
+    scale = np.zeros(s.n)
+    scale[r["spid"]] = np.linalg.norm(r["acc"], axis=1) + r["mag"]
+    return np.linalg.norm(g_vel[:, :3] - r["vel"], axis=1) / (dt * np.maximum(scale, 1e-30))
+
+
+def _errors(s, g_rp, g_vel, g_pos, ref, ref_forces):
     act = ref["rho_p"][:, 0] > 0
     rel_rho = np.abs(g_rp[:, 0] - ref["rho_p"][:, 0]) / ref["rho_p"][:, 0]
     dp = np.abs(g_rp[:, 1] - ref["rho_p"][:, 1]) / (s.fluid.gasConstant * s.fluid.restDensity)
-    dt = float(s.common.deltaTimeIntegration)
-    scale = np.zeros(s.n)
-    scale[ref["spid"]] = np.linalg.norm(ref["acc"], axis=1) + ref["mag"]
-    dv = np.linalg.norm(g_vel[:, :3] - ref["vel"], axis=1) / (dt * np.maximum(scale, 1e-30))
     dx = np.linalg.norm(g_pos[:, :3] - ref["pos"], axis=1) / s.common.kernel_radius
 
     def st(e):
         return {"max": float(np.max(e)), "mean": float(np.mean(e))}
-    return {"rho": st(rel_rho[act]), "p": st(dp[act]), "v": st(dv), "x": st(dx)}
+    return {"rho": st(rel_rho[act]), "p": st(dp[act]), "v_forces": st(_rel_v(s, g_vel, ref_forces)),
+            "v": st(_rel_v(s, g_vel, ref)), "x": st(dx)}
 
 
 def test_fp32_step_error_against_fp64_oracle(fglib, orc):
@@ -68,7 +90,8 @@ def test_fp32_step_error_against_fp64_oracle(fglib, orc):
             g = sim.arrays()
             pos1, vel1 = sim.state()
             ref = _f64_step(orc, s, pos0, vel0)
-            report[f"after_{k}_steps" if k > 1 else "step_1"] = _errors(s, g["rho_p"], vel1, pos1, ref)
+            ref_forces = _f64_forces_on(orc, s, ref, g["rho_p"], vel0)
+            report[f"after_{k}_steps" if k > 1 else "step_1"] = _errors(s, g["rho_p"], vel1, pos1, ref, ref_forces)
     finally:
         sim.close()
     print(json.dumps(report, indent=1))
@@ -79,4 +102,5 @@ def test_fp32_step_error_against_fp64_oracle(fglib, orc):
             json.dump(report, f, indent=1)
     for key in ("step_1", "after_200_steps"):
         assert report[key]["rho"]["max"] <= 1e-5
-        assert report[key]["v"]["max"] <= 1e-4
+        assert report[key]["v_forces"]["max"] <= 1e-5
+        assert report[key]["v"]["max"] <= 5e-5
