@@ -18,6 +18,8 @@
 //    block waits for belongs to a block that is already running;
 //  * digit widths are chosen from the key width (<= 8 bits per pass), so
 //    16-bit keys take two passes, 22-bit keys three, 25/27-bit keys four.
+#include <mutex>
+
 #include "fg_internal.hpp"
 
 namespace fg {
@@ -330,13 +332,13 @@ struct SortScratch {
     size_t bytes = 0;
 };
 
-static SortScratch& scratch() {
+static SortScratch& sort_scratch() {
     static SortScratch s;
     return s;
 }
 
 static void* ensure_scratch(size_t bytes) {
-    SortScratch& s = scratch();
+    SortScratch& s = sort_scratch();
     if (s.bytes >= bytes) return s.ptr;
     if (s.ptr) {
         (void)hipStreamSynchronize(state().stream);
@@ -405,11 +407,78 @@ static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, voi
     return check_launch("radix sort") ? FG_OK : fg_peek_error();
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free sort: the workspace is a file-scope device array, one copy
+// per GPU, kept until the process ends.  Host code never passes `scratch`
+// itself to a kernel or copy (in host code the name is a host-side stand-in,
+// not a device address): each device's copy is found once with
+// hipGetSymbolAddress, with that device current, and only that address is
+// used.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoAllocMaxPairs = FG_SORT_NOALLOC_MAX_PAIRS;
+constexpr size_t kNoAllocScratchBytes = FG_SORT_NOALLOC_SCRATCH_BYTES;
+static_assert(kNoAllocScratchBytes % 256 == 0, "scratch size");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch`, or null (with the error recorded).
+static void* noalloc_scratch() {
+    int dev = -1;
+    if (!check_hip(hipGetDevice(&dev), "fg_sort_pairs_noalloc hipGetDevice")) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error(FG_ERR_UNSUPPORTED, "fg_sort_pairs_noalloc: device %d beyond %d", dev, kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (!check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "fg_sort_pairs_noalloc hipGetSymbolAddress"))
+            return nullptr;
+        if (!ptr) {
+            set_error(FG_ERR_HIP, "fg_sort_pairs_noalloc: no device address for the scratch array");
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace fg
 
 using namespace fg;
 
 extern "C" {
+
+int fg_sort_pairs_noalloc(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits,
+                          int values_are_iota) {
+    if (n == 0) return FG_OK;
+    if (!keys || !values) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sort_pairs_noalloc: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    if (keyBits < 1 || keyBits > 32) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sort_pairs_noalloc: keyBits %d outside 1..32", keyBits);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    if (n > kNoAllocMaxPairs) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sort_pairs_noalloc: n = %u exceeds the %u pairs its %zu-byte "
+                  "static workspace holds; use fg_sort_pairs_ws", n, kNoAllocMaxPairs, kNoAllocScratchBytes);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    const size_t need = make_layout(n, keyBits).total;
+    if (need > kNoAllocScratchBytes) {  // cannot happen for n <= kNoAllocMaxPairs; kept as a guard
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_sort_pairs_noalloc: %zu workspace bytes needed, %zu held", need,
+                  kNoAllocScratchBytes);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    void* ws = noalloc_scratch();
+    if (!ws) return fg_peek_error();
+    return run_sort(keys, values, n, keyBits, ws, kNoAllocScratchBytes, values_are_iota != 0, state().stream);
+}
 
 size_t fg_sort_workspace_size(unsigned int n, int keyBits) { return make_layout(n, keyBits).total; }
 
@@ -439,7 +508,7 @@ static int sort_with_scratch(unsigned int* keys, unsigned int* values, unsigned 
     const size_t need = fg_sort_workspace_size(n, keyBits);
     void* ws = ensure_scratch(need);
     if (!ws) return fg_peek_error();
-    return run_sort(keys, values, n, keyBits, ws, scratch().bytes, iota, state().stream);
+    return run_sort(keys, values, n, keyBits, ws, sort_scratch().bytes, iota, state().stream);
 }
 
 int fg_sort_pairs(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits) {
@@ -474,7 +543,7 @@ void gSort(unsigned int* _key, unsigned int* _values, int _numParticles) {
         return;
     int bits = 1;
     while (bits < 32 && (host_max >> bits) != 0) bits++;
-    run_sort(_key, _values, n, bits, ws, scratch().bytes, false, st);
+    run_sort(_key, _values, n, bits, ws, sort_scratch().bytes, false, st);
 }
 
 }  // extern "C"

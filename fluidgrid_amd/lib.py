@@ -52,6 +52,7 @@ SIGNATURES = {
     "fg_sort_pairs_iota": (i32, [vp, vp, u32, i32]),
     "fg_sort_workspace_size": (sz, [u32, i32]),
     "fg_sort_pairs_ws": (i32, [vp, vp, u32, i32, vp, sz, i32]),
+    "fg_sort_pairs_noalloc": (i32, [vp, vp, u32, i32, i32]),
     "fg_calc_hash_key_only": (i32, [vp, vp, u32, u32, abi.Float3]),
     "fg_sph_forces_integrate": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32]),
     "fg_neighbor_mask_bytes": (sz, [u32]),

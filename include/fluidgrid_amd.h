@@ -297,6 +297,21 @@ size_t fg_sort_workspace_size(unsigned int n, int keyBits);
 int fg_sort_pairs_ws(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits,
                      void* workspace, size_t workspace_bytes, int values_are_iota);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc -- fg_sort_pairs_ws without the workspace argument:
+ * the same stable sort (same results), with its workspace in a file-scope
+ * device array of FG_SORT_NOALLOC_SCRATCH_BYTES that every GPU the library
+ * runs on holds until the process ends.  It calls no hipMalloc, hipFree or
+ * allocating routine.  n is limited to FG_SORT_NOALLOC_MAX_PAIRS (larger n:
+ * FG_ERR_INVALID_ARGUMENT, nothing launched).  All calls on one GPU share
+ * that array and each call clears what must start at zero, so calls that
+ * overlap on one GPU (concurrent streams or host threads) race. */
+#define FG_SORT_NOALLOC_MAX_PAIRS 1048576u
+#define FG_SORT_NOALLOC_SCRATCH_BYTES (17u * 1024u * 1024u + 64u * 1024u)
+int fg_sort_pairs_noalloc(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits,
+                          int values_are_iota);
+
 /* gCalcHashKey without the id output (the sort that follows generates the
  * identity ids itself, see fg_sort_pairs_iota). */
 int fg_calc_hash_key_only(unsigned int* hashKey, fg_float4* positions, unsigned int numParticles,
