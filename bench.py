@@ -23,6 +23,11 @@ scenes, fixed and autotuned, beside the same scenes stepped one at a time
 Every single-fluid line carries a `drift` report from the FP64 conservation
 sums (mass, momentum against the gravity impulse, energies).
 
+Timing: 5 runs, each a fresh simulator from the scene's initial state: W
+untimed warm-up steps, then one region of exactly K steps bracketed by a
+device synchronisation and HIP events on the simulator's stream; `value`
+and `ms_per_step` come from the median run.
+
 Roofline: algorithmic bytes per particle (hash 24, one sort pass 16, cells +
 reorder 72, density 24, forces 60, integrate 48) plus 4 B per bucket for the
 cellStart fill, 24 B per occupied cell (start/end write, two reads) and 176 B
@@ -32,7 +37,9 @@ stage over a second, eager pass of the same steps.
 
 cpu_baseline: the oracle (a C++/OpenMP restatement of the reference step;
 the reference itself has no CPU SPH path) timed on this host's cores on a
-bounded sample of the same workload.
+bounded sample of the same workload, on every OpenMP thread and on one, plus
+configuration 0 (16,384 particles) for its 100 steps; `config.cfg0` times
+the same 100 steps on the GPU.
 """
 import argparse
 import json
@@ -72,44 +79,83 @@ def stage_bytes(stage, n, c_occ, fused, m=0):
     return by
 
 
-def measure(scene, steps, warmup, fused=True):
-    sim = lib.Simulator(scene, fused=fused, use_graph=True)
-    bodies = scene.info.get("bodies")
+REPEATS = 5
+
+
+def timed_region(step, stream, sync, steps):
+    """One timed region of exactly `steps` steps, the device synchronised
+    before and after, HIP events on the stream the steps run on around it
+    (the wall clock kept beside).  Returns (event ms, wall ms)."""
+    t = lib.StreamTimer(stream)
     try:
-        if bodies is not None:
-            sim.add_bodies(bodies)
-        if scene.info.get("inflow") is not None:
-            sim.set_inflow(scene.info["inflow"])
-        c_start = sim.conservation()
-        sim.step(warmup)
-        sim.synchronize()
-        t0 = time.perf_counter()
-        sim.step(steps)
-        sim.synchronize()
-        dt = time.perf_counter() - t0
-        c_end = sim.conservation()
-        drift = drift_report(scene, c_start, c_end, warmup + steps)
-        c_occ = occupied_cells(sim)
-        # per-stage durations: eager steps with events between stages
-        sim.set_options(fused=fused, use_graph=False, stage_timing=True)
-        acc = None
-        nprof = max(3, min(steps, 10))
-        for _ in range(nprof):
-            sim.step(1)
-            t = sim.stage_times()
-            acc = t if acc is None else {k: acc[k] + t[k] for k in t}
-        stage_ms = {k: v / nprof for k, v in acc.items()}
-        pos, _ = sim.state()
-        finite = bool(np.all(np.isfinite(pos)))
-        body = None
-        if bodies is not None:
-            bs = sim.body_state()
-            finite = finite and bool(np.all(np.isfinite(bs["pos"])))
-            body = {"centre_y_m": [round(float(v), 6) for v in bs["pos"][:, 1]]}
+        sync()
+        w0 = time.perf_counter()
+        t.start()
+        step(steps)
+        ev = t.stop_ms()
+        sync()
+        return ev, 1e3 * (time.perf_counter() - w0)
     finally:
-        sim.close()
+        t.close()
+
+
+def median_index(values):
+    return sorted(range(len(values)), key=lambda k: values[k])[len(values) // 2]
+
+
+def new_sim(scene, fused):
+    sim = lib.Simulator(scene, fused=fused, use_graph=True)
+    if scene.info.get("bodies") is not None:
+        sim.add_bodies(scene.info["bodies"])
+    if scene.info.get("inflow") is not None:
+        sim.set_inflow(scene.info["inflow"])
+    return sim
+
+
+def measure(scene, steps, warmup, fused=True):
+    """REPEATS runs, each a fresh simulator from the scene's initial state:
+    `warmup` untimed steps, then one timed region of exactly `steps` steps;
+    the median run is reported.  The last run also gives the drift report,
+    the occupied cells and per-stage durations (eager steps with HIP events
+    between the stages, after its timed region)."""
+    bodies = scene.info.get("bodies")
+    ev_all, wall_all = [], []
+    for rep in range(REPEATS):
+        sim = new_sim(scene, fused)
+        try:
+            last = rep == REPEATS - 1
+            if last:
+                c_start = sim.conservation()
+            sim.step(warmup)
+            ev, wall = timed_region(sim.step, sim.stream(), sim.synchronize, steps)
+            ev_all.append(ev)
+            wall_all.append(wall)
+            if not last:
+                continue
+            c_end = sim.conservation()
+            drift = drift_report(scene, c_start, c_end, warmup + steps)
+            c_occ = occupied_cells(sim)
+            sim.set_options(fused=fused, use_graph=False, stage_timing=True)
+            acc = None
+            nprof = max(3, min(steps, 10))
+            for _ in range(nprof):
+                sim.step(1)
+                t = sim.stage_times()
+                acc = t if acc is None else {k: acc[k] + t[k] for k in t}
+            stage_ms = {k: v / nprof for k, v in acc.items()}
+            pos, _ = sim.state()
+            finite = bool(np.all(np.isfinite(pos)))
+            body = None
+            if bodies is not None:
+                bs = sim.body_state()
+                finite = finite and bool(np.all(np.isfinite(bs["pos"])))
+                body = {"centre_y_m": [round(float(v), 6) for v in bs["pos"][:, 1]]}
+        finally:
+            sim.close()
+    med = median_index(ev_all)
+    dt = ev_all[med] * 1e-3
     return dict(seconds=dt, ms_per_step=1e3 * dt / steps, c_occ=c_occ, stage_ms=stage_ms, finite=finite, body=body,
-                drift=drift)
+                drift=drift, repeats_ms=[round(x, 4) for x in ev_all], wall_ms=wall_all[med])
 
 
 def drift_report(scene, c0, c1, steps):
@@ -131,49 +177,58 @@ def drift_report(scene, c0, c1, steps):
 
 
 def measure_batched(ens, steps, warmup, tune_steps=5):
-    """The batched form over an ensemble: fixed launch shape (256-lane
-    workgroups, 8-bit digits, row schedules), then the autotuned shape, each
-    on a fresh simulator from the ensemble's initial state (tuning restores
-    the state, so both runs time the same particle configurations);
-    per-stage times of the tuned shape; and the same scenes stepped one at a
-    time through the single-fluid simulator (the plain form's way to run
-    them)."""
+    """The batched form over an ensemble: the launch shape is autotuned once;
+    then the fixed shape (256-lane workgroups, 8-bit digits, row schedules)
+    and the tuned shape are each timed over REPEATS fresh simulators from the
+    ensemble's initial state (median reported); per-stage times of the tuned
+    shape; and the same scenes stepped one at a time through the
+    single-fluid simulator (the plain form's way to run them)."""
     out = {}
     bodies = ens.bodies or [None] * len(ens.scenes)
+
+    def fresh():
+        b = lib.BatchSimulator(ens)
+        for s, bs in enumerate(bodies):
+            if bs is not None:
+                b.add_bodies(s, bs)
+        return b
+
     try:
-        for label in ("fixed", "tuned"):
-            b = lib.BatchSimulator(ens)
-            try:
-                for s, bs in enumerate(bodies):
-                    if bs is not None:
-                        b.add_bodies(s, bs)
-                if label == "fixed":
-                    b.set_tuning(256, 8, False, False)
-                else:
-                    t = b.autotune(trial_steps=tune_steps)
-                    out["tuning"] = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
-                                     "flat_density": t["flat_density"], "flat_forces": t["flat_forces"],
-                                     "trials_ms": [[a, c, fd, ff, round(m, 4)] for a, c, fd, ff, m in t["trials"]]}
-                b.step(warmup)
-                b.synchronize()
-                t0 = time.perf_counter()
-                b.step(steps)
-                b.synchronize()
-                dt = time.perf_counter() - t0
-                out[label] = {"ms_per_step": 1e3 * dt / steps, "value": ens.n * steps / dt}
-                if label == "tuned":
-                    b.set_options(use_graph=False, stage_timing=True)
-                    acc = None
-                    for _ in range(5):
-                        b.step(1)
-                        st = b.stage_times()
-                        acc = st if acc is None else {k: acc[k] + st[k] for k in st}
-                    out["stage_ms"] = {k: round(v / 5, 5) for k, v in acc.items()}
-                    pos, _ = b.state()
-                    out["finite"] = bool(np.all(np.isfinite(pos)))
-                    info = b.info()
-            finally:
-                b.close()
+        b = fresh()
+        try:
+            t = b.autotune(trial_steps=tune_steps)
+        finally:
+            b.close()
+        out["tuning"] = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
+                         "flat_density": t["flat_density"], "flat_forces": t["flat_forces"],
+                         "trials_ms": [[a, c, fd, ff, round(m, 4)] for a, c, fd, ff, m in t["trials"]]}
+        shapes = {"fixed": (256, 8, False, False),
+                  "tuned": (t["sph_block"], t["sort_digit_bits"], t["flat_density"], t["flat_forces"])}
+        for label, shape in shapes.items():
+            ev_all = []
+            for rep in range(REPEATS):
+                b = fresh()
+                try:
+                    b.set_tuning(*shape)
+                    b.step(warmup)
+                    ev, _ = timed_region(b.step, b.stream(), b.synchronize, steps)
+                    ev_all.append(ev)
+                    if label == "tuned" and rep == REPEATS - 1:
+                        b.set_options(use_graph=False, stage_timing=True)
+                        acc = None
+                        for _ in range(5):
+                            b.step(1)
+                            st = b.stage_times()
+                            acc = st if acc is None else {k: acc[k] + st[k] for k in st}
+                        out["stage_ms"] = {k: round(v / 5, 5) for k, v in acc.items()}
+                        pos, _ = b.state()
+                        out["finite"] = bool(np.all(np.isfinite(pos)))
+                        info = b.info()
+                finally:
+                    b.close()
+            dt = ev_all[median_index(ev_all)] * 1e-3
+            out[label] = {"ms_per_step": 1e3 * dt / steps, "value": ens.n * steps / dt,
+                          "repeats_ms": [round(x, 4) for x in ev_all]}
     finally:
         lib.load().fg_set_tuning(256, 8)
         lib.load().fg_set_schedule(0, 0)
@@ -242,14 +297,15 @@ def step_bytes(scene, c_occ):
     return 244 * scene.n + 4 * scene.num_buckets + 24 * c_occ + 176 * m
 
 
-def cpu_baseline(scene, seconds_budget=15.0):
+def oracle_rate(scene, seconds_budget, max_steps=50):
+    """Particle-steps/s of the oracle's step (coupled when the scene has
+    bodies) over at least one step and about `seconds_budget` of host time."""
     from oracle import oracle
-    threads = oracle.num_threads()
     bodies = scene.info.get("bodies")
     steps, t_total = 0, 0.0
     if bodies is not None:
         st = oracle.CoupledState(scene, bodies)
-        while t_total < seconds_budget and steps < 50:
+        while (t_total < seconds_budget or steps == 0) and steps < max_steps:
             t0 = time.perf_counter()
             st = oracle.coupled_step(st)
             t_total += time.perf_counter() - t0
@@ -258,16 +314,36 @@ def cpu_baseline(scene, seconds_budget=15.0):
     else:
         pos, vel = scene.pos.copy(), scene.vel.copy()
         rho_p = np.zeros((scene.n, 2), np.float32)
-        while t_total < seconds_budget and steps < 50:
+        while (t_total < seconds_budget or steps == 0) and steps < max_steps:
             t0 = time.perf_counter()
             st = oracle.step(pos, vel, scene.fluid, scene.common, scene.kernels, scene.key_mode, rho_p_prev=rho_p)
             t_total += time.perf_counter() - t0
             pos, vel, rho_p = st.pos, st.vel, st.rho_p
             steps += 1
         what = f"oracle steps of the {scene.n:,}-particle scene"
-    return dict(value=scene.n * steps / t_total, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"{steps} {what} ({t_total:.1f} s, {threads} OpenMP threads)", cpu=cpu_model(),
-                host_cpus=os.cpu_count())
+    return scene.n * steps / t_total, steps, t_total, what
+
+
+def cpu_baseline(scene, seconds_budget=12.0, one_core_budget=6.0):
+    """The oracle on this host: the headline workload on every OpenMP thread
+    and on one, and configuration 0 (16,384 particles, the reference's host
+    step) for its 100 steps."""
+    from oracle import oracle
+    threads = oracle.num_threads()
+    value, steps, t_total, what = oracle_rate(scene, seconds_budget)
+    oracle.set_num_threads(1)
+    try:
+        v1, s1, t1, _ = oracle_rate(scene, one_core_budget)
+    finally:
+        oracle.set_num_threads(threads)
+    s0 = scenes.config_scene(0)
+    v0, k0, t0, _ = oracle_rate(s0, 1e9, max_steps=s0.steps)
+    return dict(value=value, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"{steps} {what} ({t_total:.1f} s, {threads} OpenMP threads)",
+                value_1core=v1, sample_1core=f"{s1} steps of the same on 1 thread ({t1:.1f} s)",
+                cfg0_host_path={"workload": f"configuration 0, {s0.n:,} particles, {k0} oracle steps",
+                                "seconds": t0, "value": v0, "cores": threads},
+                cpu=cpu_model(), host_cpus=os.cpu_count())
 
 
 def cpu_model():
@@ -338,6 +414,8 @@ def main():
     ap.add_argument("--no-8m", action="store_true")
     ap.add_argument("--no-33m", action="store_true", help="skip the 33.5M-particle config-4 line (--no-8m skips it too)")
     ap.add_argument("--unfused", action="store_true")
+    ap.add_argument("--untiled", action="store_true",
+                    help="run the masked density/force passes without LDS tiles (row-loop schedules)")
     ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--batch-scenes", type=int, default=256)
@@ -356,6 +434,8 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     log("device:", lib.device_name(0))
+    if a.untiled:
+        lib.check(lib.load().fg_set_tiled(0), "fg_set_tiled")
     if a.copy_calibration:
         nbytes = a.copy_calibration << 20
         src, dst = lib.DeviceArray((nbytes,), np.uint8), lib.DeviceArray((nbytes,), np.uint8)
@@ -386,12 +466,19 @@ def main():
     dom_bytes = stage_bytes(dom, n, r["c_occ"], fused, bodies.m if bodies is not None else 0)
     achieved = dom_bytes / (stage_ms[dom] * 1e-3)
     step_gbs = step_bytes(scene, r["c_occ"]) / (r["ms_per_step"] * 1e-3)
-    traffic = None
+    # HBM bytes per dispatch of the dominant kernel come from rocprofv3
+    # counter passes committed under profiles/ (counters cannot be read in
+    # this run); the source and the commit they were measured at are named.
+    traffic, traffic_source = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
             t = json.load(open(tpath))
             traffic = t.get(scene.name, {}).get(dom)
+            if traffic is not None:
+                traffic_source = (f"profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE passes "
+                                  f"(calibrated), measured at commit {t.get('_commit', 'unknown')}, "
+                                  "not in this run")
         except Exception:
             traffic = None
     out = {
@@ -433,8 +520,14 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
+            "traffic_source": traffic_source,
             "algorithmic_bytes_per_launch": dom_bytes,
         },
+        "timing": {"method": f"{REPEATS} runs from the scene's initial state, each {a.warmup} untimed steps "
+                             f"then HIP events on the simulator's stream around exactly {a.steps} graph-replayed "
+                             "steps (device synchronised before and after); value and ms_per_step from the median "
+                             "run",
+                   "regions_ms": r["repeats_ms"], "median_region_wall_ms": round(r["wall_ms"], 4)},
     }
     if not a.no_8m and a.config != 2:
         k8 = max(20, a.steps // 4)
@@ -459,6 +552,11 @@ def main():
     if not a.no_mesh:
         log("mesh sampler ...")
         out["config"]["mesh_sampler"] = measure_mesh()
+    s0 = scenes.config_scene(0)
+    r0 = measure(s0, s0.steps, 10, fused)
+    out["config"]["cfg0"] = {"workload": f"configuration 0: {s0.n:,} particles, its {s0.steps} steps",
+                             "ms_per_step": r0["ms_per_step"], "value": s0.n * s0.steps / r0["seconds"],
+                             "seconds_100_steps": r0["seconds"], "finite": r0["finite"]}
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(scene)

@@ -35,6 +35,9 @@ struct State {
     // lane (true) or row by row (false); results are identical.
     bool flat_forces = false;
     bool flat_density = false;
+    // LDS-tiled single-fluid passes (density lists + list-walking forces);
+    // they take precedence over the two schedules above when switched on.
+    bool tiled = false;
 };
 
 State& state();

@@ -136,6 +136,16 @@ int fg_set_schedule(int flatDensity, int flatForces) {
     return FG_OK;
 }
 
+int fg_set_tiled(int enable) {
+    state().tiled = enable != 0;
+    return FG_OK;
+}
+
+int fg_get_tiled(int* enable) {
+    if (enable) *enable = state().tiled ? 1 : 0;
+    return FG_OK;
+}
+
 int fg_get_schedule(int* flatDensity, int* flatForces) {
     if (flatDensity) *flatDensity = state().flat_density ? 1 : 0;
     if (flatForces) *flatForces = state().flat_forces ? 1 : 0;
@@ -267,6 +277,13 @@ int fg_event_destroy(void* ev) {
 
 int fg_event_record(void* ev) {
     return check_hip(hipEventRecord(reinterpret_cast<hipEvent_t>(ev), state().stream), "hipEventRecord")
+               ? FG_OK
+               : t_err;
+}
+
+int fg_event_record_stream(void* ev, void* stream) {
+    return check_hip(hipEventRecord(reinterpret_cast<hipEvent_t>(ev), reinterpret_cast<hipStream_t>(stream)),
+                     "hipEventRecord")
                ? FG_OK
                : t_err;
 }

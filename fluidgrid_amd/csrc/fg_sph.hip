@@ -17,6 +17,10 @@
 //    reference; only rounding differs.
 #include "fg_internal.hpp"
 #include "fg_sph_fast.hpp"
+#include "fg_sph_tile.hpp"
+
+#include <mutex>
+#include <unordered_map>
 
 namespace fg {
 
@@ -239,9 +243,137 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8)))
                                    &fr);
 }
 
+// LDS-tiled passes (fg_sph_tile.hpp).  Tile b of the grid is the sorted
+// particles [b * kTileLanes, (b + 1) * kTileLanes); blocks are remapped so
+// each XCD walks a contiguous range of tiles.
+__global__ void __launch_bounds__(kTileLanes)
+    k_density_tile(const float4* __restrict__ sp, const float4* __restrict__ sv, float2* __restrict__ rho_p,
+                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, uint32_t n, float k,
+                   float rho0, float m, fg_common_fluid_params cp, fg_kernels_cache kc,
+                   unsigned long long* __restrict__ masks) {
+    __shared__ TileGeom g;
+    __shared__ TileSetupLds L;
+    __shared__ float4 lpos[kTileCap];
+    __shared__ __align__(16) uint16_t llists[kTileLanes * kListMax];
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = tile * kTileLanes + threadIdx.x;
+    const bool have = i < n;
+    const uint32_t nvalid = min(n - tile * kTileLanes, (uint32_t)kTileLanes);
+    const float4 p = have ? sp[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 v = have ? sv[i] : p;  // (loaded before the set-up, whose latency it shares)
+    const uint32_t fb = tile_setup(g, L, n, nvalid, have, p, cp, 0u, start, end);
+    // the staging plan, for the force pass
+    if (threadIdx.x < kTileDescWords) {
+        const uint32_t t = threadIdx.x, nseg = fb ? 0u : 9u * g.nrows;
+        uint32_t v = 0u;
+        if (t == 0) v = fb;
+        else if (t == 1) v = fb ? 0u : g.nrows;
+        else if (t == 2) v = fb ? 0u : g.total;
+        else if (t >= 4 && t < 4 + (uint32_t)kTileSegs) v = t - 4 < nseg ? g.sb[t - 4] : 0u;
+        else if (t >= 4 + (uint32_t)kTileSegs && t < 4 + 2 * (uint32_t)kTileSegs)
+            v = t - 4 - kTileSegs < nseg ? g.len[t - 4 - kTileSegs] : 0u;
+        else if (t >= 4 + 2 * (uint32_t)kTileSegs && t < 4 + 3 * (uint32_t)kTileSegs)
+            v = t - 4 - 2 * kTileSegs < nseg ? g.off[t - 4 - 2 * kTileSegs] : 0u;
+        nbr_descs(masks, n)[(size_t)tile * kTileDescWords + t] = v;
+    }
+    if (fb) {
+        if (have) {
+            density_particle<true>(i, sp, sv, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks);
+            nbr_counts(masks, n)[i] = kListOverflow;
+        }
+        return;
+    }
+    tile_fill(g, sp, 1, lpos);
+    tile_fill_wait();
+    if (have)
+        tile_density_particle(g, lpos, llists + threadIdx.x * kListMax, i, p, v, rho_p, start, end, n, k, rho0, m,
+                              cp, kc, masks);
+}
+
+template <bool COUPLED>
+__global__ void __launch_bounds__(kTileLanes)
+    k_forces_tile(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
+                  const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, float4* __restrict__ vout,
+                  float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
+                  fg_common_fluid_params cp, fg_kernels_cache kc, const unsigned long long* __restrict__ masks,
+                  Contacts bc) {
+    __shared__ TileGeom g;
+    __shared__ __align__(16) float4 recs[2 * kTileCap];
+    float4* recA = recs;
+    float4* recB = recs + kTileCap;
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = tile * kTileLanes + threadIdx.x;
+    const bool have = i < n;
+    // Everything a lane needs besides the staged records is loaded first, so
+    // its latency overlaps the descriptor load and the staging.
+    float4 pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vel = pos;
+    float2 dpi = make_float2(1.0f, 0.0f);
+    uint32_t cnt = 0u, oid = n;
+    uint4 l0 = make_uint4(0u, 0u, 0u, 0u), l1 = l0;
+    if (have) {
+        const uint4* lists = nbr_lists(masks, n);
+        pos = sp[i];
+        vel = sv[i];
+        dpi = rho_p[i];
+        cnt = nbr_counts(masks, n)[i];
+        oid = id[i];
+        l0 = lists[i];
+        l1 = lists[(size_t)n + i];
+    }
+    // the staging plan the density pass wrote for this tile
+    if (threadIdx.x < kTileDescWords) {
+        const uint32_t t = threadIdx.x;
+        const uint32_t v = nbr_descs(masks, n)[(size_t)tile * kTileDescWords + t];
+        if (t == 0) g.fallback = v;
+        else if (t == 1) g.nrows = v;
+        else if (t == 2) g.total = v;
+        else if (t >= 4 && t < 4 + (uint32_t)kTileSegs) g.sb[t - 4] = v;
+        else if (t >= 4 + (uint32_t)kTileSegs && t < 4 + 2 * (uint32_t)kTileSegs) g.len[t - 4 - kTileSegs] = v;
+        else if (t >= 4 + 2 * (uint32_t)kTileSegs && t < 4 + 3 * (uint32_t)kTileSegs)
+            g.off[t - 4 - 2 * kTileSegs] = v;
+    }
+    __syncthreads();
+    if (g.fallback != 0u || g.total > kTileCap || g.nrows > (uint32_t)kTileRowsMax) {
+        if (have)
+            forces_particle<true, COUPLED>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u, masks,
+                                           bc);
+        return;
+    }
+    const float4* rec = nbr_records(masks, n);
+    tile_fill(g, rec, 2, recA);
+    tile_fill(g, rec + 1, 2, recB);
+    tile_fill_wait();
+    if (!have) return;
+    F3 a = f3(0.0f, 0.0f, 0.0f);
+    if (pos.w > 0.0f) {
+        if (cnt > kListMax) cnt = kListOverflow;  // (a count the density pass cannot write reads the masks)
+        a = tile_forces_particle<COUPLED>(recA, recB, i, pos, vel, dpi, cnt, l0, l1, sp, sv, rho_p, start, end, n, m,
+                                          mu, cp, kc, masks, bc);
+    }
+    write_back(oid, n, pos, vel, a, cp, vout, pout);
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
+
+// Which form the last density pass wrote into a neighbour buffer: lists (the
+// tiled pass) or row masks only.  The force pass follows it, whatever the
+// schedule is when it runs, so the two passes always agree on the format.
+static std::mutex g_nbr_mu;
+static std::unordered_map<const void*, bool>& nbr_tiled() {
+    static std::unordered_map<const void*, bool> m;
+    return m;
+}
+static void note_nbr_format(const void* masks, bool tiled) {
+    std::lock_guard<std::mutex> lock(g_nbr_mu);
+    nbr_tiled()[masks] = tiled;
+}
+static bool nbr_format_tiled(const void* masks) {
+    std::lock_guard<std::mutex> lock(g_nbr_mu);
+    auto it = nbr_tiled().find(masks);
+    return it != nbr_tiled().end() && it->second;
+}
 static bool have_params(const char* who) {
     if (!state().have_common || !state().have_kernels) {
         set_error(FG_ERR_NOT_INITIALIZED, "%s: call gInitializeCommonFluidParams and gInitializeKernelParams first",
@@ -298,7 +430,11 @@ int launch_density(float4* sp, float2* rho_p, uint32_t* start, uint32_t* end, ui
     if (!stencil_covers_h("density pass")) return FG_ERR_UNSUPPORTED;
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
-    if (use_fast() && masks && st.flat_density)
+    if (masks) note_nbr_format(masks, use_fast() && st.tiled);
+    if (use_fast() && masks && st.tiled)
+        hipLaunchKernelGGL(k_density_tile, dim3(grid_for(n, kTileLanes)), dim3(kTileLanes), 0, st.stream, sp, sv,
+                           rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
+    else if (use_fast() && masks && st.flat_density)
         FG_SPH_LAUNCH(k_density_flat, true, n, sp, sv, rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
     else if (use_fast() && masks)
         FG_SPH_LAUNCH(k_density_fast, true, n, sp, sv, rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
@@ -323,7 +459,14 @@ int launch_forces(float4* sp, float4* sv, float2* rho_p, uint32_t* start, uint32
     const bool coupled = bc.coupled != 0;
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
-    if (use_fast() && masks && st.flat_forces) {
+    if (use_fast() && masks && nbr_format_tiled(masks)) {
+        if (coupled)
+            hipLaunchKernelGGL(k_forces_tile<true>, dim3(grid_for(n, kTileLanes)), dim3(kTileLanes), 0, st.stream, sp,
+                               sv, rho_p, start, end, vout, pout, id, n, m, mu, st.common, st.kernels, masks, bc);
+        else
+            hipLaunchKernelGGL(k_forces_tile<false>, dim3(grid_for(n, kTileLanes)), dim3(kTileLanes), 0, st.stream,
+                               sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, st.common, st.kernels, masks, bc);
+    } else if (use_fast() && masks && st.flat_forces) {
         if (coupled)
             FG_SPH_LAUNCH(k_forces_flat, true, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, st.common,
                           st.kernels, masks, bc);
@@ -443,8 +586,9 @@ int fg_sph_forces_integrate(fg_float4* orderedPositions, fg_float4* orderedVeloc
 }
 
 size_t fg_neighbor_mask_bytes(unsigned int numParticles) {
-    // 9 row masks, then (from a 256-byte boundary) a 32-byte neighbour record per particle
-    return nbr_record_offset(numParticles) + 2 * sizeof(float4) * (size_t)numParticles;
+    // 9 row masks, then (each from a 256-byte boundary) a 32-byte neighbour
+    // record, a u16 list count and a list of kListMax u16 per particle
+    return nbr_buffer_bytes(numParticles);
 }
 
 int fg_sph_density_nbr(fg_float4* orderedPositions, const fg_float4* orderedVelocity, fg_float2* mass_pressure,

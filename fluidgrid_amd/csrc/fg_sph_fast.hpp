@@ -386,6 +386,74 @@ __device__ __forceinline__ void pair_term_rp(PairAcc& A, uint32_t i, uint32_t j,
 }
 #pragma clang fp contract(fast)
 
+// The row loop of the force pass: the 9 stencil rows in turn, each row's
+// neighbours from its mask (ascending, two per iteration) or, without a mask
+// or for a row longer than 64 candidates, by re-testing every candidate.
+template <bool MASKS>
+__device__ __forceinline__ void forces_rowloop(PairAcc& A, uint32_t i, const float4& pos, const float4& vel,
+                                               float2 dpi, const float4* __restrict__ sp,
+                                               const float4* __restrict__ sv, const float2* __restrict__ rho_p,
+                                               const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                               uint32_t n, const fg_common_fluid_params& cp, uint32_t key_base,
+                                               const unsigned long long* __restrict__ masks) {
+    const float h = cp.kernel_radius, hh = h * h;
+    const float s = cp.simulation_scale;
+    const float hu = h / s, hhu = hu * hu;
+    const float cut = hh * 1.0001f;
+    const RowWalker w = make_rows(pos, cp);
+    const float4* rec = MASKS ? nbr_records(masks, n) : nullptr;
+    for (int row = 0; row < 9; row++) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        // The row's mask is read before its range is known, so the two
+        // reads share one memory round trip (the asm keeps the compiler
+        // from sinking the load behind the range test).
+        unsigned long long mrow = 0ull;
+        if (MASKS) {
+            mrow = masks[(size_t)row * n + i];
+            asm volatile("" ::"v"(mrow));
+        }
+        uint32_t rb, re;
+        if (!row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) continue;
+        if (MASKS && re - rb <= kRowMaskBits) {
+            unsigned long long mm = mrow;
+            // the particle itself is in its own row's mask; it adds nothing
+            if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
+            // The mask as two 32-bit words, low word first (the same
+            // ascending order): 32-bit bit scans and clears, and rows of
+            // at most 32 candidates never enter the second loop.
+            uint32_t word = (uint32_t)mm, base = rb;
+            for (int half = 0; half < 2; half++) {
+                while (word) {
+                    const uint32_t k1 = (uint32_t)__builtin_ctz(word);
+                    word &= word - 1u;
+                    const bool two = word != 0u;
+                    const uint32_t k2 = two ? (uint32_t)__builtin_ctz(word) : k1;
+                    word &= word - 1u;  // (0 stays 0)
+                    const uint32_t j1 = base + k1, j2 = base + k2;
+                    const float4 a1 = rec[2 * (size_t)j1], b1 = rec[2 * (size_t)j1 + 1];
+                    const float4 a2 = rec[2 * (size_t)j2], b2 = rec[2 * (size_t)j2 + 1];
+                    const float3 q1 = make_float3(a1.x, a1.y, a1.z), q2 = make_float3(a2.x, a2.y, a2.z);
+                    const float3 v1 = make_float3(b1.x, b1.y, b1.z), v2 = make_float3(b2.x, b2.y, b2.z);
+                    const float2 d1 = make_float2(a1.w, b1.w), d2 = make_float2(a2.w, b2.w);
+                    // Keeps the second pair's loads beside the first pair's (one
+                    // memory round trip per iteration): without a use here the
+                    // compiler sinks them into the `two` branch below.
+                    asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(v2.x), "v"(v2.y), "v"(v2.z),
+                                 "v"(d2.x), "v"(d2.y));
+                    pair_term_masked(A, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s);
+                    if (two) pair_term_masked(A, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s);
+                }
+                word = (uint32_t)(mm >> 32);  // then the high word
+                base = rb + 32u;
+                if (!word) break;
+            }
+        } else {
+            for (uint32_t j = rb; j < re; j++)
+                pair_term_rp(A, i, j, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), rho_p[j], h, s, hhu);
+        }
+    }
+}
+
 // Per-lane row table of the flat force schedule, in LDS: for each of the 9
 // stencil rows, the sorted index of its first particle and its neighbour
 // mask (0 for a culled row).  The flat schedule takes rows of at most 32
@@ -428,7 +496,6 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
     if (pos.w > 0.0f) {
         const float h = cp.kernel_radius, hh = h * h;
         const float s = cp.simulation_scale;
-        const float hu = h / s, hhu = hu * hu;
         const float cut = hh * 1.0001f;
         const float2 dpi = rho_p[i];
         const RowWalker w = make_rows(pos, cp);
@@ -491,56 +558,7 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                 }
             }
         }
-        for (int row = 0; rowloop && row < 9; row++) {
-            const int dz = row / 3 - 1, dy = row % 3 - 1;
-            // The row's mask is read before its range is known, so the two
-            // reads share one memory round trip (the asm keeps the compiler
-            // from sinking the load behind the range test).
-            unsigned long long mrow = 0ull;
-            if (MASKS) {
-                mrow = masks[(size_t)row * n + i];
-                asm volatile("" ::"v"(mrow));
-            }
-            uint32_t rb, re;
-            if (!row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) continue;
-            if (MASKS && re - rb <= kRowMaskBits) {
-                unsigned long long mm = mrow;
-                // the particle itself is in its own row's mask; it adds nothing
-                if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
-                // The mask as two 32-bit words, low word first (the same
-                // ascending order): 32-bit bit scans and clears, and rows of
-                // at most 32 candidates never enter the second loop.
-                uint32_t word = (uint32_t)mm, base = rb;
-                for (int half = 0; half < 2; half++) {
-                    while (word) {
-                        const uint32_t k1 = (uint32_t)__builtin_ctz(word);
-                        word &= word - 1u;
-                        const bool two = word != 0u;
-                        const uint32_t k2 = two ? (uint32_t)__builtin_ctz(word) : k1;
-                        word &= word - 1u;  // (0 stays 0)
-                        const uint32_t j1 = base + k1, j2 = base + k2;
-                        const float4 a1 = rec[2 * (size_t)j1], b1 = rec[2 * (size_t)j1 + 1];
-                        const float4 a2 = rec[2 * (size_t)j2], b2 = rec[2 * (size_t)j2 + 1];
-                        const float3 q1 = make_float3(a1.x, a1.y, a1.z), q2 = make_float3(a2.x, a2.y, a2.z);
-                        const float3 v1 = make_float3(b1.x, b1.y, b1.z), v2 = make_float3(b2.x, b2.y, b2.z);
-                        const float2 d1 = make_float2(a1.w, b1.w), d2 = make_float2(a2.w, b2.w);
-                        // Keeps the second pair's loads beside the first pair's (one
-                        // memory round trip per iteration): without a use here the
-                        // compiler sinks them into the `two` branch below.
-                        asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(v2.x), "v"(v2.y), "v"(v2.z),
-                                     "v"(d2.x), "v"(d2.y));
-                        pair_term_masked(A, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s);
-                        if (two) pair_term_masked(A, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s);
-                    }
-                    word = (uint32_t)(mm >> 32);  // then the high word
-                    base = rb + 32u;
-                    if (!word) break;
-                }
-            } else {
-                for (uint32_t j = rb; j < re; j++)
-                    pair_term_rp(A, i, j, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), rho_p[j], h, s, hhu);
-            }
-        }
+        if (rowloop) forces_rowloop<MASKS>(A, i, pos, vel, dpi, sp, sv, rho_p, start, end, n, cp, key_base, masks);
         // pressure: (s d_u / r) = d_u / r_u, so no scale factor remains
         const float iri = 1.0f / dpi.x;
         const float cP = -kc.kSpikyGradient * iri;

@@ -114,6 +114,8 @@ int fg_bsim_step(int steps) {
     }
 }
 
+void* fg_bsim_stream(void) { return batch() ? reinterpret_cast<void*>(batch()->stream()) : nullptr; }
+
 int fg_bsim_synchronize(void) {
     if (!batch()) return FG_ERR_NOT_INITIALIZED;
     try {

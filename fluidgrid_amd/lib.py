@@ -71,6 +71,7 @@ SIGNATURES = {
     "fg_event_create": (i32, [C.POINTER(C.c_void_p)]),
     "fg_event_destroy": (i32, [vp]),
     "fg_event_record": (i32, [vp]),
+    "fg_event_record_stream": (i32, [vp, vp]),
     "fg_event_elapsed_ms": (i32, [vp, vp, C.POINTER(C.c_float)]),
     # simulator
     "fg_sim_init": (i32, [C.POINTER(abi.CommonFluidParams), i32, i32]),
@@ -134,10 +135,13 @@ SIGNATURES = {
     "fg_get_tuning": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fg_set_schedule": (i32, [i32, i32]),
     "fg_get_schedule": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fg_set_tiled": (i32, [i32]),
+    "fg_get_tiled": (i32, [C.POINTER(C.c_int)]),
     "fg_batch_hash": (i32, [vp, u32, vp, vp, vp, u32]),
     "fg_batch_density_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, u32]),
     "fg_batch_forces_integrate_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32]),
     "fg_bsim_create": (i32, [u32, vp, vp, vp, vp, vp]),
+    "fg_bsim_stream": (vp, []),
     "fg_bsim_add_bodies": (i32, [u32, vp, vp, u32, vp, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
     "fg_bsim_body_state": (i32, [C.POINTER(C.c_uint), vp, vp, vp, vp]),
     "fg_bsim_set_options": (i32, [i32, i32]),
@@ -243,6 +247,35 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class StreamTimer:
+    """HIP events recorded on a given stream around a region of work; the
+    elapsed milliseconds are read after the stop event completes."""
+
+    def __init__(self, stream):
+        L = load()
+        self.stream = stream
+        self.e0, self.e1 = C.c_void_p(), C.c_void_p()
+        check(L.fg_event_create(C.byref(self.e0)), "event")
+        check(L.fg_event_create(C.byref(self.e1)), "event")
+
+    def start(self):
+        check(load().fg_event_record_stream(self.e0, self.stream), "event record")
+
+    def stop_ms(self):
+        L = load()
+        check(L.fg_event_record_stream(self.e1, self.stream), "event record")
+        ms = C.c_float()
+        check(L.fg_event_elapsed_ms(self.e0, self.e1, C.byref(ms)), "elapsed")
+        return float(ms.value)
+
+    def close(self):
+        L = load()
+        for e in (self.e0, self.e1):
+            if e:
+                L.fg_event_destroy(e)
+        self.e0 = self.e1 = None
 
 
 def init_params(common, kernels, key_mode, math_mode=abi.MATH_FAST):
@@ -448,6 +481,9 @@ class BatchSimulator:
 
     def step(self, steps=1):
         check(load().fg_bsim_step(int(steps)), "fg_bsim_step")
+
+    def stream(self):
+        return load().fg_bsim_stream()
 
     def synchronize(self):
         check(load().fg_bsim_synchronize(), "fg_bsim_synchronize")

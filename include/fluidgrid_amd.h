@@ -284,6 +284,15 @@ int   fg_get_tuning(int* sphBlock, int* sortDigitBits);
  * order, and so every result bit, is the same. */
 int   fg_set_schedule(int flatDensity, int flatForces);
 int   fg_get_schedule(int* flatDensity, int* flatForces);
+/* LDS-tiled masked passes (default off): fg_sph_density_nbr stages each
+ * 128-particle tile's stencil rows in LDS and writes a neighbour list per
+ * particle; the force pass that reads the buffer walks the lists over
+ * LDS-staged neighbour records.  They take precedence over the schedules
+ * above and give the same bits.  The force pass follows the form the last
+ * density pass wrote into its neighbour buffer, whatever is set when it
+ * runs. */
+int   fg_set_tiled(int enable);
+int   fg_get_tiled(int* enable);
 int   fg_get_common_fluid_params(fg_common_fluid_params* out);
 int   fg_get_kernels_cache(fg_kernels_cache* out);
 
@@ -550,6 +559,7 @@ int fg_synchronize(void);                  /* the selected stream */
 int fg_event_create(void** ev);
 int fg_event_destroy(void* ev);
 int fg_event_record(void* ev);             /* on the selected stream */
+int fg_event_record_stream(void* ev, void* stream);  /* on a given hipStream_t (e.g. fg_sim_stream()) */
 int fg_event_elapsed_ms(void* start, void* stop, float* ms);
 
 #ifdef __cplusplus

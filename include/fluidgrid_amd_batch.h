@@ -34,6 +34,7 @@ int fg_bsim_body_state(unsigned int* numBodies, fg_float4* positions, fg_float4*
 int fg_bsim_set_options(int use_graph, int stage_timing);
 int fg_bsim_step(int steps);                      /* asynchronous */
 int fg_bsim_synchronize(void);
+void* fg_bsim_stream(void);                       /* hipStream_t of the batch (null without one) */
 int fg_bsim_download(fg_float4* positions, fg_float4* velocities);
 /* (max(1,rho), p) in sorted order and the sorted particle ids of the last step. */
 int fg_bsim_download_density(fg_float2* rho_p, unsigned int* particleId);

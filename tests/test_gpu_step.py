@@ -135,11 +135,12 @@ def test_trajectory_100_steps_appmain_ref_xor(fglib, orc):
 
 @pytest.mark.parametrize("which", ["cfg0", "compressed", "coupled"])
 def test_schedules_and_block_sizes_bit_identical(fglib, which):
-    """The flat and row-by-row schedules of the density and force passes,
-    at every workgroup size, give the same bits after 10 graph steps
-    ('compressed' has rows longer than the mask width, which the flat force
-    schedule hands to the row loop; 'coupled' runs the coupled force kernels
-    with a torus resting in the surface)."""
+    """The LDS-tiled passes and the flat and row-by-row schedules of the
+    density and force passes, at every workgroup size, give the same bits
+    after 10 graph steps ('compressed' has rows longer than the mask width,
+    which the flat force schedule hands to the row loop and the tiled passes
+    to their mask fallback, and tiles past the LDS capacity; 'coupled' runs
+    the coupled force kernels with a torus resting in the surface)."""
     bodies = None
     if which == "cfg0":
         s = scenes.config_scene(0)
@@ -154,7 +155,9 @@ def test_schedules_and_block_sizes_bit_identical(fglib, which):
     L = fglib.load()
     ref = None
     try:
-        for fd, ff, blk in ((0, 0, 256), (1, 0, 64), (0, 1, 128), (1, 1, 512), (1, 1, 256)):
+        for tiled, fd, ff, blk in ((0, 0, 0, 256), (0, 1, 0, 64), (0, 0, 1, 128), (0, 1, 1, 512), (0, 1, 1, 256),
+                                   (1, 0, 0, 256)):
+            L.fg_set_tiled(tiled)
             L.fg_set_schedule(fd, ff)
             L.fg_set_tuning(blk, 8)
             sim = fglib.Simulator(s, fused=True, use_graph=True)
@@ -168,7 +171,8 @@ def test_schedules_and_block_sizes_bit_identical(fglib, which):
             if ref is None:
                 ref = (pos, vel)
             else:
-                assert np.array_equal(pos, ref[0]) and np.array_equal(vel, ref[1]), (fd, ff, blk)
+                assert np.array_equal(pos, ref[0]) and np.array_equal(vel, ref[1]), (tiled, fd, ff, blk)
     finally:
+        L.fg_set_tiled(1)
         L.fg_set_schedule(0, 0)
         L.fg_set_tuning(256, 8)
