@@ -47,6 +47,26 @@ __global__ void __launch_bounds__(256) k_hash(uint32_t* __restrict__ key, uint32
     if (id) id[i] = i;
 }
 
+// k_hash without ids, where key[] holds on entry the previous sorted keys:
+// before its key is replaced, each thread clears cellStart of its old key
+// when that key starts a run inside the block (runs that cross a block
+// boundary are cleared once per block, which is harmless).  Neighbouring old
+// keys are exchanged through LDS, so no thread reads a key another thread
+// overwrites.
+__global__ void __launch_bounds__(256) k_hash_clear(uint32_t* __restrict__ key, const float4* __restrict__ pos,
+                                                    uint32_t n, GridDesc g, uint32_t* __restrict__ start,
+                                                    uint32_t nb) {
+    __shared__ uint32_t s_old[256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t old = i < n ? key[i] : kEmpty;
+    s_old[threadIdx.x] = old;
+    __syncthreads();
+    if (i >= n) return;
+    if (old < nb && (threadIdx.x == 0 || s_old[threadIdx.x - 1] != old)) start[old] = kEmpty;
+    const float4 p = pos[i];
+    key[i] = grid_key(p.x, p.y, p.z, g);
+}
+
 // K_HashGrid.cu:43-63 with K_MathUtils.cu:160-181
 __global__ void __launch_bounds__(256) k_hash_with_offset(uint32_t* __restrict__ key, uint32_t* __restrict__ id,
                                                           const float4* __restrict__ offset,
@@ -93,6 +113,18 @@ __global__ void __launch_bounds__(256) k_cell_offsets(uint32_t* __restrict__ sta
     if (WITH_VEL) svel[i] = vel[src];
 }
 
+// cellStart[k] = kEmpty for every key k of a sorted key array (the cells
+// that array occupied): with the rest of cellStart already kEmpty, this
+// clears it for the next cell-range pass at a cost of one write per
+// occupied cell instead of one per bucket.
+__global__ void __launch_bounds__(256) k_clear_occupied(uint32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ key, uint32_t n, uint32_t nb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = key[i];
+    if (k < nb && (i == 0 || key[i - 1] != k)) start[k] = kEmpty;
+}
+
 }  // namespace fg
 
 using namespace fg;
@@ -131,6 +163,21 @@ int fg_calc_hash_key_only(unsigned int* hashKey, fg_float4* positions, unsigned 
     hipLaunchKernelGGL(k_hash, dim3(grid_for(numParticles, 256)), dim3(256), 0, st.stream, hashKey,
                        (uint32_t*)nullptr, FG_F4(positions), numParticles, g);
     return check_launch("fg_calc_hash_key_only") ? FG_OK : fg_peek_error();
+}
+
+int fg_calc_hash_key_clear(unsigned int* hashKey, fg_float4* positions, unsigned int numParticles,
+                           unsigned int numBuckets, fg_float3 cell_size, unsigned int* gridStart) {
+    if (numParticles == 0) return FG_OK;
+    if (!hashKey || !positions || !gridStart) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_calc_hash_key_clear: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    const State& st = state();
+    GridDesc g = fluid_grid_desc(st.common, cell_size, numBuckets, st.key_mode);
+    if (!validate_grid(g, "fg_calc_hash_key_clear")) return FG_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(numParticles, 256)), dim3(256), 0, st.stream, hashKey,
+                       FG_F4(positions), numParticles, g, gridStart, numBuckets);
+    return check_launch("fg_calc_hash_key_clear") ? FG_OK : fg_peek_error();
 }
 
 // K_HashGrid.cu:221-233
@@ -172,6 +219,37 @@ void gFindCellOffsetOrderingPosVel(unsigned int* gridPosStart, unsigned int* gri
                            FG_F4(orderedParticlePositions), FG_F4(particleVel), FG_F4(particleOrderedVel),
                            numParticles, numBuckets);
     check_launch("gFindCellOffsetOrderingPosVel");
+}
+
+int fg_clear_occupied_cells(unsigned int* gridStart, const unsigned int* sortedKeys, unsigned int numParticles,
+                            unsigned int numBuckets) {
+    if (numParticles == 0) return FG_OK;
+    if (!gridStart || !sortedKeys) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_clear_occupied_cells: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    hipLaunchKernelGGL(k_clear_occupied, dim3(grid_for(numParticles, 256)), dim3(256), 0, state().stream, gridStart,
+                       sortedKeys, numParticles, numBuckets);
+    return check_launch("fg_clear_occupied_cells") ? FG_OK : fg_peek_error();
+}
+
+int fg_cell_offsets_pos_vel(unsigned int* gridPosStart, unsigned int* gridPosEnd, unsigned int* hashTable,
+                            unsigned int* particleID, fg_float4* particlePositions, fg_float4* orderedParticlePositions,
+                            fg_float4* particleVel, fg_float4* particleOrderedVel, unsigned int numParticles,
+                            unsigned int numBuckets, int fillStart) {
+    if (!gridPosStart || (numParticles && (!gridPosEnd || !hashTable || !particleID || !particlePositions ||
+                                           !orderedParticlePositions || !particleVel || !particleOrderedVel))) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_cell_offsets_pos_vel: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    const State& st = state();
+    if (fillStart) launch_fill_u32(gridPosStart, kEmpty, numBuckets, st.stream);
+    if (numParticles)
+        hipLaunchKernelGGL(k_cell_offsets<true>, dim3(grid_for(numParticles, 256)), dim3(256), 0, st.stream,
+                           gridPosStart, gridPosEnd, hashTable, particleID, FG_F4(particlePositions),
+                           FG_F4(orderedParticlePositions), FG_F4(particleVel), FG_F4(particleOrderedVel),
+                           numParticles, numBuckets);
+    return check_launch("fg_cell_offsets_pos_vel") ? FG_OK : fg_peek_error();
 }
 
 // K_HashGrid.cu:287-318

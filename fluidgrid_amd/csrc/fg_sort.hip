@@ -24,8 +24,15 @@
 
 namespace fg {
 
+#ifndef FG_SORT_ITEMS
+#define FG_SORT_ITEMS 16
+#endif
+#ifndef FG_SORT_LOOKBACK
+#define FG_SORT_LOOKBACK 8
+#endif
+
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = FG_SORT_ITEMS;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kMaxPasses = 4;
@@ -33,7 +40,7 @@ constexpr int kMaxRadix = 256;
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagInclusive = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1;
-constexpr int kLookback = 8;  // predecessors read per look-back round trip
+constexpr int kLookback = FG_SORT_LOOKBACK;  // predecessors read per look-back round trip
 
 struct SortPlan {
     int passes;

@@ -59,6 +59,11 @@ GPUSPHFluid::GPUSPHFluid(Application::FluidParams fluidParams, Emitter::IEmitter
     hip_check(hipMemset(d_mass_pressure, 0, sizeof(float2) * n), "hipMemset mass_pressure");
     hip_check(hipMemset(d_forces, 0, sizeof(float4) * n), "hipMemset forces");
     hip_check(hipMemset(d_gridPosEnd, 0, sizeof(unsigned int) * mNumBuckets), "hipMemset gridPosEnd");
+    // Every cell empty and no key occupying one: the grid state the fused
+    // step keeps between steps (it clears only the cells the last sorted keys
+    // occupied, fg_clear_occupied_cells, instead of filling every bucket).
+    hip_check(hipMemset(d_gridPosStart, 0xFF, sizeof(unsigned int) * mNumBuckets), "hipMemset gridPosStart");
+    hip_check(hipMemset(d_hashKey, 0xFF, sizeof(unsigned int) * (n > 0 ? n : 1)), "hipMemset hashKey");
 
     std::vector<float4> host(n > 0 ? n : 1);
     if (mEmitter) mEmitter->reset(host.data(), n);

@@ -119,9 +119,12 @@ void HeterogeneousSimulator::findFluidNeighbourhood(Fluid::GPUSPHFluid* fluid) {
     const unsigned int nb = (unsigned int)cp.getNumBuckets();
     const unsigned int n = (unsigned int)fluid->getNumParticles();
     if (mOptions.fused) {
-        // Keys only; the first sort pass generates the identity ids.
-        fg_calc_hash_key_only(fluid->getHashKey(), reinterpret_cast<fg_float4*>(fluid->getPosition()), n, nb,
-                              abi3(cp.cell_size));
+        // Keys only; the first sort pass generates the identity ids.  The
+        // last step's sorted keys, which the new keys replace, name the only
+        // cells whose start is set: clearing those leaves cellStart empty for
+        // this step's cell pass, which then skips the fill of every bucket.
+        fg_calc_hash_key_clear(fluid->getHashKey(), reinterpret_cast<fg_float4*>(fluid->getPosition()), n, nb,
+                               abi3(cp.cell_size), fluid->getHashStart());
     } else {
         gCalcHashKey(fluid->getHashKey(), fluid->getParticleID(), reinterpret_cast<fg_float4*>(fluid->getPosition()),
                      n, nb, abi3(cp.cell_size));
@@ -130,6 +133,14 @@ void HeterogeneousSimulator::findFluidNeighbourhood(Fluid::GPUSPHFluid* fluid) {
     fg_sort_pairs_ws(fluid->getHashKey(), fluid->getParticleID(), n, key_bits_for(nb), fluid->getSortWorkspace(),
                      fluid->getSortWorkspaceBytes(), mOptions.fused ? 1 : 0);
     if (mOptions.stage_timing) (void)hipEventRecord(mEv[2], mStream);
+    if (mOptions.fused) {
+        fg_cell_offsets_pos_vel(fluid->getHashStart(), fluid->getHashEnd(), fluid->getHashKey(),
+                                fluid->getParticleID(), reinterpret_cast<fg_float4*>(fluid->getPosition()),
+                                reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
+                                reinterpret_cast<fg_float4*>(fluid->getVelocity()),
+                                reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()), n, nb, 0);
+        return;
+    }
     gFindCellOffsetOrderingPosVel(fluid->getHashStart(), fluid->getHashEnd(), fluid->getHashKey(),
                                   fluid->getParticleID(), reinterpret_cast<fg_float4*>(fluid->getPosition()),
                                   reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),

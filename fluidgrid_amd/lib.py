@@ -54,6 +54,9 @@ SIGNATURES = {
     "fg_sort_pairs_ws": (i32, [vp, vp, u32, i32, vp, sz, i32]),
     "fg_sort_pairs_noalloc": (i32, [vp, vp, u32, i32, i32]),
     "fg_calc_hash_key_only": (i32, [vp, vp, u32, u32, abi.Float3]),
+    "fg_clear_occupied_cells": (i32, [vp, vp, u32, u32]),
+    "fg_calc_hash_key_clear": (i32, [vp, vp, u32, u32, abi.Float3, vp]),
+    "fg_cell_offsets_pos_vel": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, u32, u32, i32]),
     "fg_sph_forces_integrate": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32]),
     "fg_neighbor_mask_bytes": (sz, [u32]),
     "fg_sph_density_nbr": (i32, [vp, vp, vp, vp, vp, vp, u32, f32, f32, f32]),

@@ -321,6 +321,27 @@ int fg_sort_pairs_ws(unsigned int* keys, unsigned int* values, unsigned int n, i
 int fg_sort_pairs_noalloc(unsigned int* keys, unsigned int* values, unsigned int n, int keyBits,
                           int values_are_iota);
 
+/* cellStart[k] := 0xFFFFFFFF for every key k in sortedKeys (the cells a
+ * sorted key array occupied).  When the rest of cellStart is already
+ * 0xFFFFFFFF, this resets the grid for the next cell-range pass with one
+ * write per occupied cell instead of the fill of every bucket
+ * (gFindCellOffsetOrderingPosVel, K_HashGrid.cu:254-255).  Keys >= numBuckets
+ * are skipped. */
+int fg_clear_occupied_cells(unsigned int* gridStart, const unsigned int* sortedKeys, unsigned int numParticles,
+                            unsigned int numBuckets);
+/* fg_calc_hash_key_only fused with fg_clear_occupied_cells: hashKey holds
+ * on entry the previous step's sorted keys (whose cells are cleared in
+ * gridStart) and on exit the new unsorted keys of positions. */
+int fg_calc_hash_key_clear(unsigned int* hashKey, fg_float4* positions, unsigned int numParticles,
+                           unsigned int numBuckets, fg_float3 cell_size, unsigned int* gridStart);
+/* gFindCellOffsetOrderingPosVel with the cellStart fill optional
+ * (fillStart = 0: the caller guarantees cellStart is 0xFFFFFFFF outside the
+ * cells of these keys, e.g. after fg_clear_occupied_cells). */
+int fg_cell_offsets_pos_vel(unsigned int* gridPosStart, unsigned int* gridPosEnd, unsigned int* hashTable,
+                            unsigned int* particleID, fg_float4* particlePositions, fg_float4* orderedParticlePositions,
+                            fg_float4* particleVel, fg_float4* particleOrderedVel, unsigned int numParticles,
+                            unsigned int numBuckets, int fillStart);
+
 /* gCalcHashKey without the id output (the sort that follows generates the
  * identity ids itself, see fg_sort_pairs_iota). */
 int fg_calc_hash_key_only(unsigned int* hashKey, fg_float4* positions, unsigned int numParticles,

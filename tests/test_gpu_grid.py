@@ -180,3 +180,55 @@ def test_grid_1m_properties(fglib, orc):
     assert np.all(sk[start[occ]] == occ)
     assert np.all(sk[end[occ] - 1] == occ)
     assert np.array_equal(g["spos"], s.pos[sp])
+
+
+@pytest.mark.parametrize("which", ["cfg0", "appmain"])
+@pytest.mark.parametrize("fused", [False, True])
+def test_clear_occupied_then_no_fill_offsets(fglib, orc, which, fused):
+    """Two grids on the same buffers: after the first, clearing the cells its
+    sorted keys occupied (alone, or inside the next hash) leaves every cell
+    start empty, and the second grid built without the bucket fill equals
+    the oracle's (which fills)."""
+    s = scenes.config_scene(0) if which == "cfg0" else scenes.appmain_scene()
+    L = fglib.load()
+    g0 = _grid_gpu(fglib, s)  # leaves init_params set for s
+    nb = s.num_buckets
+    dstart = fglib.DeviceArray.from_numpy(g0["start"])
+    dkey = fglib.DeviceArray.from_numpy(g0["skey"])
+
+    rng = np.random.default_rng(7)
+    pos2 = s.pos.copy()
+    pos2[:, :3] = pos2[rng.permutation(s.n), :3] + rng.normal(0, 0.3 * s.common.cell_size.x, (s.n, 3))
+    vel2 = rng.normal(0, 1, (s.n, 4)).astype(np.float32)
+    dpos, dvel = fglib.DeviceArray.from_numpy(pos2.astype(np.float32)), fglib.DeviceArray.from_numpy(vel2)
+    did = fglib.DeviceArray.from_numpy(np.arange(s.n, dtype=np.uint32))
+    dend = fglib.DeviceArray((nb,), np.uint32)
+    dend.fill_u32(0xABCDEF)
+    dspos, dsvel = fglib.DeviceArray((s.n, 4), np.float32), fglib.DeviceArray((s.n, 4), np.float32)
+    cell = abi.Float3(*s.common.cell_size)
+    if fused:
+        assert L.fg_calc_hash_key_clear(dkey.ptr, dpos.ptr, s.n, nb, cell, dstart.ptr) == 0
+        fglib.check(None, "hash+clear")
+    else:
+        assert L.fg_clear_occupied_cells(dstart.ptr, dkey.ptr, s.n, nb) == 0
+        fglib.check(None, "clear")
+        L.gCalcHashKey(dkey.ptr, did.ptr, dpos.ptr, s.n, nb, cell)
+    assert np.all(dstart.numpy() == EMPTY)
+    L.gSort(dkey.ptr, did.ptr, s.n)
+    assert L.fg_cell_offsets_pos_vel(dstart.ptr, dend.ptr, dkey.ptr, did.ptr, dpos.ptr, dspos.ptr, dvel.ptr,
+                                     dsvel.ptr, s.n, nb, 0) == 0
+    fglib.check(None, "grid2")
+    key, pid = orc.hash_keys(pos2.astype(np.float32), nb, tuple(s.common.cell_size), s.common, s.key_mode)
+    sk, sp = orc.sort_pairs(key, pid)
+    start, end, spos, svel = orc.cell_offsets(sk, sp, pos2.astype(np.float32), vel2, nb,
+                                              np.full(nb, 0xABCDEF, np.uint32))
+    assert np.array_equal(dkey.numpy(), sk)
+    assert np.array_equal(dstart.numpy(), start)
+    assert np.array_equal(dend.numpy(), end)
+    assert np.array_equal(dspos.numpy(), spos)
+    assert np.array_equal(dsvel.numpy(), svel)
+    # null pointers are refused, not dereferenced
+    assert L.fg_clear_occupied_cells(None, dkey.ptr, s.n, nb) != 0
+    L.fg_last_error()
+    assert L.fg_calc_hash_key_clear(dkey.ptr, dpos.ptr, s.n, nb, cell, None) != 0
+    L.fg_last_error()

@@ -173,6 +173,6 @@ def test_schedules_and_block_sizes_bit_identical(fglib, which):
             else:
                 assert np.array_equal(pos, ref[0]) and np.array_equal(vel, ref[1]), (tiled, fd, ff, blk)
     finally:
-        L.fg_set_tiled(1)
+        L.fg_set_tiled(0)
         L.fg_set_schedule(0, 0)
         L.fg_set_tuning(256, 8)
