@@ -16,7 +16,8 @@ The default workload is the 1,048,576-particle dam break coupled two-way to a
 rigid torus (R 0.15 m, r 0.05 m, ~20K boundary spheres, density 500) dropped
 from 0.3 m above the fluid.  Further lines in `config`: the 8,388,608-particle
 step with four rigid boxes (`also_8M`) and without them
-(`also_8M_fluid_only`); the 33,554,432-particle step with 64 rigid bodies and
+(`also_8M_fluid_only`), and with the boxes in the mixed-precision form
+(`mixed_8M`, with its ratio to `also_8M`); the 33,554,432-particle step with 64 rigid bodies and
 a 0.5 m/s inflow (`also_33M`); the batched form over 256 independent
 scenes, fixed and autotuned, beside the same scenes stepped one at a time
 (`batched`); and the mesh sampler's winding-number kernel (`mesh_sampler`).
@@ -103,8 +104,8 @@ def median_index(values):
     return sorted(range(len(values)), key=lambda k: values[k])[len(values) // 2]
 
 
-def new_sim(scene, fused):
-    sim = lib.Simulator(scene, fused=fused, use_graph=True)
+def new_sim(scene, fused, precision=abi.PRECISION_FP32):
+    sim = lib.Simulator(scene, fused=fused, use_graph=True, precision=precision)
     if scene.info.get("bodies") is not None:
         sim.add_bodies(scene.info["bodies"])
     if scene.info.get("inflow") is not None:
@@ -112,7 +113,7 @@ def new_sim(scene, fused):
     return sim
 
 
-def measure(scene, steps, warmup, fused=True):
+def measure(scene, steps, warmup, fused=True, precision=abi.PRECISION_FP32):
     """REPEATS runs, each a fresh simulator from the scene's initial state:
     `warmup` untimed steps, then one timed region of exactly `steps` steps;
     the median run is reported.  The last run also gives the drift report,
@@ -121,7 +122,7 @@ def measure(scene, steps, warmup, fused=True):
     bodies = scene.info.get("bodies")
     ev_all, wall_all = [], []
     for rep in range(REPEATS):
-        sim = new_sim(scene, fused)
+        sim = new_sim(scene, fused, precision)
         try:
             last = rep == REPEATS - 1
             if last:
@@ -417,6 +418,8 @@ def main():
     ap.add_argument("--untiled", action="store_true",
                     help="run the masked density/force passes without LDS tiles (row-loop schedules)")
     ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
+    ap.add_argument("--mixed", action="store_true",
+                    help="the main line in the mixed-precision form (FG_PRECISION_MIXED) instead of plain FP32")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-sampler line")
@@ -458,7 +461,8 @@ def main():
     fused = not a.unfused
     log(f"scene {scene.name}: n={scene.n:,} buckets={scene.num_buckets:,} "
         f"body spheres={bodies.m if bodies is not None else 0:,}")
-    r = measure(scene, a.steps, a.warmup, fused)
+    prec = abi.PRECISION_MIXED if a.mixed else abi.PRECISION_FP32
+    r = measure(scene, a.steps, a.warmup, fused, prec)
     n = scene.n
     value = n * a.steps / r["seconds"]
     stage_ms = r["stage_ms"]
@@ -503,7 +507,8 @@ def main():
             "particles": n,
             "buckets": scene.num_buckets,
             "occupied_cells": r["c_occ"],
-            "form": "plain FP32, fused forces+integrate, HIP graph replay" if fused else "plain FP32, unfused",
+            "form": ("mixed precision density (FP16 cull, exact FP32 survivors), " if a.mixed else "plain FP32, ")
+                    + ("fused forces+integrate, HIP graph replay" if fused else "unfused"),
             "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
             "step_algorithmic_GBps": step_gbs / 1e9,
             "step_hbm_frac": step_gbs / HBM_PEAK,
@@ -531,12 +536,17 @@ def main():
     }
     if not a.no_8m and a.config != 2:
         k8 = max(20, a.steps // 4)
-        for key, s8 in (("also_8M", scenes.coupled_config_scene(2)), ("also_8M_fluid_only", scenes.config_scene(2))):
-            log(f"scene {s8.name}: n={s8.n:,}")
-            r8 = measure(s8, k8, max(5, a.warmup // 2), fused)
+        s8c = scenes.coupled_config_scene(2)
+        for key, s8, prec in (("also_8M", s8c, abi.PRECISION_FP32),
+                              ("also_8M_fluid_only", scenes.config_scene(2), abi.PRECISION_FP32),
+                              ("mixed_8M", s8c, abi.PRECISION_MIXED)):
+            log(f"scene {s8.name}: n={s8.n:,}" + (" (mixed precision)" if prec == abi.PRECISION_MIXED else ""))
+            r8 = measure(s8, k8, max(5, a.warmup // 2), fused, prec)
             b8 = s8.info.get("bodies")
+            form = ("mixed precision (FP16 cell offsets, packed-FP16 conservative cull, exact FP32 survivors, "
+                    "compensated density; forces FP32)" if prec == abi.PRECISION_MIXED else "plain FP32")
             out["config"][key] = {
-                "workload": f"plain FP32 dam break, {s8.n:,} particles, "
+                "workload": f"{form} dam break, {s8.n:,} particles, "
                             + (f"coupled two-way with {b8.b} rigid boxes ({b8.m:,} boundary spheres)"
                                if b8 is not None else "fluid only"),
                 "particles": s8.n, "steps": k8, "value": s8.n * k8 / r8["seconds"],
@@ -545,6 +555,8 @@ def main():
                 "stage_ms": {k: round(v, 5) for k, v in r8["stage_ms"].items()},
                 "finite": r8["finite"], "drift": r8["drift"],
             }
+        mx = out["config"]["mixed_8M"]
+        mx["ratio_vs_plain_8M"] = mx["value"] / out["config"]["also_8M"]["value"]
     if not a.no_33m and not a.no_8m and a.config == 1:
         out["config"]["also_33M"] = line_33m(a, fused)
     if not a.no_batch:

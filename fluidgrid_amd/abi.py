@@ -126,6 +126,8 @@ KEY_REF_XOR = 0
 KEY_LINEAR = 1
 MATH_FAST = 0
 MATH_REFERENCE = 1
+PRECISION_FP32 = 0
+PRECISION_MIXED = 1
 
 # fg_status codes (include/fluidgrid_amd.h)
 OK = 0

@@ -4,6 +4,7 @@
 // All kernels are streaming: one lane per particle, 256-lane workgroups
 // (four wave64s), 16-byte loads and stores for the float4 arrays.
 #include "fg_internal.hpp"
+#include "fg_sph_mixed.hpp"
 #include "fg_quat.hpp"
 
 namespace fg {
@@ -111,6 +112,63 @@ __global__ void __launch_bounds__(256) k_cell_offsets(uint32_t* __restrict__ sta
     if (src >= n) return;
     spos[i] = pos[src];
     if (WITH_VEL) svel[i] = vel[src];
+}
+
+// k_cell_offsets<true> that also writes the FP16 pair records of the mixed
+// precision form (fg_sph_mixed.hpp): record i holds the offsets of sorted
+// particles i and i+1.  Particle i+1's offsets come from the next lane; the
+// last lane of a wavefront gathers that particle itself.
+__global__ void __launch_bounds__(256) k_cell_offsets_half(uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                           const uint32_t* __restrict__ key,
+                                                           const uint32_t* __restrict__ id,
+                                                           const float4* __restrict__ pos, float4* __restrict__ spos,
+                                                           const float4* __restrict__ vel, float4* __restrict__ svel,
+                                                           uint32_t n, uint32_t nb, uint4* __restrict__ hrec,
+                                                           fg_common_fluid_params cp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const HalfGeom g = half_geom(cp);
+    float3 o = make_float3(0.0f, 0.0f, 0.0f);
+    if (valid) {
+        const uint32_t h = key[i];
+        if (i == 0) {
+            if (h < nb) start[h] = 0;
+        } else {
+            const uint32_t prev = key[i - 1];
+            if (h != prev) {
+                if (h < nb) start[h] = i;
+                if (prev < nb) end[prev] = i;
+            }
+        }
+        if (i == n - 1 && h < nb) end[h] = n;
+        const uint32_t src = id[i];
+        if (src < n) {
+            const float4 p = pos[src];
+            spos[i] = p;
+            svel[i] = vel[src];
+            o = half_offsets(p, cp, g);
+        }
+    }
+    const uint32_t xy = pack_half2(o.x, o.y), zz = pack_half2(o.z, 0.0f);
+    uint32_t nxy = (uint32_t)__shfl_down((int)xy, 1), nzz = (uint32_t)__shfl_down((int)zz, 1);
+    if (__lane_id() == 63u) {
+        float3 q = make_float3(0.0f, 0.0f, 0.0f);
+        if (i + 1 < n) {
+            const uint32_t src = id[i + 1];
+            if (src < n) q = half_offsets(pos[src], cp, g);
+        }
+        nxy = pack_half2(q.x, q.y);
+        nzz = pack_half2(q.z, 0.0f);
+    } else if (i + 1 >= n) {
+        nxy = nzz = 0u;
+    }
+    if (valid)
+        hrec[i] = make_uint4(__builtin_amdgcn_perm(nxy, xy, 0x05040100u), __builtin_amdgcn_perm(nxy, xy, 0x07060302u),
+                             __builtin_amdgcn_perm(nzz, zz, 0x05040100u), 0u);
+    if (i == 0) {
+        hrec[n] = make_uint4(0u, 0u, 0u, 0u);
+        hrec[n + 1] = make_uint4(0u, 0u, 0u, 0u);
+    }
 }
 
 // cellStart[k] = kEmpty for every key k of a sorted key array (the cells
@@ -236,19 +294,31 @@ int fg_clear_occupied_cells(unsigned int* gridStart, const unsigned int* sortedK
 int fg_cell_offsets_pos_vel(unsigned int* gridPosStart, unsigned int* gridPosEnd, unsigned int* hashTable,
                             unsigned int* particleID, fg_float4* particlePositions, fg_float4* orderedParticlePositions,
                             fg_float4* particleVel, fg_float4* particleOrderedVel, unsigned int numParticles,
-                            unsigned int numBuckets, int fillStart) {
+                            unsigned int numBuckets, int fillStart, unsigned long long* neighborBuffer) {
     if (!gridPosStart || (numParticles && (!gridPosEnd || !hashTable || !particleID || !particlePositions ||
                                            !orderedParticlePositions || !particleVel || !particleOrderedVel))) {
         set_error(FG_ERR_INVALID_ARGUMENT, "fg_cell_offsets_pos_vel: null pointer");
         return FG_ERR_INVALID_ARGUMENT;
     }
     const State& st = state();
+    if (neighborBuffer && !st.have_common) {
+        set_error(FG_ERR_NOT_INITIALIZED, "fg_cell_offsets_pos_vel: FP16 offsets need gInitializeCommonFluidParams");
+        return FG_ERR_NOT_INITIALIZED;
+    }
     if (fillStart) launch_fill_u32(gridPosStart, kEmpty, numBuckets, st.stream);
-    if (numParticles)
+    if (numParticles && neighborBuffer) {
+        uint4* hrec = reinterpret_cast<uint4*>(reinterpret_cast<char*>(neighborBuffer) + nbr_half_offset(numParticles));
+        hipLaunchKernelGGL(k_cell_offsets_half, dim3(grid_for(numParticles, 256)), dim3(256), 0, st.stream,
+                           gridPosStart, gridPosEnd, hashTable, particleID, FG_F4(particlePositions),
+                           FG_F4(orderedParticlePositions), FG_F4(particleVel), FG_F4(particleOrderedVel),
+                           numParticles, numBuckets, hrec, st.common);
+        note_half_records(neighborBuffer, numParticles);
+    } else if (numParticles) {
         hipLaunchKernelGGL(k_cell_offsets<true>, dim3(grid_for(numParticles, 256)), dim3(256), 0, st.stream,
                            gridPosStart, gridPosEnd, hashTable, particleID, FG_F4(particlePositions),
                            FG_F4(orderedParticlePositions), FG_F4(particleVel), FG_F4(particleOrderedVel),
                            numParticles, numBuckets);
+    }
     return check_launch("fg_cell_offsets_pos_vel") ? FG_OK : fg_peek_error();
 }
 

@@ -38,9 +38,17 @@ struct State {
     // LDS-tiled single-fluid passes (density lists + list-walking forces);
     // they take precedence over the two schedules above when switched on.
     bool tiled = false;
+    // Arithmetic of the fused density pass (fg_set_precision): FP32, or FP16
+    // culling with exact FP32 survivors; the latter takes precedence over the
+    // schedules above for density.
+    int precision = FG_PRECISION_FP32;
 };
 
 State& state();
+
+// Records that a cell pass wrote the FP16 pair records of n particles into a
+// neighbour buffer (the mixed density pass requires it).
+void note_half_records(const void* buf, uint32_t n);
 
 // Thread-local error slot.
 void set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));

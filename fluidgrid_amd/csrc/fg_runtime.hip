@@ -141,6 +141,17 @@ int fg_set_tiled(int enable) {
     return FG_OK;
 }
 
+int fg_set_precision(int mode) {
+    if (mode != FG_PRECISION_FP32 && mode != FG_PRECISION_MIXED) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_set_precision: unknown mode %d", mode);
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    state().precision = mode;
+    return FG_OK;
+}
+
+int fg_get_precision(void) { return state().precision; }
+
 int fg_get_tiled(int* enable) {
     if (enable) *enable = state().tiled ? 1 : 0;
     return FG_OK;

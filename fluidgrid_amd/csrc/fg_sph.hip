@@ -18,6 +18,7 @@
 #include "fg_internal.hpp"
 #include "fg_sph_fast.hpp"
 #include "fg_sph_tile.hpp"
+#include "fg_sph_mixed.hpp"
 
 #include <mutex>
 #include <unordered_map>
@@ -243,6 +244,19 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8)))
                                    &fr);
 }
 
+// Mixed-precision density (fg_sph_mixed.hpp): the FP32 pass's masks and
+// records, FP16-culled candidates, compensated sums.
+template <int BS>
+__global__ void __launch_bounds__(BS)
+    k_density_mixed(const float4* __restrict__ sp, const float4* __restrict__ sv, float2* __restrict__ rho_p,
+                    const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, uint32_t n, float k,
+                    float rho0, float m, fg_common_fluid_params cp, fg_kernels_cache kc,
+                    unsigned long long* __restrict__ masks) {
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (i >= n) return;
+    density_particle_mixed(i, sp, sv, rho_p, start, end, n, k, rho0, m, cp, kc, masks);
+}
+
 // LDS-tiled passes (fg_sph_tile.hpp).  Tile b of the grid is the sorted
 // particles [b * kTileLanes, (b + 1) * kTileLanes); blocks are remapped so
 // each XCD walks a contiguous range of tiles.
@@ -374,6 +388,21 @@ static bool nbr_format_tiled(const void* masks) {
     auto it = nbr_tiled().find(masks);
     return it != nbr_tiled().end() && it->second;
 }
+// Neighbour buffers whose FP16 pair records a cell pass has written, with
+// the particle count it wrote them for (the records' place depends on it).
+static std::unordered_map<const void*, uint32_t>& half_written() {
+    static std::unordered_map<const void*, uint32_t> m;
+    return m;
+}
+void note_half_records(const void* buf, uint32_t n) {
+    std::lock_guard<std::mutex> lock(g_nbr_mu);
+    half_written()[buf] = n;
+}
+static bool half_records_for(const void* buf, uint32_t n) {
+    std::lock_guard<std::mutex> lock(g_nbr_mu);
+    auto it = half_written().find(buf);
+    return it != half_written().end() && it->second == n;
+}
 static bool have_params(const char* who) {
     if (!state().have_common || !state().have_kernels) {
         set_error(FG_ERR_NOT_INITIALIZED, "%s: call gInitializeCommonFluidParams and gInitializeKernelParams first",
@@ -430,8 +459,17 @@ int launch_density(float4* sp, float2* rho_p, uint32_t* start, uint32_t* end, ui
     if (!stencil_covers_h("density pass")) return FG_ERR_UNSUPPORTED;
     const State& st = state();
     const dim3 grid(grid_for(n, kSphThreads)), block(kSphThreads);
-    if (masks) note_nbr_format(masks, use_fast() && st.tiled);
-    if (use_fast() && masks && st.tiled)
+    const bool mixed = st.precision == FG_PRECISION_MIXED && use_fast() && masks;
+    if (mixed && !half_records_for(masks, n)) {
+        set_error(FG_ERR_NOT_INITIALIZED, "density pass (mixed precision): no cell pass has written FP16 offsets "
+                  "for %u particles into this neighbour buffer (fg_cell_offsets_pos_vel with neighborBuffer)", n);
+        return FG_ERR_NOT_INITIALIZED;
+    }
+    if (masks) note_nbr_format(masks, use_fast() && st.tiled && !mixed);
+    if (mixed)
+        FG_SPH_LAUNCH_BS(k_density_mixed<256>, 256, n, sp, sv, rho_p, start, end, n, k, rho0, m, st.common,
+                         st.kernels, masks);
+    else if (use_fast() && masks && st.tiled)
         hipLaunchKernelGGL(k_density_tile, dim3(grid_for(n, kTileLanes)), dim3(kTileLanes), 0, st.stream, sp, sv,
                            rho_p, start, end, n, k, rho0, m, st.common, st.kernels, masks);
     else if (use_fast() && masks && st.flat_density)

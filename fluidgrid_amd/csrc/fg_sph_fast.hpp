@@ -298,6 +298,7 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
         uint32_t rb, re;
         if (row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) {
             const uint32_t len = re - rb, n4 = len & ~3u;
+            const bool encoded = len <= kRowMaskBits;
             const float4* pp = sp + rb;
             uint32_t kk = 0;
             if (n4) {
@@ -325,8 +326,10 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
                 const uint32_t nib = density4(p, t0, t1, t2, far, h2u, hhu, acc);
                 if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
             }
+            if (!encoded) msk = 0ull;
         }
-        // Rows longer than 64 particles are never read through their mask.
+        // Rows longer than 64 particles are never read through their mask
+        // (it is written as 0).
         if (MASKS) masks[(size_t)row * n + i] = msk;
     }
     store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * acc, k, rho0, rho_p, rec, p, v);

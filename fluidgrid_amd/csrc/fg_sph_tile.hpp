@@ -59,9 +59,15 @@ constexpr uint32_t kTileDescWords = 64;
 __host__ __device__ __forceinline__ size_t nbr_desc_offset(uint32_t n) {
     return nbr_align256(nbr_list_offset(n) + 2 * (size_t)kListMax * n);
 }
-__host__ __device__ __forceinline__ size_t nbr_buffer_bytes(uint32_t n) {
+// After the descriptors, the FP16 cell-offset pair records of the mixed
+// precision form (fg_sph_mixed.hpp): 16 bytes per particle and two padding
+// records.
+__host__ __device__ __forceinline__ size_t nbr_half_offset(uint32_t n) {
     const size_t tiles = ((size_t)n + kTileLanes - 1) / kTileLanes;
-    return nbr_desc_offset(n) + tiles * kTileDescWords * sizeof(uint32_t);
+    return nbr_align256(nbr_desc_offset(n) + tiles * kTileDescWords * sizeof(uint32_t));
+}
+__host__ __device__ __forceinline__ size_t nbr_buffer_bytes(uint32_t n) {
+    return nbr_half_offset(n) + 16 * ((size_t)n + 2);
 }
 __device__ __forceinline__ uint16_t* nbr_counts(unsigned long long* masks, uint32_t n) {
     return reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(masks) + nbr_count_offset(n));

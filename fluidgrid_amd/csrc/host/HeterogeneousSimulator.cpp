@@ -39,6 +39,13 @@ struct StreamScope {
     ~StreamScope() { fg_set_stream(prev); }
 };
 
+// Makes `mode` the density arithmetic for the duration of a scope.
+struct PrecisionScope {
+    int prev;
+    explicit PrecisionScope(int mode) : prev(fg_get_precision()) { fg_set_precision(mode); }
+    ~PrecisionScope() { fg_set_precision(prev); }
+};
+
 fg_float3 abi3(float3 v) { return fg_float3{v.x, v.y, v.z}; }
 }  // namespace
 
@@ -138,7 +145,8 @@ void HeterogeneousSimulator::findFluidNeighbourhood(Fluid::GPUSPHFluid* fluid) {
                                 fluid->getParticleID(), reinterpret_cast<fg_float4*>(fluid->getPosition()),
                                 reinterpret_cast<fg_float4*>(fluid->getOrderedPosition()),
                                 reinterpret_cast<fg_float4*>(fluid->getVelocity()),
-                                reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()), n, nb, 0);
+                                reinterpret_cast<fg_float4*>(fluid->getOrderedVelocities()), n, nb, 0,
+                                fg_get_precision() == FG_PRECISION_MIXED ? fluid->getNeighborMasks() : nullptr);
         return;
     }
     gFindCellOffsetOrderingPosVel(fluid->getHashStart(), fluid->getHashEnd(), fluid->getHashKey(),
@@ -321,6 +329,7 @@ void HeterogeneousSimulator::enqueueStep() {
 // HeterogeneousSimulator.cpp:54-127
 void HeterogeneousSimulator::doTimeStep() {
     StreamScope scope(mStream);
+    PrecisionScope precision(mOptions.fused ? mOptions.precision : FG_PRECISION_FP32);
     RigidBody::RigidBodySimulator::instance()->setStream(mStream);
     enqueueStep();
     fg_check("doTimeStep");
@@ -344,6 +353,7 @@ void HeterogeneousSimulator::doTimeSteps(int steps) {
         return;
     }
     StreamScope scope(mStream);
+    PrecisionScope precision(mOptions.fused ? mOptions.precision : FG_PRECISION_FP32);
     RigidBody::RigidBodySimulator::instance()->setStream(mStream);
     if (!mGraphExec) {
         hip_check(hipStreamBeginCapture(mStream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");

@@ -107,6 +107,18 @@ int fg_sim_set_overlap(int overlap_bodies) {
     return FG_OK;
 }
 
+int fg_sim_set_precision(int mode) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    if (mode != FG_PRECISION_FP32 && mode != FG_PRECISION_MIXED) {
+        fg_record_error(FG_ERR_INVALID_ARGUMENT, "fg_sim_set_precision: unknown mode");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    Simulators::StepOptions o = Simulators::HeterogeneousSimulator::instance()->options();
+    o.precision = mode;
+    Simulators::HeterogeneousSimulator::instance()->setOptions(o);
+    return FG_OK;
+}
+
 int fg_sim_step(int steps) {
     if (!sim()) return FG_ERR_NOT_INITIALIZED;
     try {

@@ -56,7 +56,7 @@ SIGNATURES = {
     "fg_calc_hash_key_only": (i32, [vp, vp, u32, u32, abi.Float3]),
     "fg_clear_occupied_cells": (i32, [vp, vp, u32, u32]),
     "fg_calc_hash_key_clear": (i32, [vp, vp, u32, u32, abi.Float3, vp]),
-    "fg_cell_offsets_pos_vel": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, u32, u32, i32]),
+    "fg_cell_offsets_pos_vel": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, u32, u32, i32, vp]),
     "fg_sph_forces_integrate": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, u32, f32, f32]),
     "fg_neighbor_mask_bytes": (sz, [u32]),
     "fg_sph_density_nbr": (i32, [vp, vp, vp, vp, vp, vp, u32, f32, f32, f32]),
@@ -81,6 +81,7 @@ SIGNATURES = {
     "fg_sim_add_fluid": (i32, [C.POINTER(abi.FluidParams), vp, vp]),
     "fg_sim_set_options": (i32, [i32, i32, i32]),
     "fg_sim_set_overlap": (i32, [i32]),
+    "fg_sim_set_precision": (i32, [i32]),
     "fg_sim_step": (i32, [i32]),
     "fg_sim_synchronize": (i32, []),
     "fg_sim_stream": (vp, []),
@@ -139,6 +140,8 @@ SIGNATURES = {
     "fg_set_schedule": (i32, [i32, i32]),
     "fg_get_schedule": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fg_set_tiled": (i32, [i32]),
+    "fg_set_precision": (i32, [i32]),
+    "fg_get_precision": (i32, []),
     "fg_get_tiled": (i32, [C.POINTER(C.c_int)]),
     "fg_batch_hash": (i32, [vp, u32, vp, vp, vp, u32]),
     "fg_batch_density_nbr": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, vp, u32]),
@@ -299,7 +302,8 @@ class Simulator:
     """The C++ HeterogeneousSimulator (one process-wide instance) driven
     through include/fluidgrid_amd_sim.h."""
 
-    def __init__(self, scene, math_mode=abi.MATH_FAST, fused=True, use_graph=True, stage_timing=False):
+    def __init__(self, scene, math_mode=abi.MATH_FAST, fused=True, use_graph=True, stage_timing=False,
+                 precision=abi.PRECISION_FP32):
         L = load()
         self.scene = scene
         self.common = scene.common
@@ -312,11 +316,16 @@ class Simulator:
         self.idx = idx
         self.n = scene.n
         check(L.fg_sim_set_options(int(fused), int(use_graph), int(stage_timing)), "options")
+        check(L.fg_sim_set_precision(int(precision)), "precision")
         self.buffers = FluidBuffers()
         check(L.fg_sim_fluid_buffers(idx, C.byref(self.buffers)), "buffers")
 
     def set_options(self, fused=True, use_graph=True, stage_timing=False):
         check(load().fg_sim_set_options(int(fused), int(use_graph), int(stage_timing)), "options")
+
+    def set_precision(self, mode):
+        """Density arithmetic of the fused step: abi.PRECISION_FP32 or abi.PRECISION_MIXED."""
+        check(load().fg_sim_set_precision(int(mode)), "fg_sim_set_precision")
 
     def set_inflow(self, desc):
         """Attach an abi.InflowDesc: reserve slots are emitted from the next step."""

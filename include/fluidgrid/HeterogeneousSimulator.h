@@ -29,6 +29,9 @@ struct StepOptions {
     bool use_graph = true;   // doTimeSteps replays a captured step
     bool stage_timing = false;
     bool overlap_bodies = true;  // body work on a second stream (results unchanged)
+    // Density arithmetic of the fused step (FG_PRECISION_FP32 or
+    // FG_PRECISION_MIXED); the unfused step is always FP32.
+    int precision = FG_PRECISION_FP32;
 };
 
 struct StageTimes {

@@ -261,6 +261,19 @@ enum { FG_KEY_REF_XOR = 0, FG_KEY_LINEAR = 1 };
  *    expressions, no contraction. */
 enum { FG_MATH_FAST = 0, FG_MATH_REFERENCE = 1 };
 
+/* Arithmetic of the fused (masked, FG_MATH_FAST, FG_KEY_LINEAR) density pass.
+ *  FG_PRECISION_FP32: every candidate tested and summed in FP32 (default).
+ *  FG_PRECISION_MIXED: the cell pass also writes each sorted particle's
+ *    offset inside its cell as FP16 (fg_cell_offsets_pos_vel with a
+ *    neighbour buffer); density culls candidates two at a time in packed
+ *    FP16 with a margin that never drops a pair the FP32 test accepts,
+ *    evaluates the survivors exactly as FP32 does, and sums them with
+ *    Neumaier compensation.  The row masks and neighbour records are those
+ *    of FP32, so the force pass is unchanged.  Mode of this process; set it
+ *    between steps (a density pass in the mixed mode needs the FP16 offsets
+ *    of the same sorted order, and refuses a buffer without them). */
+enum { FG_PRECISION_FP32 = 0, FG_PRECISION_MIXED = 1 };
+
 int         fg_last_error(void);          /* last error code of this thread, then FG_OK */
 const char* fg_last_error_string(void);   /* message of the last error of this thread */
 int         fg_peek_error(void);          /* last error code without clearing */
@@ -293,6 +306,8 @@ int   fg_get_schedule(int* flatDensity, int* flatForces);
  * runs. */
 int   fg_set_tiled(int enable);
 int   fg_get_tiled(int* enable);
+int   fg_set_precision(int mode);
+int   fg_get_precision(void);
 int   fg_get_common_fluid_params(fg_common_fluid_params* out);
 int   fg_get_kernels_cache(fg_kernels_cache* out);
 
@@ -336,11 +351,14 @@ int fg_calc_hash_key_clear(unsigned int* hashKey, fg_float4* positions, unsigned
                            unsigned int numBuckets, fg_float3 cell_size, unsigned int* gridStart);
 /* gFindCellOffsetOrderingPosVel with the cellStart fill optional
  * (fillStart = 0: the caller guarantees cellStart is 0xFFFFFFFF outside the
- * cells of these keys, e.g. after fg_clear_occupied_cells). */
+ * cells of these keys, e.g. after fg_clear_occupied_cells).  With a
+ * neighbour buffer (fg_neighbor_mask_bytes, else NULL) it also writes the
+ * sorted particles' FP16 cell offsets that the mixed-precision density pass
+ * reads (FG_PRECISION_MIXED). */
 int fg_cell_offsets_pos_vel(unsigned int* gridPosStart, unsigned int* gridPosEnd, unsigned int* hashTable,
                             unsigned int* particleID, fg_float4* particlePositions, fg_float4* orderedParticlePositions,
                             fg_float4* particleVel, fg_float4* particleOrderedVel, unsigned int numParticles,
-                            unsigned int numBuckets, int fillStart);
+                            unsigned int numBuckets, int fillStart, unsigned long long* neighborBuffer);
 
 /* gCalcHashKey without the id output (the sort that follows generates the
  * identity ids itself, see fg_sort_pairs_iota). */

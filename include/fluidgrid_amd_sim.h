@@ -40,6 +40,9 @@ int fg_sim_set_options(int fused, int use_graph, int stage_timing);
 /* With rigid bodies: run the body work on a second stream beside the fluid
  * stages (default on; results are the same either way). */
 int fg_sim_set_overlap(int overlap_bodies);
+/* Density arithmetic of the fused step: FG_PRECISION_FP32 (default) or
+ * FG_PRECISION_MIXED (fluidgrid_amd.h). */
+int fg_sim_set_precision(int mode);
 int fg_sim_step(int steps);                       /* asynchronous */
 int fg_sim_synchronize(void);
 void* fg_sim_stream(void);                        /* hipStream_t of the simulator */

@@ -216,7 +216,7 @@ def test_clear_occupied_then_no_fill_offsets(fglib, orc, which, fused):
     assert np.all(dstart.numpy() == EMPTY)
     L.gSort(dkey.ptr, did.ptr, s.n)
     assert L.fg_cell_offsets_pos_vel(dstart.ptr, dend.ptr, dkey.ptr, did.ptr, dpos.ptr, dspos.ptr, dvel.ptr,
-                                     dsvel.ptr, s.n, nb, 0) == 0
+                                     dsvel.ptr, s.n, nb, 0, None) == 0
     fglib.check(None, "grid2")
     key, pid = orc.hash_keys(pos2.astype(np.float32), nb, tuple(s.common.cell_size), s.common, s.key_mode)
     sk, sp = orc.sort_pairs(key, pid)
