@@ -26,10 +26,33 @@ int batch_body_forces(const float4* world, const float4* vel, const float4* cm, 
 int batch_integrate_bodies(float4* cm, const float* mass, float4* vel, const float4* force, uint32_t nbodies,
                            const uint32_t* body_scene, const BatchBodyScene* scenes, float4* L, const float4* torque,
                            const float4* inv_inertia, float4* orient);
+// Persistent form of the density and force launches: a grid of as many
+// workgroups as the GPU holds at once takes work items (per_item blocks'
+// worth of consecutive sorted particles) from counters instead of one
+// workgroup per block.  The items are split into kQueueSlots slices, one per
+// XCD (workgroups are dispatched to the XCDs round-robin), each with its own
+// counter; a workgroup whose slice is exhausted takes from the next slices.
+// counters: kQueueSlots * kQueueStride words owned by the caller (cleared by
+// each launch); null selects the plain launch.  Results are the same bits.
+constexpr uint32_t kQueueSlots = 8;
+constexpr uint32_t kQueueStride = 16;  // one 64-byte line per counter
+struct WorkQueue {
+    uint32_t* counters;
+    uint32_t per_item;
+};
+
+// fg_batch_density_nbr / fg_batch_forces_integrate_nbr, optionally persistent.
+int batch_density(const fg_scene_desc* scenes, uint32_t nscenes, const uint32_t* scene_of, float4* sp,
+                  const float4* sv, float2* rho_p, uint32_t* start, uint32_t* end, unsigned long long* masks,
+                  uint32_t n, WorkQueue q);
+int batch_forces(const fg_scene_desc* scenes, uint32_t nscenes, const uint32_t* scene_of, float4* sp, float4* sv,
+                 float2* rho_p, uint32_t* start, uint32_t* end, const unsigned long long* masks, float4* vel,
+                 float4* pos, uint32_t* id, uint32_t n, WorkQueue q);
 // fg_batch_forces_integrate_nbr with each scene's bodies (fg_sph_forces_coupled per scene; scenes
 // without bodies take the uncoupled expression).
 int batch_forces_coupled(const fg_scene_desc* scenes, uint32_t nscenes, const uint32_t* scene_of, float4* sp,
                          float4* sv, float2* rho_p, uint32_t* start, uint32_t* end, const unsigned long long* masks,
-                         float4* vel, float4* pos, uint32_t* id, uint32_t n, const BatchBodyScene* bodies);
+                         float4* vel, float4* pos, uint32_t* id, uint32_t n, const BatchBodyScene* bodies,
+                         WorkQueue q = WorkQueue{nullptr, 1});
 
 }  // namespace fg

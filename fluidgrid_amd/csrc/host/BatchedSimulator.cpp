@@ -45,6 +45,11 @@ int key_bits_for(uint64_t nb) {
     return b;
 }
 
+// The persistent launches' work queue of a tuning (counters null: plain launches).
+fg::WorkQueue work_queue(unsigned* counters, const BatchTuning& t) {
+    return fg::WorkQueue{t.queueItems > 0 ? counters : nullptr, (uint32_t)(t.queueItems > 0 ? t.queueItems : 1)};
+}
+
 struct StreamScope {
     void* prev;
     explicit StreamScope(hipStream_t s) : prev(fg_get_stream()) { fg_set_stream(s); }
@@ -130,8 +135,11 @@ BatchedSimulator::BatchedSimulator(const std::vector<BatchScene>& scenes) {
     hip_check(hipMemcpy(d_vel, vel.data(), sizeof(float4) * mN, hipMemcpyHostToDevice), "upload");
     hip_check(hipMemset(d_rho, 0, sizeof(float2) * mN), "memset");
     hip_check(hipMemset(d_end, 0, sizeof(unsigned) * mBuckets), "memset");
+    d_queue = dalloc<unsigned>(fg::kQueueSlots * fg::kQueueStride);
     hip_check(hipStreamCreateWithFlags(&mStream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&mBodyStream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : mEv) hip_check(hipEventCreate(&e), "hipEventCreate");
+    for (auto& e : mSync) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     int blk = 256, bits = 8, fd = 0, ff = 0;
     fg_get_tuning(&blk, &bits);
     fg_get_schedule(&fd, &ff);
@@ -145,11 +153,15 @@ BatchedSimulator::BatchedSimulator(const std::vector<BatchScene>& scenes) {
 BatchedSimulator::~BatchedSimulator() {
     invalidateGraph();
     if (mStream) (void)hipStreamSynchronize(mStream);
+    if (mBodyStream) (void)hipStreamSynchronize(mBodyStream);
     for (auto& e : mEv)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : mSync)
+        if (e) (void)hipEventDestroy(e);
+    if (mBodyStream) (void)hipStreamDestroy(mBodyStream);
     if (mStream) (void)hipStreamDestroy(mStream);
     dfree(d_descs); dfree(d_sceneOf); dfree(d_pos); dfree(d_vel); dfree(d_spos); dfree(d_svel); dfree(d_rho);
-    dfree(d_key); dfree(d_id); dfree(d_start); dfree(d_end); dfree(d_masks);
+    dfree(d_key); dfree(d_id); dfree(d_start); dfree(d_end); dfree(d_masks); dfree(d_queue);
     if (d_sortWs) (void)hipFree(d_sortWs);
     releaseBodies();
 }
@@ -354,17 +366,72 @@ void BatchedSimulator::invalidateGraph() {
 }
 
 void BatchedSimulator::setUseGraph(bool on) { mUseGraph = on; }
+void BatchedSimulator::setOverlap(bool on) {
+    if (on != mOverlap) invalidateGraph();
+    mOverlap = on;
+}
 void BatchedSimulator::setStageTiming(bool on) { mTiming = on; }
 
 void BatchedSimulator::setTuning(const BatchTuning& t) {
+    if (t.queueItems < 0 || t.queueItems > 64)
+        throw std::invalid_argument("BatchedSimulator::setTuning: queueItems outside 0..64");
     if (fg_set_tuning(t.sphBlock, t.sortDigitBits) != FG_OK) fg_check("setTuning");
     fg_set_schedule(t.flatDensity ? 1 : 0, t.flatForces ? 1 : 0);
     mTuning = t;
     invalidateGraph();  // the launch shapes are recorded in the graph
 }
 
+// Coupled step with the body stages on the body stream, in the dependency
+// order of HeterogeneousSimulator::enqueueCoupledOverlapped:
+//   bodies:  world transform, body grid, bounds   (last step's body state)
+//   fluid:   hash, sort, cells -> [cells]; density
+//   bodies:  forces from the sorted fluid         (after [cells])
+//   fluid:   coupled forces + integration         (after the body grid)
+//   bodies:  FP64 sums, integration, clears       (after the fluid forces,
+//            which read the body velocities)
+// and the fluid stream waits for the body stream at the end, so a step (and
+// a graph captured from it) is complete when the fluid stream is.  Every
+// kernel reads what it reads in the serial order.
+void BatchedSimulator::enqueueStepOverlapped() {
+    enum { kFork, kBodyGrid, kCells, kForces, kJoin };
+    const unsigned S = (unsigned)mDescs.size();
+    hip_check(hipEventRecord(mSync[kFork], mStream), "event");
+    hip_check(hipStreamWaitEvent(mBodyStream, mSync[kFork], 0), "wait");
+    fg_set_stream(mBodyStream);
+    enqueueBodyStages(true);
+    hip_check(hipEventRecord(mSync[kBodyGrid], mBodyStream), "event");
+    fg_set_stream(mStream);
+    fg_batch_hash(d_descs, S, d_sceneOf, reinterpret_cast<fg_float4*>(d_pos), d_key, mN);
+    fg_sort_pairs_ws(d_key, d_id, mN, mKeyBits, d_sortWs, mSortWsBytes, 1);
+    gFindCellOffsetOrderingPosVel(d_start, d_end, d_key, d_id, reinterpret_cast<fg_float4*>(d_pos),
+                                  reinterpret_cast<fg_float4*>(d_spos), reinterpret_cast<fg_float4*>(d_vel),
+                                  reinterpret_cast<fg_float4*>(d_svel), mN, mBuckets);
+    hip_check(hipEventRecord(mSync[kCells], mStream), "event");
+    fg::batch_density(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, mN, work_queue(d_queue, mTuning));
+    hip_check(hipStreamWaitEvent(mBodyStream, mSync[kCells], 0), "wait");
+    fg_set_stream(mBodyStream);
+    fg::batch_body_forces(d_bworld, d_bvel, d_bcm, d_bpf, d_bpt, mSphereCount, mBodyCount, d_bodyScene,
+                          static_cast<const fg::BatchBodyScene*>(d_bscenes), d_spos, d_svel, d_start, d_end, mN,
+                          d_bodyBounds);
+    fg_set_stream(mStream);
+    hip_check(hipStreamWaitEvent(mStream, mSync[kBodyGrid], 0), "wait");
+    fg::batch_forces_coupled(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, d_vel, d_pos,
+                             d_id, mN, static_cast<const fg::BatchBodyScene*>(d_bscenes), work_queue(d_queue, mTuning));
+    hip_check(hipEventRecord(mSync[kForces], mStream), "event");
+    hip_check(hipStreamWaitEvent(mBodyStream, mSync[kForces], 0), "wait");
+    fg_set_stream(mBodyStream);
+    enqueueBodyStages(false);
+    hip_check(hipEventRecord(mSync[kJoin], mBodyStream), "event");
+    fg_set_stream(mStream);
+    hip_check(hipStreamWaitEvent(mStream, mSync[kJoin], 0), "wait");
+}
+
 // Stage order of HeterogeneousSimulator::doTimeStep for the fused fast path.
 void BatchedSimulator::enqueueStep() {
+    if (mBodyCount && mOverlap && !mTiming) {
+        enqueueStepOverlapped();
+        return;
+    }
     const unsigned S = (unsigned)mDescs.size();
     if (mTiming) (void)hipEventRecord(mEv[0], mStream);
     fg_batch_hash(d_descs, S, d_sceneOf, reinterpret_cast<fg_float4*>(d_pos), d_key, mN);
@@ -376,22 +443,18 @@ void BatchedSimulator::enqueueStep() {
                                   reinterpret_cast<fg_float4*>(d_svel), mN, mBuckets);
     if (mBodyCount) enqueueBodyStages(true);
     if (mTiming) (void)hipEventRecord(mEv[3], mStream);
-    fg_batch_density_nbr(d_descs, S, d_sceneOf, reinterpret_cast<fg_float4*>(d_spos),
-                         reinterpret_cast<const fg_float4*>(d_svel), reinterpret_cast<fg_float2*>(d_rho), d_start,
-                         d_end, d_masks, mN);
+    fg::batch_density(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, mN, work_queue(d_queue, mTuning));
     if (mTiming) (void)hipEventRecord(mEv[4], mStream);
     if (mBodyCount) {
         fg::batch_body_forces(d_bworld, d_bvel, d_bcm, d_bpf, d_bpt, mSphereCount, mBodyCount, d_bodyScene,
                               static_cast<const fg::BatchBodyScene*>(d_bscenes), d_spos, d_svel, d_start, d_end, mN,
                               d_bodyBounds);
         fg::batch_forces_coupled(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, d_vel, d_pos,
-                                 d_id, mN, static_cast<const fg::BatchBodyScene*>(d_bscenes));
+                                 d_id, mN, static_cast<const fg::BatchBodyScene*>(d_bscenes), work_queue(d_queue, mTuning));
         enqueueBodyStages(false);
     } else {
-        fg_batch_forces_integrate_nbr(d_descs, S, d_sceneOf, reinterpret_cast<fg_float4*>(d_spos),
-                                      reinterpret_cast<fg_float4*>(d_svel), reinterpret_cast<fg_float2*>(d_rho),
-                                      d_start, d_end, d_masks, reinterpret_cast<fg_float4*>(d_vel),
-                                      reinterpret_cast<fg_float4*>(d_pos), d_id, mN);
+        fg::batch_forces(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, d_vel, d_pos, d_id,
+                         mN, work_queue(d_queue, mTuning));
     }
     if (mTiming) (void)hipEventRecord(mEv[5], mStream);
 }
@@ -496,7 +559,7 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
     };
     // one discarded trial first: the clocks of an idle GPU ramp up over it
     (void)time_shape(mTuning, false);
-    // Coordinate descent: workgroup size first, then the digit width.
+    // Coordinate descent: workgroup size, schedules, work queue, digit width.
     BatchTuning best = mTuning;
     float best_ms = 1e30f;
     for (int blk : {64, 128, 256, 512}) {
@@ -510,6 +573,13 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
         t.flatDensity = (sched & 1) != 0;
         t.flatForces = (sched & 2) != 0;
         if (t.flatDensity == best.flatDensity && t.flatForces == best.flatForces) continue;
+        const float ms = time_shape(t);
+        if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+    for (int items : {0, 1, 4}) {
+        BatchTuning t = best;
+        t.queueItems = items;
+        if (t.queueItems == best.queueItems) continue;
         const float ms = time_shape(t);
         if (ms < best_ms) { best_ms = ms; best = t; }
     }

@@ -104,6 +104,12 @@ int fg_bsim_set_options(int use_graph, int stage_timing) {
     return FG_OK;
 }
 
+int fg_bsim_set_overlap(int overlap_bodies) {
+    if (!batch()) return FG_ERR_NOT_INITIALIZED;
+    batch()->setOverlap(overlap_bodies != 0);
+    return FG_OK;
+}
+
 int fg_bsim_step(int steps) {
     if (!batch()) return FG_ERR_NOT_INITIALIZED;
     try {
@@ -170,14 +176,17 @@ int fg_bsim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, 
             tuning[1] = t.sortDigitBits;
             tuning[2] = t.flatDensity ? 1 : 0;
             tuning[3] = t.flatForces ? 1 : 0;
+            tuning[4] = t.queueItems;
         }
         int k = 0;
         for (; trials && k < maxTrials && k < (int)log.size(); k++) {
-            trials[5 * k] = (float)log[k].first.sphBlock;
-            trials[5 * k + 1] = (float)log[k].first.sortDigitBits;
-            trials[5 * k + 2] = log[k].first.flatDensity ? 1.0f : 0.0f;
-            trials[5 * k + 3] = log[k].first.flatForces ? 1.0f : 0.0f;
-            trials[5 * k + 4] = log[k].second;
+            float* r = trials + 6 * k;
+            r[0] = (float)log[k].first.sphBlock;
+            r[1] = (float)log[k].first.sortDigitBits;
+            r[2] = log[k].first.flatDensity ? 1.0f : 0.0f;
+            r[3] = log[k].first.flatForces ? 1.0f : 0.0f;
+            r[4] = (float)log[k].first.queueItems;
+            r[5] = log[k].second;
         }
         if (numTrials) *numTrials = k;
         return FG_OK;
@@ -186,7 +195,7 @@ int fg_bsim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, 
     }
 }
 
-int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces) {
+int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces, int queueItems) {
     if (!batch()) return FG_ERR_NOT_INITIALIZED;
     try {
         Simulators::BatchTuning t;
@@ -194,6 +203,7 @@ int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int fla
         t.sortDigitBits = sortDigitBits;
         t.flatDensity = flatDensity != 0;
         t.flatForces = flatForces != 0;
+        t.queueItems = queueItems;
         batch()->setTuning(t);
         return FG_OK;
     } catch (const std::exception& e) {

@@ -151,6 +151,7 @@ SIGNATURES = {
     "fg_bsim_add_bodies": (i32, [u32, vp, vp, u32, vp, f32, C.POINTER(abi.CommonRigidBodyParams), f32, f32]),
     "fg_bsim_body_state": (i32, [C.POINTER(C.c_uint), vp, vp, vp, vp]),
     "fg_bsim_set_options": (i32, [i32, i32]),
+    "fg_bsim_set_overlap": (i32, [i32]),
     "fg_bsim_step": (i32, [i32]),
     "fg_bsim_synchronize": (i32, []),
     "fg_bsim_download": (i32, [vp, vp]),
@@ -158,7 +159,7 @@ SIGNATURES = {
     "fg_bsim_stage_times": (i32, [vp]),
     "fg_bsim_info": (i32, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_int)]),
     "fg_bsim_autotune": (i32, [i32, vp, vp, i32, C.POINTER(C.c_int)]),
-    "fg_bsim_set_tuning": (i32, [i32, i32, i32, i32]),
+    "fg_bsim_set_tuning": (i32, [i32, i32, i32, i32, i32]),
     "fg_bsim_destroy": (i32, []),
 }
 
@@ -470,6 +471,10 @@ class BatchSimulator:
         self.counts = counts
         self.set_options(use_graph, stage_timing)
 
+    def set_overlap(self, on=True):
+        """Body stages on a second stream (default) or in line."""
+        check(load().fg_bsim_set_overlap(int(on)), "fg_bsim_set_overlap")
+
     def set_options(self, use_graph=True, stage_timing=False):
         check(load().fg_bsim_set_options(int(use_graph), int(stage_timing)), "fg_bsim_set_options")
 
@@ -524,17 +529,18 @@ class BatchSimulator:
 
     def autotune(self, trial_steps=10):
         nt = C.c_int()
-        tun = np.zeros(4, np.int32)
-        trials = np.zeros((32, 5), np.float32)
+        tun = np.zeros(5, np.int32)
+        trials = np.zeros((32, 6), np.float32)
         check(load().fg_bsim_autotune(int(trial_steps), tun.ctypes.data, trials.ctypes.data, 32, C.byref(nt)),
               "fg_bsim_autotune")
         return dict(sph_block=int(tun[0]), sort_digit_bits=int(tun[1]), flat_density=bool(tun[2]),
-                    flat_forces=bool(tun[3]),
-                    trials=[(int(a), int(b), int(c), int(d), float(e)) for a, b, c, d, e in trials[:nt.value]])
+                    flat_forces=bool(tun[3]), queue_items=int(tun[4]),
+                    trials=[(int(a), int(b), int(c), int(d), int(q), float(e)) for a, b, c, d, q, e in
+                            trials[:nt.value]])
 
-    def set_tuning(self, sph_block=256, sort_digit_bits=8, flat_density=False, flat_forces=False):
-        check(load().fg_bsim_set_tuning(int(sph_block), int(sort_digit_bits), int(flat_density), int(flat_forces)),
-              "fg_bsim_set_tuning")
+    def set_tuning(self, sph_block=256, sort_digit_bits=8, flat_density=False, flat_forces=False, queue_items=0):
+        check(load().fg_bsim_set_tuning(int(sph_block), int(sort_digit_bits), int(flat_density), int(flat_forces),
+                                        int(queue_items)), "fg_bsim_set_tuning")
 
     def close(self):
         load().fg_bsim_destroy()

@@ -47,6 +47,9 @@ struct BatchTuning {
     int sortDigitBits = 8;    // widest radix digit
     bool flatDensity = false; // density candidates as one stream per lane
     bool flatForces = false;  // force neighbours as one stream per lane
+    // Persistent density and force launches taking work items of this many
+    // workgroups' particles from a queue (0: one workgroup per block).
+    int queueItems = 0;
 };
 
 class BatchedSimulator {
@@ -60,10 +63,14 @@ public:
     void doTimeSteps(int steps);       // graph replay (eager when timing is on)
     void setUseGraph(bool on);
     void setStageTiming(bool on);
+    // Body stages on a second stream beside the fluid stages (default on;
+    // results unchanged; stage timing runs them in line).
+    void setOverlap(bool on);
     void synchronize();
 
     // Times launch shapes and schedules by coordinate descent (workgroup
-    // size, then the density and force schedules, then the digit width;
+    // size, then the density and force schedules, then the persistent
+    // work-queue launches (none, 1 or 4 workgroups per item), then the digit width;
     // trialSteps graph steps each after a warm-up, every trial from the
     // current particle state) and keeps the fastest; the state is restored
     // afterwards, so tuning does not advance the simulation.  No choice
@@ -95,6 +102,7 @@ public:
 
 private:
     void enqueueStep();
+    void enqueueStepOverlapped();
     void enqueueBodyStages(bool before_density);
     void invalidateGraph();
     void setupBodies();
@@ -104,8 +112,10 @@ private:
     unsigned mN = 0, mBuckets = 0;
     int mKeyBits = 1;
     BatchTuning mTuning;
-    bool mUseGraph = true, mTiming = false;
+    bool mUseGraph = true, mTiming = false, mOverlap = true;
     hipStream_t mStream = nullptr;
+    hipStream_t mBodyStream = nullptr;
+    hipEvent_t mSync[5] = {};
     hipGraph_t mGraph = nullptr;
     hipGraphExec_t mGraphExec = nullptr;
     hipEvent_t mEv[6] = {};
@@ -117,6 +127,7 @@ private:
     float2* d_rho = nullptr;
     unsigned *d_key = nullptr, *d_id = nullptr, *d_start = nullptr, *d_end = nullptr;
     unsigned long long* d_masks = nullptr;
+    unsigned* d_queue = nullptr;  // work-queue counters of the persistent launches
     void* d_sortWs = nullptr;
     size_t mSortWsBytes = 0;
 

@@ -32,6 +32,9 @@ int fg_bsim_add_bodies(unsigned int scene, const fg_float4* worldSpheres, const 
 int fg_bsim_body_state(unsigned int* numBodies, fg_float4* positions, fg_float4* velocities, fg_float4* orientations,
                        fg_float4* angularMomenta);
 int fg_bsim_set_options(int use_graph, int stage_timing);
+/* Body stages on a second stream beside the fluid stages (default 1; the
+ * results are the same bits either way; stage timing runs them in line). */
+int fg_bsim_set_overlap(int overlap_bodies);
 int fg_bsim_step(int steps);                      /* asynchronous */
 int fg_bsim_synchronize(void);
 void* fg_bsim_stream(void);                       /* hipStream_t of the batch (null without one) */
@@ -43,13 +46,18 @@ int fg_bsim_stage_times(float* out5);
 /* numParticles, numBuckets, keyBits of the batch. */
 int fg_bsim_info(unsigned int* numParticles, unsigned int* numBuckets, int* keyBits);
 /* Launch-shape tuning: times each candidate for trialSteps steps, keeps the
- * fastest and restores the particle state.  tuning (optional) receives
- * (sphBlock, sortDigitBits, flatDensity, flatForces) of the choice; trials
- * (optional, capacity maxTrials) receives one (sphBlock, sortDigitBits,
- * flatDensity, flatForces, ms per step) record per candidate timed;
- * *numTrials their count. */
+ * fastest and restores the particle state.  tuning (optional, 5 ints)
+ * receives (sphBlock, sortDigitBits, flatDensity, flatForces, queueItems) of
+ * the choice; trials (optional, capacity maxTrials records of 6 floats)
+ * receives one (sphBlock, sortDigitBits, flatDensity, flatForces,
+ * queueItems, ms per step) record per candidate timed; *numTrials their
+ * count. */
 int fg_bsim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, int* numTrials);
-int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces);
+/* queueItems: 0 launches one workgroup per block of particles; 1..64 runs
+ * the density and force passes as persistent launches (as many workgroups
+ * as the GPU holds at once) that take work items of that many blocks'
+ * particles from per-XCD counters.  No choice changes any result bit. */
+int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces, int queueItems);
 int fg_bsim_destroy(void);
 
 #ifdef __cplusplus

@@ -116,11 +116,13 @@ def test_tuning_changes_no_result_and_restores_state(fglib):
         assert t["sph_block"] in (64, 128, 256, 512) and 6 <= t["sort_digit_bits"] <= 8
         assert len(t["trials"]) >= 6
         results = []
-        for blk, bits, fd, ff in ((256, 8, 0, 0), (64, 6, 1, 1), (512, 7, 1, 0), (128, 8, 0, 1)):
+        assert t["queue_items"] in (0, 1, 4)
+        for blk, bits, fd, ff, q in ((256, 8, 0, 0, 0), (64, 6, 1, 1, 0), (512, 7, 1, 0, 0), (128, 8, 0, 1, 0),
+                                     (256, 8, 0, 0, 1), (64, 8, 1, 1, 4), (512, 8, 0, 0, 2)):
             # each shape starts from the initial state of the ensemble
             b.close()
             b = fglib.BatchSimulator(ens)
-            b.set_tuning(blk, bits, fd, ff)
+            b.set_tuning(blk, bits, fd, ff, q)
             b.step(7)
             results.append(b.state())
         for r in results[1:]:

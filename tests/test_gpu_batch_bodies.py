@@ -34,9 +34,12 @@ def _ensemble():
     return ens
 
 
-def _batch(fglib, ens, steps, graph=True):
+def _batch(fglib, ens, steps, graph=True, overlap=True, tuning=None):
     b = fglib.BatchSimulator(ens, use_graph=graph)
     try:
+        b.set_overlap(overlap)
+        if tuning is not None:
+            b.set_tuning(*tuning)
         for s, bs in enumerate(ens.bodies):
             if bs is not None:
                 b.add_bodies(s, bs)
@@ -85,6 +88,35 @@ def test_eager_and_graph_identical(fglib):
     ens = _ensemble()
     a = _batch(fglib, ens, 40, graph=True)
     b = _batch(fglib, ens, 40, graph=False)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for k in a[2]:
+        assert np.array_equal(a[2][k], b[2][k]), k
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_overlapped_and_inline_body_stages_identical(fglib, graph):
+    """Body stages on their own stream (the default) give the bits of the
+    in-line order, replayed as a graph and eagerly."""
+    ens = _ensemble()
+    a = _batch(fglib, ens, 40, graph=graph, overlap=True)
+    b = _batch(fglib, ens, 40, graph=graph, overlap=False)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for k in a[2]:
+        assert np.array_equal(a[2][k], b[2][k]), k
+
+
+@pytest.mark.parametrize("tuning", [(256, 8, 0, 0, 1), (128, 8, 0, 1, 4), (64, 8, 1, 1, 1)])
+def test_persistent_queue_launches_identical(fglib, tuning):
+    """The persistent work-queue form of the density and (coupled) force
+    launches gives the bits of the one-workgroup-per-block launches."""
+    ens = _ensemble()
+    L = fglib.load()
+    try:
+        a = _batch(fglib, ens, 30)
+        b = _batch(fglib, ens, 30, tuning=tuning)
+    finally:
+        L.fg_set_schedule(0, 0)
+        L.fg_set_tuning(256, 8)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     for k in a[2]:
         assert np.array_equal(a[2][k], b[2][k]), k
