@@ -202,14 +202,14 @@ def measure_batched(ens, steps, warmup, tune_steps=5):
             b.close()
         out["tuning"] = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
                          "flat_density": t["flat_density"], "flat_forces": t["flat_forces"],
-                         "queue_items": t["queue_items"],
-                         "trials_ms": [[a, c, fd, ff, q, round(m, 4)] for a, c, fd, ff, q, m in t["trials"]],
+                         "queue_items": t["queue_items"], "defer_contact": t["defer_contact"],
+                         "trials_ms": [[a, c, fd, ff, q, dc, round(m, 4)] for a, c, fd, ff, q, dc, m in t["trials"]],
                          "trials_fields": ["sph_block", "sort_digit_bits", "flat_density", "flat_forces",
                                            "queue_items (0: one workgroup per block; k: persistent, k blocks "
-                                           "per work item)", "ms_per_step"]}
-        shapes = {"fixed": (256, 8, False, False, 0),
+                                           "per work item)", "defer_contact", "ms_per_step"]}
+        shapes = {"fixed": (256, 8, False, False, 0, False),
                   "tuned": (t["sph_block"], t["sort_digit_bits"], t["flat_density"], t["flat_forces"],
-                            t["queue_items"])}
+                            t["queue_items"], t["defer_contact"])}
         for label, shape in shapes.items():
             ev_all = []
             for rep in range(REPEATS):

@@ -181,6 +181,59 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5)))
     });
 }
 
+// The coupled force pass with body contact deferred (forces_particle with
+// DEFER): without the contact walk it needs fewer registers than
+// k_batch_forces_coupled.  amdgpu_waves_per_eu(7): 72 VGPRs without
+// scratch (73 and occupancy 6 without the hint; 8 waves spill to scratch).
+template <int BS, bool FLAT, bool PERSIST>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7)))
+    k_batch_forces_deferred(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes,
+                            const uint32_t* __restrict__ scene_of, const float4* __restrict__ sp,
+                            const float4* __restrict__ sv, const float2* __restrict__ rho_p,
+                            const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                            const unsigned long long* __restrict__ masks, float4* __restrict__ vout,
+                            float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n,
+                            const BatchBodyScene* __restrict__ bodies, DeferredContact dc, WorkQueue q) {
+    __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
+    __shared__ uint32_t s_mm[FLAT ? 9 * BS : 1];
+    FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
+    for_each_particle<BS, PERSIST>(q, n, [&](uint32_t i) {
+        for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
+            const fg_scene_desc& d = scenes[s];
+            if (!scene_stencil_covers_h(d.common)) return refuse_scene_particle(i, id, n, vout);
+            Contacts c{};
+            c.dyn = bodies[s].dyn;
+            c.coupled = c.dyn.enabled;
+            forces_particle<true, true, true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, d.mass, d.viscosity,
+                                              d.common, d.kernels, d.keyBase, masks, c, FLAT ? &fr : nullptr, &dc);
+        });
+    });
+}
+
+// Second pass of the deferred form: contact, walls, clamp and integration
+// of the deferred particles (finish_accel<true> and write_back, as
+// forces_particle ends).
+__global__ void __launch_bounds__(256)
+    k_batch_contact_finish(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes,
+                           const uint32_t* __restrict__ scene_of, const float4* __restrict__ sp,
+                           const float4* __restrict__ sv, float4* __restrict__ vout, float4* __restrict__ pout,
+                           const uint32_t* __restrict__ id, uint32_t n, const BatchBodyScene* __restrict__ bodies,
+                           DeferredContact dc) {
+    const uint32_t cnt = min(*dc.count, n);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const uint32_t i = dc.list[k];
+        for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
+            const fg_scene_desc& d = scenes[s];
+            Contacts c{};
+            c.dyn = bodies[s].dyn;
+            c.coupled = c.dyn.enabled;
+            const float4 pos = sp[i], vel = sv[i], ia = dc.acc[i];
+            const F3 a = finish_accel<true>(f3(ia.x, ia.y, ia.z), d.mass, pos, vel, d.common, c);
+            write_back(id[i], n, pos, vel, a, d.common, vout, pout);
+        });
+    }
+}
+
 // Workgroups of kernel k (BS lanes) the GPU holds at once, a multiple of the
 // queue's slices: the grid of a persistent launch.
 static unsigned persistent_grid(const void* k, int bs) {
@@ -293,6 +346,34 @@ int batch_forces_coupled(const fg_scene_desc* scenes, uint32_t nscenes, const ui
                          (const float4*)sv, (const float2*)rho_p, (const uint32_t*)start, (const uint32_t*)end, masks,
                          vel, pos, (const uint32_t*)id, n, bodies);
     return check_launch("batch_forces_coupled") ? FG_OK : fg_peek_error();
+}
+
+int batch_forces_deferred(const fg_scene_desc* scenes, uint32_t nscenes, const uint32_t* scene_of, float4* sp,
+                          float4* sv, float2* rho_p, uint32_t* start, uint32_t* end,
+                          const unsigned long long* masks, float4* vel, float4* pos, uint32_t* id, uint32_t n,
+                          const BatchBodyScene* bodies, DeferredContact dc, WorkQueue q) {
+    if (n == 0) return FG_OK;
+    if (!scenes || !scene_of || !sp || !sv || !rho_p || !start || !end || !masks || !vel || !id || !bodies ||
+        nscenes == 0 || (void*)vel == (void*)sv || !dc.acc || !dc.list || !dc.count) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "batch_forces_deferred: null or aliased pointer, or no scenes");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    if (!queue_ok(q, "batch_forces_deferred")) return FG_ERR_INVALID_ARGUMENT;
+    hipStream_t st = state().stream;
+    launch_fill_u32(dc.count, 0u, 1, st);
+    if (state().flat_forces)
+        FG_BATCH_LAUNCH2(k_batch_forces_deferred, true, n, q, scenes, nscenes, scene_of, (const float4*)sp,
+                         (const float4*)sv, (const float2*)rho_p, (const uint32_t*)start, (const uint32_t*)end, masks,
+                         vel, pos, (const uint32_t*)id, n, bodies, dc);
+    else
+        FG_BATCH_LAUNCH2(k_batch_forces_deferred, false, n, q, scenes, nscenes, scene_of, (const float4*)sp,
+                         (const float4*)sv, (const float2*)rho_p, (const uint32_t*)start, (const uint32_t*)end, masks,
+                         vel, pos, (const uint32_t*)id, n, bodies, dc);
+    unsigned blocks = grid_for(n, 256);
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_batch_contact_finish, dim3(blocks), dim3(256), 0, st, scenes, nscenes, scene_of,
+                       (const float4*)sp, (const float4*)sv, vel, pos, (const uint32_t*)id, n, bodies, dc);
+    return check_launch("batch_forces_deferred") ? FG_OK : fg_peek_error();
 }
 
 }  // namespace fg

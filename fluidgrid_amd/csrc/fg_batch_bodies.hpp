@@ -54,5 +54,13 @@ int batch_forces_coupled(const fg_scene_desc* scenes, uint32_t nscenes, const ui
                          float4* sv, float2* rho_p, uint32_t* start, uint32_t* end, const unsigned long long* masks,
                          float4* vel, float4* pos, uint32_t* id, uint32_t n, const BatchBodyScene* bodies,
                          WorkQueue q = WorkQueue{nullptr, 1});
+// The same in two passes with the same bits: the force pass (compiled
+// without the contact walk) finishes the particles outside every body
+// grid's bounds and defers the others (dc: n entries each, count cleared
+// here), which a second pass finishes with the contact walk.
+int batch_forces_deferred(const fg_scene_desc* scenes, uint32_t nscenes, const uint32_t* scene_of, float4* sp,
+                          float4* sv, float2* rho_p, uint32_t* start, uint32_t* end,
+                          const unsigned long long* masks, float4* vel, float4* pos, uint32_t* id, uint32_t n,
+                          const BatchBodyScene* bodies, DeferredContact dc, WorkQueue q = WorkQueue{nullptr, 1});
 
 }  // namespace fg

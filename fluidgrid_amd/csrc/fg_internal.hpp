@@ -93,6 +93,15 @@ struct Contacts {
     int coupled;
 };
 
+// Deferred body contact: the force pass hands particles on which contact
+// can act to a second pass (contact, walls, clamp, integration) with their
+// pair sum `internal`, appended to a list; the rest it finishes itself.
+struct DeferredContact {
+    float4* acc;      // internal acceleration per sorted particle (xyz)
+    uint32_t* list;   // sorted indices of the deferred particles
+    uint32_t* count;  // their number
+};
+
 // Rigid bodies of one scene of a batch.  All scenes' bodies share global
 // arrays (bodies scene-major, spheres in body order, w = global body index);
 // each scene's body grid occupies its own bucket range (grid.key_base) of

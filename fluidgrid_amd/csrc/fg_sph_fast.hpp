@@ -13,15 +13,16 @@ namespace fg {
 #pragma clang fp contract(off)
 
 // Acceleration after gravity, walls and clamp (K_FluidSim.cu:287-303), from
-// the pair sum `internal`.
-template <bool COUPLED>
+// the pair sum `internal`.  NO_CONTACT: the caller knows the particle lies
+// outside every body grid's bounds, where fluid_external is exactly +0.
+template <bool COUPLED, bool NO_CONTACT = false>
 __device__ __forceinline__ F3 finish_accel(F3 internal, float m, const float4& pos, const float4& vel,
                                            const fg_common_fluid_params& cp, const Contacts& bc) {
     const F3 g = f3(cp.global_external_force.x, cp.global_external_force.y, cp.global_external_force.z);
     F3 total = add(mul(internal, m), g);
     // K_FluidSim.cu:389-390: body contact forces are added after the mass
     // factor (the uncoupled kernel adds nothing here).
-    if (COUPLED && bc.coupled) total = add(total, fluid_external(pos, vel, bc));
+    if (COUPLED && bc.coupled) total = add(total, NO_CONTACT ? f3(0.0f, 0.0f, 0.0f) : fluid_external(pos, vel, bc));
     F3 wall = walls_no_penetration(xyz(pos), xyz(vel), cp);
     wall = add(wall, walls_no_slip(xyz(pos), xyz(vel), cp));
     total = add(total, wall);
@@ -31,6 +32,32 @@ __device__ __forceinline__ F3 finish_accel(F3 internal, float m, const float4& p
         total = mul(total, t);
     }
     return total;
+}
+
+// Whether body contact can act on a particle at pos: it lies inside the
+// expanded bounds of an enabled body grid (or the grid has none), the test
+// with which fluid_body_contact skips its walk.
+__device__ __forceinline__ bool contact_possible(const float4& pos, const Contacts& c) {
+    auto inside = [&](const BodyContact& b) -> bool {
+        if (!b.enabled) return false;
+        if (!b.aabb) return true;
+        const float4 lo = b.aabb[0], hi = b.aabb[1];
+        return !(pos.x < lo.x || pos.y < lo.y || pos.z < lo.z || pos.x > hi.x || pos.y > hi.y || pos.z > hi.z);
+    };
+    return inside(c.st) || inside(c.dyn);
+}
+
+__device__ __forceinline__ void defer_particle(const DeferredContact& d, uint32_t i, F3 internal) {
+    d.acc[i] = make_float4(internal.x, internal.y, internal.z, 0.0f);
+    // one atomic per wavefront: the deferring lanes take consecutive slots
+    const uint64_t lanes = __ballot(1);
+    const uint32_t leader = (uint32_t)__ffsll((long long)lanes) - 1u;
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(d.count, (uint32_t)__popcll(lanes));
+    base = (uint32_t)__shfl((int)base, (int)leader);
+    const uint32_t lane = __lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (lanes & ((~0ull) >> (64 - lane)));
+    d.list[base + (uint32_t)__popcll(below)] = i;
 }
 
 // vel[id] = v + a*dt, and optionally pos[id] = p + vel[id]*(dt/s) where
@@ -484,7 +511,7 @@ struct FlatRows {
 //    for its lane with the most neighbours overall rather than, row by row,
 //    for the lane with the most neighbours in that row.  A particle with a
 //    row longer than kFlatRowBits takes the row loop.
-template <bool MASKS, bool COUPLED = false>
+template <bool MASKS, bool COUPLED = false, bool DEFER = false>
 __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __restrict__ sp,
                                                 const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                                                 const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
@@ -492,7 +519,8 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                                                 const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
                                                 const fg_common_fluid_params& cp, const fg_kernels_cache& kc,
                                                 uint32_t key_base, const unsigned long long* __restrict__ masks,
-                                                const Contacts& bc, FlatRows* flat = nullptr) {
+                                                const Contacts& bc, FlatRows* flat = nullptr,
+                                                const DeferredContact* defer = nullptr) {
     const float4 pos = sp[i];
     const float4 vel = sv[i];
     F3 a = f3(0.0f, 0.0f, 0.0f);
@@ -568,7 +596,12 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
         const float cV = kc.kViscosityLaplacian * mu * iri;
         const F3 internal = f3(__builtin_fmaf(cP, A.px, cV * A.vx), __builtin_fmaf(cP, A.py, cV * A.vy),
                                __builtin_fmaf(cP, A.pz, cV * A.vz));
-        a = finish_accel<COUPLED>(internal, m, pos, vel, cp, bc);
+        if (DEFER && bc.coupled && contact_possible(pos, bc)) {
+            defer_particle(*defer, i, internal);
+            return;
+        }
+        a = DEFER ? finish_accel<true, true>(internal, m, pos, vel, cp, bc)
+                  : finish_accel<COUPLED>(internal, m, pos, vel, cp, bc);
     }
     write_back(id[i], n, pos, vel, a, cp, vout, pout);
 }

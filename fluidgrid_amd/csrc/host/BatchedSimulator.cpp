@@ -173,6 +173,7 @@ void BatchedSimulator::releaseBodies() {
     dfree(d_brel); dfree(d_bworld); dfree(d_bsorted); dfree(d_bpf); dfree(d_bpt); dfree(d_bcm); dfree(d_bvel);
     dfree(d_borient); dfree(d_bL); dfree(d_bforce); dfree(d_btorque); dfree(d_binv); dfree(d_sceneBounds);
     dfree(d_bodyBounds); dfree(d_bmass); dfree(d_bsumF); dfree(d_bsumT);
+    dfree(d_defAcc); dfree(d_defList); dfree(d_defCount);
     if (d_bsortWs) (void)hipFree(d_bsortWs);
     d_bsortWs = nullptr;
     mBodiesReady = false;
@@ -285,6 +286,9 @@ void BatchedSimulator::setupBodies() {
     d_binv = dalloc<float4>(3 * B);
     d_sceneBounds = dalloc<float4>(2 * S);
     d_bodyBounds = dalloc<float4>(2 * B);
+    d_defAcc = dalloc<float4>(mN);
+    d_defList = dalloc<unsigned>(mN);
+    d_defCount = dalloc<unsigned>(64);
     d_bmass = dalloc<float>(B);
     d_bsumF = dalloc<double>(4 * B);
     d_bsumT = dalloc<double>(4 * B);
@@ -415,8 +419,7 @@ void BatchedSimulator::enqueueStepOverlapped() {
                           d_bodyBounds);
     fg_set_stream(mStream);
     hip_check(hipStreamWaitEvent(mStream, mSync[kBodyGrid], 0), "wait");
-    fg::batch_forces_coupled(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, d_vel, d_pos,
-                             d_id, mN, static_cast<const fg::BatchBodyScene*>(d_bscenes), work_queue(d_queue, mTuning));
+    enqueueCoupledForces();
     hip_check(hipEventRecord(mSync[kForces], mStream), "event");
     hip_check(hipStreamWaitEvent(mBodyStream, mSync[kForces], 0), "wait");
     fg_set_stream(mBodyStream);
@@ -424,6 +427,19 @@ void BatchedSimulator::enqueueStepOverlapped() {
     hip_check(hipEventRecord(mSync[kJoin], mBodyStream), "event");
     fg_set_stream(mStream);
     hip_check(hipStreamWaitEvent(mStream, mSync[kJoin], 0), "wait");
+}
+
+// The coupled fluid force pass (forces + contact + walls + integration).
+void BatchedSimulator::enqueueCoupledForces() {
+    const unsigned S = (unsigned)mDescs.size();
+    const fg::BatchBodyScene* bs = static_cast<const fg::BatchBodyScene*>(d_bscenes);
+    if (mTuning.deferContact)
+        fg::batch_forces_deferred(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, d_vel, d_pos,
+                                  d_id, mN, bs, fg::DeferredContact{d_defAcc, d_defList, d_defCount},
+                                  work_queue(d_queue, mTuning));
+    else
+        fg::batch_forces_coupled(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, d_vel, d_pos,
+                                 d_id, mN, bs, work_queue(d_queue, mTuning));
 }
 
 // Stage order of HeterogeneousSimulator::doTimeStep for the fused fast path.
@@ -449,8 +465,7 @@ void BatchedSimulator::enqueueStep() {
         fg::batch_body_forces(d_bworld, d_bvel, d_bcm, d_bpf, d_bpt, mSphereCount, mBodyCount, d_bodyScene,
                               static_cast<const fg::BatchBodyScene*>(d_bscenes), d_spos, d_svel, d_start, d_end, mN,
                               d_bodyBounds);
-        fg::batch_forces_coupled(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, d_vel, d_pos,
-                                 d_id, mN, static_cast<const fg::BatchBodyScene*>(d_bscenes), work_queue(d_queue, mTuning));
+        enqueueCoupledForces();
         enqueueBodyStages(false);
     } else {
         fg::batch_forces(d_descs, S, d_sceneOf, d_spos, d_svel, d_rho, d_start, d_end, d_masks, d_vel, d_pos, d_id,
@@ -580,6 +595,12 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
         BatchTuning t = best;
         t.queueItems = items;
         if (t.queueItems == best.queueItems) continue;
+        const float ms = time_shape(t);
+        if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+    if (mBodyCount) {
+        BatchTuning t = best;
+        t.deferContact = !best.deferContact;
         const float ms = time_shape(t);
         if (ms < best_ms) { best_ms = ms; best = t; }
     }

@@ -177,16 +177,18 @@ int fg_bsim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, 
             tuning[2] = t.flatDensity ? 1 : 0;
             tuning[3] = t.flatForces ? 1 : 0;
             tuning[4] = t.queueItems;
+            tuning[5] = t.deferContact ? 1 : 0;
         }
         int k = 0;
         for (; trials && k < maxTrials && k < (int)log.size(); k++) {
-            float* r = trials + 6 * k;
+            float* r = trials + 7 * k;
             r[0] = (float)log[k].first.sphBlock;
             r[1] = (float)log[k].first.sortDigitBits;
             r[2] = log[k].first.flatDensity ? 1.0f : 0.0f;
             r[3] = log[k].first.flatForces ? 1.0f : 0.0f;
             r[4] = (float)log[k].first.queueItems;
-            r[5] = log[k].second;
+            r[5] = log[k].first.deferContact ? 1.0f : 0.0f;
+            r[6] = log[k].second;
         }
         if (numTrials) *numTrials = k;
         return FG_OK;
@@ -195,7 +197,8 @@ int fg_bsim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, 
     }
 }
 
-int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces, int queueItems) {
+int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces, int queueItems,
+                       int deferContact) {
     if (!batch()) return FG_ERR_NOT_INITIALIZED;
     try {
         Simulators::BatchTuning t;
@@ -204,6 +207,7 @@ int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int fla
         t.flatDensity = flatDensity != 0;
         t.flatForces = flatForces != 0;
         t.queueItems = queueItems;
+        t.deferContact = deferContact != 0;
         batch()->setTuning(t);
         return FG_OK;
     } catch (const std::exception& e) {

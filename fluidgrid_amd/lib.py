@@ -159,7 +159,7 @@ SIGNATURES = {
     "fg_bsim_stage_times": (i32, [vp]),
     "fg_bsim_info": (i32, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_int)]),
     "fg_bsim_autotune": (i32, [i32, vp, vp, i32, C.POINTER(C.c_int)]),
-    "fg_bsim_set_tuning": (i32, [i32, i32, i32, i32, i32]),
+    "fg_bsim_set_tuning": (i32, [i32, i32, i32, i32, i32, i32]),
     "fg_bsim_destroy": (i32, []),
 }
 
@@ -529,18 +529,19 @@ class BatchSimulator:
 
     def autotune(self, trial_steps=10):
         nt = C.c_int()
-        tun = np.zeros(5, np.int32)
-        trials = np.zeros((32, 6), np.float32)
+        tun = np.zeros(6, np.int32)
+        trials = np.zeros((32, 7), np.float32)
         check(load().fg_bsim_autotune(int(trial_steps), tun.ctypes.data, trials.ctypes.data, 32, C.byref(nt)),
               "fg_bsim_autotune")
         return dict(sph_block=int(tun[0]), sort_digit_bits=int(tun[1]), flat_density=bool(tun[2]),
-                    flat_forces=bool(tun[3]), queue_items=int(tun[4]),
-                    trials=[(int(a), int(b), int(c), int(d), int(q), float(e)) for a, b, c, d, q, e in
-                            trials[:nt.value]])
+                    flat_forces=bool(tun[3]), queue_items=int(tun[4]), defer_contact=bool(tun[5]),
+                    trials=[(int(a), int(b), int(c), int(d), int(q), int(dc), float(e))
+                            for a, b, c, d, q, dc, e in trials[:nt.value]])
 
-    def set_tuning(self, sph_block=256, sort_digit_bits=8, flat_density=False, flat_forces=False, queue_items=0):
+    def set_tuning(self, sph_block=256, sort_digit_bits=8, flat_density=False, flat_forces=False, queue_items=0,
+                   defer_contact=True):
         check(load().fg_bsim_set_tuning(int(sph_block), int(sort_digit_bits), int(flat_density), int(flat_forces),
-                                        int(queue_items)), "fg_bsim_set_tuning")
+                                        int(queue_items), int(defer_contact)), "fg_bsim_set_tuning")
 
     def close(self):
         load().fg_bsim_destroy()

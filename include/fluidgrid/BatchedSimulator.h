@@ -50,6 +50,10 @@ struct BatchTuning {
     // Persistent density and force launches taking work items of this many
     // workgroups' particles from a queue (0: one workgroup per block).
     int queueItems = 0;
+    // Coupled force pass in two passes: particles outside every body grid's
+    // bounds are finished by a force kernel without the contact walk, the
+    // others by a second pass with it (same bits).
+    bool deferContact = true;
 };
 
 class BatchedSimulator {
@@ -70,7 +74,8 @@ public:
 
     // Times launch shapes and schedules by coordinate descent (workgroup
     // size, then the density and force schedules, then the persistent
-    // work-queue launches (none, 1 or 4 workgroups per item), then the digit width;
+    // work-queue launches (none, 1 or 4 workgroups per item), then the
+    // deferred contact pass (with bodies), then the digit width;
     // trialSteps graph steps each after a warm-up, every trial from the
     // current particle state) and keeps the fastest; the state is restored
     // afterwards, so tuning does not advance the simulation.  No choice
@@ -103,6 +108,7 @@ public:
 private:
     void enqueueStep();
     void enqueueStepOverlapped();
+    void enqueueCoupledForces();
     void enqueueBodyStages(bool before_density);
     void invalidateGraph();
     void setupBodies();
@@ -143,6 +149,8 @@ private:
     float4 *d_btorque = nullptr, *d_binv = nullptr, *d_sceneBounds = nullptr, *d_bodyBounds = nullptr;
     float* d_bmass = nullptr;
     double *d_bsumF = nullptr, *d_bsumT = nullptr;
+    float4* d_defAcc = nullptr;  // deferred contact: pair sums, particle list and count
+    unsigned *d_defList = nullptr, *d_defCount = nullptr;
     void* d_bsortWs = nullptr;
     size_t mBSortWsBytes = 0;
 };

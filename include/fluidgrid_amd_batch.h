@@ -46,18 +46,23 @@ int fg_bsim_stage_times(float* out5);
 /* numParticles, numBuckets, keyBits of the batch. */
 int fg_bsim_info(unsigned int* numParticles, unsigned int* numBuckets, int* keyBits);
 /* Launch-shape tuning: times each candidate for trialSteps steps, keeps the
- * fastest and restores the particle state.  tuning (optional, 5 ints)
- * receives (sphBlock, sortDigitBits, flatDensity, flatForces, queueItems) of
- * the choice; trials (optional, capacity maxTrials records of 6 floats)
- * receives one (sphBlock, sortDigitBits, flatDensity, flatForces,
- * queueItems, ms per step) record per candidate timed; *numTrials their
- * count. */
+ * fastest and restores the particle state.  tuning (optional, 6 ints)
+ * receives (sphBlock, sortDigitBits, flatDensity, flatForces, queueItems,
+ * deferContact) of the choice; trials (optional, capacity maxTrials records
+ * of 7 floats) receives one (sphBlock, sortDigitBits, flatDensity,
+ * flatForces, queueItems, deferContact, ms per step) record per candidate
+ * timed; *numTrials their count. */
 int fg_bsim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, int* numTrials);
 /* queueItems: 0 launches one workgroup per block of particles; 1..64 runs
  * the density and force passes as persistent launches (as many workgroups
  * as the GPU holds at once) that take work items of that many blocks'
- * particles from per-XCD counters.  No choice changes any result bit. */
-int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces, int queueItems);
+ * particles from per-XCD counters.  deferContact: with bodies, the coupled
+ * force pass runs as a force kernel without the body-contact walk, which
+ * finishes the particles outside every body grid's bounds, and a second
+ * pass that finishes the others with it (default 1).  No choice changes any
+ * result bit. */
+int fg_bsim_set_tuning(int sphBlock, int sortDigitBits, int flatDensity, int flatForces, int queueItems,
+                       int deferContact);
 int fg_bsim_destroy(void);
 
 #ifdef __cplusplus

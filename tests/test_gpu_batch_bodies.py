@@ -105,10 +105,12 @@ def test_overlapped_and_inline_body_stages_identical(fglib, graph):
         assert np.array_equal(a[2][k], b[2][k]), k
 
 
-@pytest.mark.parametrize("tuning", [(256, 8, 0, 0, 1), (128, 8, 0, 1, 4), (64, 8, 1, 1, 1)])
-def test_persistent_queue_launches_identical(fglib, tuning):
-    """The persistent work-queue form of the density and (coupled) force
-    launches gives the bits of the one-workgroup-per-block launches."""
+@pytest.mark.parametrize("tuning", [(256, 8, 0, 0, 1, 1), (128, 8, 0, 1, 4, 1), (64, 8, 1, 1, 1, 0),
+                                    (256, 8, 0, 0, 0, 0), (512, 8, 1, 0, 0, 0), (128, 7, 1, 1, 0, 1)])
+def test_queue_and_deferred_contact_identical(fglib, tuning):
+    """The persistent work-queue form of the density and force launches and
+    the coupled force pass with or without deferred contact give the bits of
+    the default launches (one workgroup per block, deferred contact)."""
     ens = _ensemble()
     L = fglib.load()
     try:
