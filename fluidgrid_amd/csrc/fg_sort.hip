@@ -5,42 +5,34 @@
 // 469, 542).  Ties keep their input order, so the result is the unique stable
 // permutation of the keys.
 //
-// Design (wave64, gfx950):
-//  * one upfront kernel builds the digit histograms of every pass in a single
-//    read of the keys (per-wave LDS sub-histograms, then global atomics);
-//  * each digit pass is a single sweep: a 256-lane workgroup owns a tile of
-//    4096 consecutive elements, ranks them stably with wave-wide ballot
-//    matching (peers with an equal digit found bit by bit), publishes its
-//    per-digit counts and finds its global offsets by decoupled look-back
-//    over the preceding tiles, stages the tile in LDS in digit order and
-//    writes coalesced runs;
-//  * tiles are numbered by an atomic counter at block start, so every tile a
-//    block waits for belongs to a block that is already running;
-//  * digit widths are chosen from the key width (<= 8 bits per pass), so
-//    16-bit keys take two passes, 22-bit keys three, 25/27-bit keys four.
+// Design (wave64, gfx950): a 256-lane workgroup owns a tile of 4096
+// consecutive elements; each digit pass is three launches,
+//  * upsweep: per-tile digit counts (per-wave LDS histograms; a run of equal
+//    digits in a wavefront is counted once, by its first lane), stored
+//    digit-major;
+//  * scan: one workgroup per digit turns its row of tile counts into
+//    exclusive offsets within the digit and records the digit's total;
+//  * downsweep: the tile is ranked stably with wave-wide ballot matching
+//    (peers with an equal digit found bit by bit), its global offsets are
+//    the digit base (an exclusive scan of the totals) plus the tile's offset
+//    within the digit, and it is staged in LDS in digit order and written
+//    out as coalesced runs.
+// No workgroup waits for another (a single-sweep pass with decoupled
+// look-back chains tiles: at 1M keys the chain, not the bytes, set its
+// time).  Digit widths are chosen from the key width (<= 8 bits per pass),
+// so 16-bit keys take two passes, 22-bit keys three, 25/27-bit keys four.
 #include <mutex>
 
 #include "fg_internal.hpp"
 
 namespace fg {
 
-#ifndef FG_SORT_ITEMS
-#define FG_SORT_ITEMS 16
-#endif
-#ifndef FG_SORT_LOOKBACK
-#define FG_SORT_LOOKBACK 8
-#endif
-
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = FG_SORT_ITEMS;
+constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kMaxPasses = 4;
 constexpr int kMaxRadix = 256;
-constexpr uint32_t kFlagAggregate = 1u << 30;
-constexpr uint32_t kFlagInclusive = 2u << 30;
-constexpr uint32_t kCountMask = (1u << 30) - 1;
-constexpr int kLookback = FG_SORT_LOOKBACK;  // predecessors read per look-back round trip
 
 struct SortPlan {
     int passes;
@@ -78,13 +70,15 @@ static SortPlan make_plan(int keyBits, int digitBits = 8) {
 }
 
 struct SortLayout {
-    size_t hist, counters, status, t1k, t1v, t2k, t2v, total;
+    size_t counters, totals, counts, t1k, t1v, t2k, t2v, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Workspace layout for n keys of keyBits bits, large enough for every digit
-// width the tuner may select.
+// width the tuner may select: a few counters, the digit totals and the
+// tile counts of one pass (passes run one after another), and the ping-pong
+// key/value buffers.
 static SortLayout make_layout(uint32_t n, int keyBits) {
     SortLayout L{};
     if (keyBits < 1) keyBits = 1;
@@ -92,9 +86,9 @@ static SortLayout make_layout(uint32_t n, int keyBits) {
     const int max_passes = plan_passes(keyBits, kMinDigitBits);
     const size_t tiles = (n + kSortTile - 1) / kSortTile;
     size_t o = 0;
-    L.hist = o; o = align256(o + sizeof(uint32_t) * kMaxPasses * kMaxRadix);
-    L.counters = o; o = align256(o + sizeof(uint32_t) * kMaxPasses + sizeof(uint32_t));
-    L.status = o; o = align256(o + sizeof(uint32_t) * (size_t)max_passes * tiles * kMaxRadix);
+    L.counters = o; o = align256(o + sizeof(uint32_t) * (kMaxPasses + 1));
+    L.totals = o; o = align256(o + sizeof(uint32_t) * kMaxRadix);
+    L.counts = o; o = align256(o + sizeof(uint32_t) * tiles * kMaxRadix);
     L.t1k = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
     L.t1v = o; o = align256(o + sizeof(uint32_t) * (size_t)n);
     if (max_passes >= 3) {
@@ -112,80 +106,111 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 }
 
-// Zeroes the digit histograms and tile counters of one sort call.
-__global__ void k_sort_zero(uint32_t* __restrict__ hist, uint32_t* __restrict__ counters) {
-    for (int i = threadIdx.x; i < kMaxPasses * kMaxRadix; i += blockDim.x) hist[i] = 0;
-    if (threadIdx.x <= kMaxPasses) counters[threadIdx.x] = 0;
-}
-
-// Digit histograms of all passes in one read; also clears the look-back
-// status words of every pass.
-__global__ void __launch_bounds__(kSortThreads) k_sort_histogram(const uint32_t* __restrict__ keys, uint32_t n,
-                                                                 SortPlan plan, uint32_t* __restrict__ hist,
-                                                                 uint32_t* __restrict__ status,
-                                                                 uint64_t status_words) {
-    __shared__ uint32_t s_hist[kSortWaves][kMaxPasses][kMaxRadix];
-    const int wave = threadIdx.x / 64;
-    for (int i = threadIdx.x; i < kSortWaves * kMaxPasses * kMaxRadix; i += blockDim.x)
-        (&s_hist[0][0][0])[i] = 0;
-    const uint64_t gstride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < status_words; i += gstride) status[i] = 0;
+// Upsweep: digit counts of tile blockIdx.x, into counts[d * tiles + tile].
+__global__ void __launch_bounds__(kSortThreads) k_sort_upsweep(const uint32_t* __restrict__ keys, uint32_t n,
+                                                               int shift, int bits, uint32_t* __restrict__ counts,
+                                                               uint32_t tiles) {
+    __shared__ uint32_t s_cnt[kSortWaves][kMaxRadix];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < kSortWaves * kMaxRadix; i += kSortThreads) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
-    // A wavefront holds 64 consecutive keys; runs of equal digits (frequent:
-    // neighbouring particles share cells) are counted with one LDS atomic
-    // per run, issued by the run's first lane.
-    const int lane = threadIdx.x & 63;
-    const uint64_t n_rounds = (n + gstride - 1) / gstride;
-    for (uint64_t r = 0; r < n_rounds; r++) {
-        const uint64_t i = r * gstride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        const bool valid = i < n;
-        const uint32_t k = valid ? keys[i] : 0u;
-        for (int p = 0; p < plan.passes; p++) {
-            const uint32_t d = valid ? (k >> plan.shift[p]) & ((1u << plan.bits[p]) - 1u) : 0xFFFFFFFFu;
-            const uint32_t prev = (uint32_t)__shfl_up((int)d, 1);
-            const bool head = valid && (lane == 0 || prev != d);
-            const uint64_t heads = __ballot(head) | __ballot(!valid);  // an invalid lane ends the run before it
-            if (head) {
-                const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1));
-                const uint32_t len = after ? (uint32_t)__ffsll((long long)after) : (uint32_t)(64 - lane);
-                atomicAdd(&s_hist[wave][p][d], len);
-            }
+    const uint32_t tile = blockIdx.x;
+    const uint32_t dmask = (1u << bits) - 1u;
+    const uint32_t wbase = tile * (uint32_t)kSortTile + (uint32_t)wave * 64u * kSortItems;
+    uint32_t key[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+        const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
+        key[k] = idx < n ? keys[idx] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+        const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (key[k] >> shift) & dmask : 0xFFFFFFFFu;
+        const uint32_t prev = (uint32_t)__shfl_up((int)d, 1);
+        const bool head = valid && (lane == 0 || prev != d);
+        const uint64_t heads = __ballot(head) | __ballot(!valid);  // an invalid lane ends the run before it
+        if (head) {
+            const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1));
+            const uint32_t len = after ? (uint32_t)__ffsll((long long)after) : (uint32_t)(64 - lane);
+            atomicAdd(&s_cnt[wave][d], len);
         }
     }
     __syncthreads();
-    for (int p = 0; p < plan.passes; p++) {
-        const int radix = 1 << plan.bits[p];
-        for (int d = threadIdx.x; d < radix; d += blockDim.x) {
-            uint32_t c = 0;
-            for (int w = 0; w < kSortWaves; w++) c += s_hist[w][p][d];
-            if (c) atomicAdd(&hist[p * kMaxRadix + d], c);
-        }
+    const uint32_t radix = 1u << bits;
+    for (uint32_t d = (uint32_t)tid; d < radix; d += kSortThreads) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; w++) c += s_cnt[w][d];
+        counts[(size_t)d * tiles + tile] = c;
     }
 }
 
-// One digit pass.  Element e of tile T is base + w*64*ITEMS + k*64 + lane,
-// so (wave, item, lane) order is index order and the ranking is stable.
+// Exclusive scan of v over the workgroup's 256 lanes (in lane order);
+// *total receives the sum.  Uses s_w[kSortWaves].
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; w++) {
+        const uint32_t t = s_w[w];
+        before += w < wave ? t : 0u;
+        sum += t;
+    }
+    __syncthreads();
+    *total = sum;
+    return before + x - v;
+}
+
+// Scan: row blockIdx.x (one digit) of the tile counts becomes the exclusive
+// offsets of the tiles within that digit; totals[d] its sum.
+__global__ void __launch_bounds__(kSortThreads) k_sort_scan(uint32_t* __restrict__ counts, uint32_t tiles,
+                                                            uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_w[kSortWaves];
+    uint32_t* row = counts + (size_t)blockIdx.x * tiles;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < tiles; c0 += kSortThreads) {
+        const uint32_t t = c0 + threadIdx.x;
+        const uint32_t v = t < tiles ? row[t] : 0u;
+        uint32_t sum;
+        const uint32_t ex = block_exclusive_scan(v, s_w, &sum);
+        if (t < tiles) row[t] = carry + ex;
+        carry += sum;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+// Downsweep of one digit pass.  Element e of tile T is base + w*64*ITEMS +
+// k*64 + lane, so (wave, item, lane) order is index order and the ranking
+// is stable.
 __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __restrict__ kin,
                                                             const uint32_t* __restrict__ vin,
                                                             uint32_t* __restrict__ kout,
                                                             uint32_t* __restrict__ vout, uint32_t n, int shift,
-                                                            int bits, const uint32_t* __restrict__ hist,
-                                                            uint32_t* __restrict__ status,
-                                                            uint32_t* __restrict__ counter, int values_iota) {
+                                                            int bits, const uint32_t* __restrict__ counts,
+                                                            uint32_t tiles, const uint32_t* __restrict__ totals,
+                                                            int values_iota) {
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t s_wcnt[kSortWaves][kMaxRadix];
     __shared__ uint32_t s_tstart[kMaxRadix];
     __shared__ uint32_t s_gbase[kMaxRadix];
-    __shared__ uint32_t s_doff[kMaxRadix];
-    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_w[kSortWaves];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t radix = 1u << bits, dmask = radix - 1u;
-    if (tid == 0) s_tile = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int i = tid; i < kSortWaves * kMaxRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kSortTile;
     const uint32_t wbase = base + (uint32_t)wave * 64u * kSortItems;
 
@@ -197,6 +222,11 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __re
         key[k] = valid ? kin[idx] : 0u;
         val[k] = valid ? (values_iota ? idx : vin[idx]) : 0u;
     }
+    // this tile's offset within each digit and the digit totals, loaded
+    // while the tile is ranked
+    const uint32_t d_own = (uint32_t)tid;
+    const uint32_t my_off = d_own < radix ? counts[(size_t)d_own * tiles + tile] : 0u;
+    const uint32_t my_total = d_own < radix ? totals[d_own] : 0u;
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int k = 0; k < kSortItems; k++) {
@@ -220,75 +250,25 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __re
     }
     __syncthreads();
 
-    // Per digit: offsets of each wave inside the tile, tile total, publish.
+    // Per digit (lane d): offsets of each wave inside the tile and the tile
+    // total; exclusive scans over digits of the tile totals (staging order)
+    // and of the digit totals (global digit bases).
     uint32_t tile_count = 0;
-    if ((uint32_t)tid < radix) {
+    if (d_own < radix) {
         uint32_t run = 0;
         for (int w = 0; w < kSortWaves; w++) {
-            const uint32_t c = s_wcnt[w][tid];
-            s_wcnt[w][tid] = run;
+            const uint32_t c = s_wcnt[w][d_own];
+            s_wcnt[w][d_own] = run;
             run += c;
         }
         tile_count = run;
-        const uint32_t flag = (tile == 0) ? kFlagInclusive : kFlagAggregate;
-        __hip_atomic_store(&status[(size_t)tile * kMaxRadix + tid], flag | tile_count, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        s_tstart[tid] = tile_count;
     }
-    __syncthreads();
-    // Exclusive scans over digits of the tile counts and of the pass's global
-    // digit counts (Hillis-Steele in LDS).
-    const uint32_t gcount = (uint32_t)tid < radix ? hist[tid] : 0u;
-    if ((uint32_t)tid < radix) s_doff[tid] = gcount;
-    __syncthreads();
-    for (uint32_t d = 1; d < radix; d <<= 1) {
-        uint32_t v = 0, g = 0;
-        if ((uint32_t)tid < radix && (uint32_t)tid >= d) {
-            v = s_tstart[tid - d];
-            g = s_doff[tid - d];
-        }
-        __syncthreads();
-        if ((uint32_t)tid < radix) {
-            s_tstart[tid] += v;
-            s_doff[tid] += g;
-        }
-        __syncthreads();
-    }
-    if ((uint32_t)tid < radix) {
-        s_tstart[tid] -= tile_count;
-        // Decoupled look-back for this digit, reading a window of kLookback
-        // predecessors per round trip: their aggregates are summed up to the
-        // first inclusive prefix; a predecessor that has published nothing
-        // yet ends the round, and the next round starts from it.
-        uint32_t excl = 0;
-        if (tile > 0) {
-            int p = (int)tile - 1;
-            for (;;) {
-                uint32_t st[kLookback];
-#pragma unroll
-                for (int u = 0; u < kLookback; u++)
-                    st[u] = (p - u >= 0) ? __hip_atomic_load(&status[(size_t)(p - u) * kMaxRadix + tid],
-                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                         : 0u;
-                int used = 0;
-                bool done = false;
-#pragma unroll
-                for (int u = 0; u < kLookback; u++) {
-                    if (done || used < u) continue;  // stop at the first inclusive or unpublished entry
-                    const uint32_t flag = st[u] & ~kCountMask;
-                    if (flag == 0u) continue;
-                    excl += st[u] & kCountMask;
-                    used = u + 1;
-                    done = flag == kFlagInclusive;
-                }
-                if (done) break;
-                p -= used;
-                if (used == 0) __builtin_amdgcn_s_sleep(1);
-            }
-            __hip_atomic_store(&status[(size_t)tile * kMaxRadix + tid], kFlagInclusive | (excl + tile_count),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_gbase[tid] = (s_doff[tid] - gcount) + excl;
+    uint32_t unused;
+    const uint32_t tstart = block_exclusive_scan(tile_count, s_w, &unused);
+    const uint32_t dbase = block_exclusive_scan(my_total, s_w, &unused);
+    if (d_own < radix) {
+        s_tstart[d_own] = tstart;
+        s_gbase[d_own] = dbase + my_off;
     }
     __syncthreads();
 
@@ -373,22 +353,13 @@ static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, voi
         return FG_ERR_INVALID_ARGUMENT;
     }
     char* w = static_cast<char*>(ws);
-    uint32_t* hist = reinterpret_cast<uint32_t*>(w + L.hist);
-    uint32_t* counters = reinterpret_cast<uint32_t*>(w + L.counters);
-    uint32_t* status = reinterpret_cast<uint32_t*>(w + L.status);
+    uint32_t* totals = reinterpret_cast<uint32_t*>(w + L.totals);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(w + L.counts);
     uint32_t* t1k = reinterpret_cast<uint32_t*>(w + L.t1k);
     uint32_t* t1v = reinterpret_cast<uint32_t*>(w + L.t1v);
     uint32_t* t2k = plan.passes >= 3 ? reinterpret_cast<uint32_t*>(w + L.t2k) : nullptr;
     uint32_t* t2v = plan.passes >= 3 ? reinterpret_cast<uint32_t*>(w + L.t2v) : nullptr;
     const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
-    const uint64_t status_words = (uint64_t)plan.passes * tiles * kMaxRadix;
-
-    hipLaunchKernelGGL(k_sort_zero, dim3(1), dim3(256), 0, st, hist, counters);
-    unsigned hblocks = grid_for(n, kSortThreads * 8);
-    if (hblocks > 2048) hblocks = 2048;
-    if (hblocks == 0) hblocks = 1;
-    hipLaunchKernelGGL(k_sort_histogram, dim3(hblocks), dim3(kSortThreads), 0, st, keys, n, plan, hist, status,
-                       status_words);
 
     const uint32_t* src_k = keys;
     const uint32_t* src_v = vals;
@@ -403,9 +374,11 @@ static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, voi
         } else {
             dk = t2k; dv = t2v;
         }
+        hipLaunchKernelGGL(k_sort_upsweep, dim3(tiles), dim3(kSortThreads), 0, st, src_k, n, plan.shift[p],
+                           plan.bits[p], counts, tiles);
+        hipLaunchKernelGGL(k_sort_scan, dim3(1u << plan.bits[p]), dim3(kSortThreads), 0, st, counts, tiles, totals);
         hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(kSortThreads), 0, st, src_k, src_v, dk, dv, n,
-                           plan.shift[p], plan.bits[p], hist + p * kMaxRadix,
-                           status + (size_t)p * tiles * kMaxRadix, counters + p, (p == 0 && iota) ? 1 : 0);
+                           plan.shift[p], plan.bits[p], counts, tiles, totals, (p == 0 && iota) ? 1 : 0);
         src_k = dk;
         src_v = dv;
     }
