@@ -420,14 +420,16 @@ def main():
     ap.add_argument("--no-8m", action="store_true")
     ap.add_argument("--no-33m", action="store_true", help="skip the 33.5M-particle config-4 line (--no-8m skips it too)")
     ap.add_argument("--unfused", action="store_true")
-    ap.add_argument("--untiled", action="store_true",
-                    help="run the masked density/force passes without LDS tiles (row-loop schedules)")
+    ap.add_argument("--tiled", action="store_true",
+                    help="run the masked density/force passes as LDS tiles (fg_set_tiled) instead of row loops")
     ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
     ap.add_argument("--mixed", action="store_true",
                     help="the main line in the mixed-precision form (FG_PRECISION_MIXED) instead of plain FP32")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-sampler line")
+    ap.add_argument("--no-cfg0", action="store_true",
+                    help="skip configuration 0's 100-step line (for profiles of one workload)")
     ap.add_argument("--only", choices=("33m", "batched", "mesh"),
                     help="run only that line and print it alone (for profiling one workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -442,8 +444,8 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     log("device:", lib.device_name(0))
-    if a.untiled:
-        lib.check(lib.load().fg_set_tiled(0), "fg_set_tiled")
+    if a.tiled:
+        lib.check(lib.load().fg_set_tiled(1), "fg_set_tiled")
     if a.copy_calibration:
         nbytes = a.copy_calibration << 20
         src, dst = lib.DeviceArray((nbytes,), np.uint8), lib.DeviceArray((nbytes,), np.uint8)
@@ -569,11 +571,12 @@ def main():
     if not a.no_mesh:
         log("mesh sampler ...")
         out["config"]["mesh_sampler"] = measure_mesh()
-    s0 = scenes.config_scene(0)
-    r0 = measure(s0, s0.steps, 10, fused)
-    out["config"]["cfg0"] = {"workload": f"configuration 0: {s0.n:,} particles, its {s0.steps} steps",
-                             "ms_per_step": r0["ms_per_step"], "value": s0.n * s0.steps / r0["seconds"],
-                             "seconds_100_steps": r0["seconds"], "finite": r0["finite"]}
+    if not a.no_cfg0:
+        s0 = scenes.config_scene(0)
+        r0 = measure(s0, s0.steps, 10, fused)
+        out["config"]["cfg0"] = {"workload": f"configuration 0: {s0.n:,} particles, its {s0.steps} steps",
+                                 "ms_per_step": r0["ms_per_step"], "value": s0.n * s0.steps / r0["seconds"],
+                                 "seconds_100_steps": r0["seconds"], "finite": r0["finite"]}
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(scene)
