@@ -423,6 +423,8 @@ def main():
     ap.add_argument("--tiled", action="store_true",
                     help="run the masked density/force passes as LDS tiles (fg_set_tiled) instead of row loops")
     ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
+    ap.add_argument("--sph-block", type=int, default=0, choices=(0, 64, 128, 256, 512),
+                    help="workgroup size of the fast density/force kernels (fg_set_tuning; 0: the library default)")
     ap.add_argument("--mixed", action="store_true",
                     help="the main line in the mixed-precision form (FG_PRECISION_MIXED) instead of plain FP32")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble line")
@@ -446,6 +448,8 @@ def main():
     log("device:", lib.device_name(0))
     if a.tiled:
         lib.check(lib.load().fg_set_tiled(1), "fg_set_tiled")
+    if a.sph_block:
+        lib.check(lib.load().fg_set_tuning(a.sph_block, 8), "fg_set_tuning")
     if a.copy_calibration:
         nbytes = a.copy_calibration << 20
         src, dst = lib.DeviceArray((nbytes,), np.uint8), lib.DeviceArray((nbytes,), np.uint8)
