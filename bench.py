@@ -423,6 +423,8 @@ def main():
     ap.add_argument("--tiled", action="store_true",
                     help="run the masked density/force passes as LDS tiles (fg_set_tiled) instead of row loops")
     ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="time the main line at the library's default launch shape instead of autotuning it")
     ap.add_argument("--sph-block", type=int, default=0, choices=(0, 64, 128, 256, 512),
                     help="workgroup size of the fast density/force kernels (fg_set_tuning; 0: the library default)")
     ap.add_argument("--mixed", action="store_true",
@@ -450,6 +452,8 @@ def main():
         lib.check(lib.load().fg_set_tiled(1), "fg_set_tiled")
     if a.sph_block:
         lib.check(lib.load().fg_set_tuning(a.sph_block, 8), "fg_set_tuning")
+    if a.sph_block or a.tiled:
+        a.tune = False  # a shape given on the command line is the one timed
     if a.copy_calibration:
         nbytes = a.copy_calibration << 20
         src, dst = lib.DeviceArray((nbytes,), np.uint8), lib.DeviceArray((nbytes,), np.uint8)
@@ -473,7 +477,27 @@ def main():
     log(f"scene {scene.name}: n={scene.n:,} buckets={scene.num_buckets:,} "
         f"body spheres={bodies.m if bodies is not None else 0:,}")
     prec = abi.PRECISION_MIXED if a.mixed else abi.PRECISION_FP32
+    tuning = None
+    if a.tune:
+        # the library's own tuner picks the launch shape (no result bit depends on it)
+        sim = new_sim(scene, fused, prec)
+        try:
+            sim.step(max(2, a.warmup // 4))
+            t = sim.autotune(trial_steps=5)
+        finally:
+            sim.close()
+        tuning = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
+                  "flat_density": t["flat_density"], "flat_forces": t["flat_forces"], "tiled": t["tiled"],
+                  "trials_ms": [list(x[:5]) + [round(x[5], 4)] for x in t["trials"]],
+                  "trials_fields": ["sph_block", "sort_digit_bits", "flat_density", "flat_forces", "tiled",
+                                    "ms_per_step"]}
     r = measure(scene, a.steps, a.warmup, fused, prec)
+    if a.tune:
+        # the other lines run with the library's default shape
+        L = lib.load()
+        lib.check(L.fg_set_tuning(256, 8), "fg_set_tuning")
+        L.fg_set_schedule(0, 0)
+        L.fg_set_tiled(0)
     n = scene.n
     value = n * a.steps / r["seconds"]
     stage_ms = r["stage_ms"]
@@ -521,6 +545,8 @@ def main():
             "form": ("mixed precision density (FP16 cull, exact FP32 survivors), " if a.mixed else "plain FP32, ")
                     + ("fused forces+integrate, HIP graph replay" if fused else "unfused"),
             "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+            "tuning": tuning if tuning is not None else "library default shape (256-lane workgroups, "
+                                                         "row loops, 8-bit digits)",
             "step_algorithmic_GBps": step_gbs / 1e9,
             "step_hbm_frac": step_gbs / HBM_PEAK,
             "finite": r["finite"],

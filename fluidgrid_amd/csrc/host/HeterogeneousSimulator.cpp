@@ -367,5 +367,146 @@ void HeterogeneousSimulator::doTimeSteps(int steps) {
     for (int s = 0; s < steps; s++) hip_check(hipGraphLaunch(mGraphExec, mStream), "hipGraphLaunch");
 }
 
+void HeterogeneousSimulator::applyTuning(const StepTuning& t) {
+    if (fg_set_tuning(t.sphBlock, t.sortDigitBits) != FG_OK) fg_check("applyTuning");
+    fg_set_schedule(t.flatDensity ? 1 : 0, t.flatForces ? 1 : 0);
+    fg_set_tiled(t.tiled ? 1 : 0);
+}
+
+StepTuning HeterogeneousSimulator::currentTuning() {
+    StepTuning t;
+    int blk = 256, bits = 8, fd = 0, ff = 0, tl = 0;
+    fg_get_tuning(&blk, &bits);
+    fg_get_schedule(&fd, &ff);
+    fg_get_tiled(&tl);
+    t.sphBlock = blk;
+    t.sortDigitBits = bits;
+    t.flatDensity = fd != 0;
+    t.flatForces = ff != 0;
+    t.tiled = tl != 0;
+    return t;
+}
+
+StepTuning HeterogeneousSimulator::autotune(int trialSteps, std::vector<std::pair<StepTuning, float>>* trials) {
+    if (trialSteps < 1) trialSteps = 1;
+    if (mFluids.empty()) throw std::runtime_error("autotune: no fluid");
+    hip_check(hipStreamSynchronize(mStream), "sync");
+    // What a step reads from the one before (as saveState keeps it), saved
+    // on the device (bodies and inflow counters through the host).
+    struct FluidSnap {
+        float4 *pos = nullptr, *vel = nullptr;
+        float2* rho = nullptr;
+        unsigned emit = 0;
+    };
+    std::vector<FluidSnap> snaps(mFluids.size());
+    for (size_t k = 0; k < mFluids.size(); k++) {
+        Fluid::GPUSPHFluid* f = mFluids[k];
+        const size_t n = (size_t)(f->getNumParticles() > 0 ? f->getNumParticles() : 1);
+        FluidSnap& sn = snaps[k];
+        hip_check(hipMalloc((void**)&sn.pos, sizeof(float4) * n), "autotune hipMalloc");
+        hip_check(hipMalloc((void**)&sn.vel, sizeof(float4) * n), "autotune hipMalloc");
+        hip_check(hipMalloc((void**)&sn.rho, sizeof(float2) * n), "autotune hipMalloc");
+        hip_check(hipMemcpy(sn.pos, f->getPosition(), sizeof(float4) * n, hipMemcpyDeviceToDevice), "save");
+        hip_check(hipMemcpy(sn.vel, f->getVelocity(), sizeof(float4) * n, hipMemcpyDeviceToDevice), "save");
+        hip_check(hipMemcpy(sn.rho, f->getDensityField(), sizeof(float2) * n, hipMemcpyDeviceToDevice), "save");
+        if (f->hasInflow()) sn.emit = f->readEmitStep();
+    }
+    RigidBody::RigidBodySimulator* rb = RigidBody::RigidBodySimulator::instance();
+    const size_t B = rb->getNumDynamicBodies();
+    std::vector<float4> bpos(B), bvel(B), bori(B), bL(B);
+    if (B) rb->downloadBodies(bpos.data(), bvel.data(), bori.data(), bL.data());
+    auto restore = [&]() {
+        hip_check(hipStreamSynchronize(mStream), "sync");
+        for (size_t k = 0; k < mFluids.size(); k++) {
+            Fluid::GPUSPHFluid* f = mFluids[k];
+            const size_t n = (size_t)(f->getNumParticles() > 0 ? f->getNumParticles() : 1);
+            const FluidSnap& sn = snaps[k];
+            hip_check(hipMemcpy(f->getPosition(), sn.pos, sizeof(float4) * n, hipMemcpyDeviceToDevice), "restore");
+            hip_check(hipMemcpy(f->getVelocity(), sn.vel, sizeof(float4) * n, hipMemcpyDeviceToDevice), "restore");
+            hip_check(hipMemcpy(f->getDensityField(), sn.rho, sizeof(float2) * n, hipMemcpyDeviceToDevice),
+                      "restore");
+            if (f->hasInflow()) f->writeEmitStep(sn.emit);
+        }
+        if (B) rb->uploadBodies(bpos.data(), bvel.data(), bori.data(), bL.data());
+        hip_check(hipDeviceSynchronize(), "restore");
+    };
+    const StepOptions saved = mOptions;
+    StepOptions o = mOptions;
+    o.use_graph = true;
+    o.stage_timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    auto time_shape = [&](const StepTuning& t, bool record) -> float {
+        applyTuning(t);
+        setOptions(o);  // drops the captured graph: the shapes are recorded in it
+        restore();
+        doTimeSteps(2);  // capture + warm-up
+        hip_check(hipEventRecord(e0, mStream), "event");
+        doTimeSteps(trialSteps);
+        hip_check(hipEventRecord(e1, mStream), "event");
+        hip_check(hipEventSynchronize(e1), "event");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+        ms /= (float)trialSteps;
+        if (trials && record) trials->push_back({t, ms});
+        return ms;
+    };
+    StepTuning best = currentTuning();
+    try {
+        (void)time_shape(best, false);  // the clocks of an idle GPU ramp up over the first trial
+        float best_ms = 1e30f;
+        auto consider = [&](const StepTuning& t) {
+            const float ms = time_shape(t, true);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        };
+        for (int blk : {64, 128, 256, 512}) {
+            StepTuning t = best;
+            t.sphBlock = blk;
+            consider(t);
+        }
+        const StepTuning base = best;
+        for (int sched = 1; sched < 4; sched++) {
+            StepTuning t = base;
+            t.flatDensity = (sched & 1) != 0;
+            t.flatForces = (sched & 2) != 0;
+            consider(t);
+        }
+        {
+            StepTuning t = best;
+            t.tiled = !best.tiled;
+            consider(t);
+        }
+        for (int bits : {7, 6}) {
+            StepTuning t = best;
+            t.sortDigitBits = bits;
+            consider(t);
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        for (FluidSnap& sn : snaps) {
+            (void)hipFree(sn.pos);
+            (void)hipFree(sn.vel);
+            (void)hipFree(sn.rho);
+        }
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    applyTuning(best);
+    setOptions(saved);
+    restore();
+    for (FluidSnap& sn : snaps) {
+        (void)hipFree(sn.pos);
+        (void)hipFree(sn.vel);
+        (void)hipFree(sn.rho);
+    }
+    return best;
+}
+
 }  // namespace Simulators
 }  // namespace Simulation

@@ -119,6 +119,35 @@ int fg_sim_set_precision(int mode) {
     return FG_OK;
 }
 
+int fg_sim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, int* numTrials) {
+    if (!sim()) return FG_ERR_NOT_INITIALIZED;
+    try {
+        std::vector<std::pair<Simulators::StepTuning, float>> log;
+        const Simulators::StepTuning t = Simulators::HeterogeneousSimulator::instance()->autotune(trialSteps, &log);
+        if (tuning) {
+            tuning[0] = t.sphBlock;
+            tuning[1] = t.sortDigitBits;
+            tuning[2] = t.flatDensity ? 1 : 0;
+            tuning[3] = t.flatForces ? 1 : 0;
+            tuning[4] = t.tiled ? 1 : 0;
+        }
+        int k = 0;
+        for (; trials && k < maxTrials && k < (int)log.size(); k++) {
+            float* r = trials + 6 * k;
+            r[0] = (float)log[k].first.sphBlock;
+            r[1] = (float)log[k].first.sortDigitBits;
+            r[2] = log[k].first.flatDensity ? 1.0f : 0.0f;
+            r[3] = log[k].first.flatForces ? 1.0f : 0.0f;
+            r[4] = log[k].first.tiled ? 1.0f : 0.0f;
+            r[5] = log[k].second;
+        }
+        if (numTrials) *numTrials = k;
+        return FG_OK;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 int fg_sim_step(int steps) {
     if (!sim()) return FG_ERR_NOT_INITIALIZED;
     try {

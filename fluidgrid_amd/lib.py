@@ -82,6 +82,7 @@ SIGNATURES = {
     "fg_sim_set_options": (i32, [i32, i32, i32]),
     "fg_sim_set_overlap": (i32, [i32]),
     "fg_sim_set_precision": (i32, [i32]),
+    "fg_sim_autotune": (i32, [i32, vp, vp, i32, C.POINTER(C.c_int)]),
     "fg_sim_step": (i32, [i32]),
     "fg_sim_synchronize": (i32, []),
     "fg_sim_stream": (vp, []),
@@ -323,6 +324,19 @@ class Simulator:
 
     def set_options(self, fused=True, use_graph=True, stage_timing=False):
         check(load().fg_sim_set_options(int(fused), int(use_graph), int(stage_timing)), "options")
+
+    def autotune(self, trial_steps=10):
+        """Launch-shape tuning of the single-fluid step (process-wide); the
+        simulation state is restored.  Returns the choice and the trials."""
+        nt = C.c_int()
+        tun = np.zeros(5, np.int32)
+        trials = np.zeros((32, 6), np.float32)
+        check(load().fg_sim_autotune(int(trial_steps), tun.ctypes.data, trials.ctypes.data, 32, C.byref(nt)),
+              "fg_sim_autotune")
+        return dict(sph_block=int(tun[0]), sort_digit_bits=int(tun[1]), flat_density=bool(tun[2]),
+                    flat_forces=bool(tun[3]), tiled=bool(tun[4]),
+                    trials=[(int(a), int(b), int(c), int(d), int(e), float(m)) for a, b, c, d, e, m in
+                            trials[:nt.value]])
 
     def set_precision(self, mode):
         """Density arithmetic of the fused step: abi.PRECISION_FP32 or abi.PRECISION_MIXED."""

@@ -34,6 +34,17 @@ struct StepOptions {
     int precision = FG_PRECISION_FP32;
 };
 
+// Launch shape of the single-fluid passes (process-wide library state:
+// fg_set_tuning, fg_set_schedule, fg_set_tiled).  No choice changes any
+// result bit.
+struct StepTuning {
+    int sphBlock = 256;        // workgroup size of the fast density/force kernels
+    int sortDigitBits = 8;     // widest radix digit
+    bool flatDensity = false;  // density candidates as one stream per lane
+    bool flatForces = false;   // force neighbours as one stream per lane
+    bool tiled = false;        // LDS-tiled density and force passes
+};
+
 struct StageTimes {
     float hash_ms = 0, sort_ms = 0, cells_ms = 0, density_ms = 0, forces_ms = 0, integrate_ms = 0, bodies_ms = 0;
 };
@@ -68,6 +79,15 @@ public:
     // changes nothing unless the whole file checks.
     void saveState(const std::string& path);
     void loadState(const std::string& path);
+    // Times launch shapes by coordinate descent (workgroup size, then the
+    // density and force schedules, then the LDS-tiled passes, then the
+    // digit width; trialSteps graph steps each after a capture and a warm-up
+    // step, every trial from the same state) and keeps the fastest; the
+    // simulation state (particles, (rho, p), inflow counters, bodies) is
+    // restored afterwards, so tuning does not advance the simulation.
+    StepTuning autotune(int trialSteps, std::vector<std::pair<StepTuning, float>>* trials = nullptr);
+    static void applyTuning(const StepTuning& t);
+    static StepTuning currentTuning();
 
 private:
     HeterogeneousSimulator();

@@ -43,6 +43,15 @@ int fg_sim_set_overlap(int overlap_bodies);
 /* Density arithmetic of the fused step: FG_PRECISION_FP32 (default) or
  * FG_PRECISION_MIXED (fluidgrid_amd.h). */
 int fg_sim_set_precision(int mode);
+/* Launch-shape tuning of the single-fluid step (process-wide:
+ * fg_set_tuning, fg_set_schedule, fg_set_tiled): times each candidate for
+ * trialSteps graph steps from the same state, keeps the fastest and restores
+ * the simulation state.  tuning (optional, 5 ints) receives (sphBlock,
+ * sortDigitBits, flatDensity, flatForces, tiled) of the choice; trials
+ * (optional, maxTrials records of 6 floats) the same five and the ms per
+ * step of each candidate timed; *numTrials their count.  No choice changes
+ * any result bit. */
+int fg_sim_autotune(int trialSteps, int* tuning, float* trials, int maxTrials, int* numTrials);
 int fg_sim_step(int steps);                       /* asynchronous */
 int fg_sim_synchronize(void);
 void* fg_sim_stream(void);                        /* hipStream_t of the simulator */

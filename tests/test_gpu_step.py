@@ -176,3 +176,46 @@ def test_schedules_and_block_sizes_bit_identical(fglib, which):
         L.fg_set_tiled(0)
         L.fg_set_schedule(0, 0)
         L.fg_set_tuning(256, 8)
+
+
+def test_single_fluid_tuning_changes_no_result_and_restores_state(fglib):
+    """The single-fluid tuner times launch shapes from one saved state and
+    puts the state back (particles, bodies); stepping on with its choice
+    gives the bits of the default shape."""
+    s = scenes.small_dam_break(lattice=(24, 10, 24), h=0.02, box=(0.5, 0.5, 0.5))
+    top = float(s.pos[:, 1].max())
+    tor = scenes.torus_spheres(0.06, 0.02, 0.01) + np.array([0.12, top, 0.12])
+    bodies = scenes.make_bodies([tor], [0.5], 0.02, s.common)
+    L = fglib.load()
+    try:
+        sim = fglib.Simulator(s, fused=True, use_graph=True)
+        try:
+            sim.add_bodies(bodies)
+            sim.step(3)
+            before, bbefore = sim.state(), sim.body_state()
+            t = sim.autotune(trial_steps=2)
+            after, bafter = sim.state(), sim.body_state()
+            assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+            for k in bbefore:
+                assert np.array_equal(bbefore[k], bafter[k]), k
+            assert t["sph_block"] in (64, 128, 256, 512) and 6 <= t["sort_digit_bits"] <= 8
+            assert len(t["trials"]) >= 8
+            sim.step(7)
+            tuned = sim.state()
+        finally:
+            sim.close()
+        L.fg_set_tuning(256, 8)
+        L.fg_set_schedule(0, 0)
+        L.fg_set_tiled(0)
+        sim = fglib.Simulator(s, fused=True, use_graph=True)
+        try:
+            sim.add_bodies(bodies)
+            sim.step(10)
+            ref = sim.state()
+        finally:
+            sim.close()
+        assert np.array_equal(tuned[0], ref[0]) and np.array_equal(tuned[1], ref[1])
+    finally:
+        L.fg_set_tuning(256, 8)
+        L.fg_set_schedule(0, 0)
+        L.fg_set_tiled(0)
