@@ -212,7 +212,13 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7)))
 
 // Second pass of the deferred form: contact, walls, clamp and integration
 // of the deferred particles (finish_accel<true> and write_back, as
-// forces_particle ends).
+// forces_particle ends), from the pair sum in dc.acc.  Each deferred
+// particle takes a pair of lanes that walk its stencil cells together
+// (contact_cells): a body grid's walk is a chain of dependent loads (bucket
+// range, then each sphere), and one lane per particle would walk its 27
+// cells one after another.  Of 1, 2, 4, 8 and 16 lanes per particle, 2
+// measured fastest (256 scenes with 479 bodies, 455k deferred particles).
+constexpr uint32_t kContactLanes = 2;
 __global__ void __launch_bounds__(256)
     k_batch_contact_finish(const fg_scene_desc* __restrict__ scenes, uint32_t nscenes,
                            const uint32_t* __restrict__ scene_of, const float4* __restrict__ sp,
@@ -220,15 +226,21 @@ __global__ void __launch_bounds__(256)
                            const uint32_t* __restrict__ id, uint32_t n, const BatchBodyScene* __restrict__ bodies,
                            DeferredContact dc) {
     const uint32_t cnt = min(*dc.count, n);
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    const uint32_t sub = threadIdx.x % kContactLanes;
+    const uint32_t groups = gridDim.x * (blockDim.x / kContactLanes);
+    // k is the same on every lane of a group, so a group is active or idle as a whole
+    for (uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) / kContactLanes; k < cnt; k += groups) {
         const uint32_t i = dc.list[k];
         for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
+            const BodyContact& dyn = bodies[s].dyn;
+            const float4 pos = sp[i], vel = sv[i];
+            // fluid_external with the dynamic grid only: +0 + its contact sum
+            const F3 ext = dyn.enabled ? add(f3(0.0f, 0.0f, 0.0f), contact_cells<kContactLanes>(pos, vel, dyn, sub))
+                                       : f3(0.0f, 0.0f, 0.0f);
+            if (sub != 0) return;
             const fg_scene_desc& d = scenes[s];
-            Contacts c{};
-            c.dyn = bodies[s].dyn;
-            c.coupled = c.dyn.enabled;
-            const float4 pos = sp[i], vel = sv[i], ia = dc.acc[i];
-            const F3 a = finish_accel<true>(f3(ia.x, ia.y, ia.z), d.mass, pos, vel, d.common, c);
+            const float4 ia = dc.acc[i];
+            const F3 a = finish_accel_ext(f3(ia.x, ia.y, ia.z), d.mass, pos, vel, d.common, dyn.enabled != 0, ext);
             write_back(id[i], n, pos, vel, a, d.common, vout, pout);
         });
     }
@@ -369,8 +381,9 @@ int batch_forces_deferred(const fg_scene_desc* scenes, uint32_t nscenes, const u
         FG_BATCH_LAUNCH2(k_batch_forces_deferred, false, n, q, scenes, nscenes, scene_of, (const float4*)sp,
                          (const float4*)sv, (const float2*)rho_p, (const uint32_t*)start, (const uint32_t*)end, masks,
                          vel, pos, (const uint32_t*)id, n, bodies, dc);
-    unsigned blocks = grid_for(n, 256);
-    if (blocks > 2048) blocks = 2048;
+    // up to one group per particle (the count is known only on the device)
+    unsigned blocks = grid_for((uint64_t)n * kContactLanes, 256);
+    if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_batch_contact_finish, dim3(blocks), dim3(256), 0, st, scenes, nscenes, scene_of,
                        (const float4*)sp, (const float4*)sv, vel, pos, (const uint32_t*)id, n, bodies, dc);
     return check_launch("batch_forces_deferred") ? FG_OK : fg_peek_error();

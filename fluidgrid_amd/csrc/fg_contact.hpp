@@ -7,35 +7,59 @@
 namespace fg {
 #pragma clang fp contract(off)
 
-// ProcessCellForcesD, K_DeviceFunctions.cu:32-106, over the sorted range
-// [s0, e_raw) of one bucket.  For each sphere j of the bucket within r_i + r_j + eps: F += (ks*l - kd*dot(n, v_j - v_i)) * n with
-// n = (x_j - x_i)/l, l = |x_j - x_i| (the spring term uses the distance).
-// Velocities: per body (indexed by w) when w_index, else per sphere; none
-// when vel is null.
+// One sphere of ProcessCellForcesD (K_DeviceFunctions.cu:32-106): sphere i
+// at pos; for a sphere within mind = r_i + r_j + eps, F += (ks*l - kd*dot(n,
+// v_j - v_i)) * n with n = (x_j - x_i)/l, l = |x_j - x_i| (the spring term
+// uses the distance).  Velocities: per body (indexed by w) when w_index,
+// else per sphere; none when vel is null.  A sphere's velocity enters only
+// a contact term, so it is read only for spheres in contact.
+//
+// cut2 = mind^2 (1 + 2^-20), rounded, exceeds mind^2 (1 + 2^-21): a sphere
+// with |x_j - x_i|^2 > cut2 has a distance above mind by more than two
+// units in the last place of mind, so its rounded square root is not below
+// mind and the exact test would reject it too.  Most spheres of a walked
+// bucket are rejected by that comparison of the squared distance, without
+// the square root.
+__device__ __forceinline__ F3 contact_term(F3 F, const float4& pos, uint32_t i, const float4& sp, const float4& sv,
+                                           float mind, float cut2, const float4* __restrict__ cvel, float ks,
+                                           float kd, bool w_index, bool group, int w_group, uint32_t vlimit) {
+    if (group && f2i_sat(pos.w) == w_group) return F;
+    const F3 rel = sub(xyz(pos), xyz(sp));
+    const float r2 = dot(rel, rel);
+    if (!(r2 <= cut2)) return F;  // also NaN, which the exact test rejects as well
+    const float lg = rn_sqrt(r2);  // len(rel)
+    if (lg < mind) {
+        float4 vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (cvel) {
+            const uint32_t vi = w_index ? (uint32_t)f2i_sat(pos.w) : i;
+            if (vi < vlimit) vel = cvel[vi];
+        }
+        const F3 n = divs(rel, lg);
+        const F3 rv = sub(xyz(vel), xyz(sv));
+        F = add(F, mul(ks * lg - kd * dot(n, rv), n));
+    }
+    return F;
+}
+
+// ProcessCellForcesD over the sorted range [s0, e_raw) of one bucket
+// (spheres in order; two positions are loaded at a time).
 __device__ __forceinline__ F3 contact_range(const float4& sp, const float4& sv, float sr, float cr,
                                             const float4* __restrict__ cpos, const float4* __restrict__ cvel,
                                             uint32_t s0, uint32_t e_raw, float ks, float kd, bool w_index,
                                             bool group, int w_group, float eps, uint32_t limit, uint32_t vlimit) {
     F3 F = f3(0.0f, 0.0f, 0.0f);
     const float mind = sr + cr + eps;
+    const float cut2 = rn_mul(rn_mul(mind, mind), 0x1.00001p+0f);
     if (s0 != kEmpty) {
         const uint32_t e = min(e_raw, limit);
-        for (uint32_t i = s0; i < e; i++) {
-            const float4 pos = cpos[i];
-            float4 vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (group && f2i_sat(pos.w) == w_group) continue;
-            if (cvel) {
-                const uint32_t vi = w_index ? (uint32_t)f2i_sat(pos.w) : i;
-                if (vi < vlimit) vel = cvel[vi];
-            }
-            const F3 rel = sub(xyz(pos), xyz(sp));
-            const float lg = len(rel);
-            if (lg < mind) {
-                const F3 n = divs(rel, lg);
-                const F3 rv = sub(xyz(vel), xyz(sv));
-                F = add(F, mul(ks * lg - kd * dot(n, rv), n));
-            }
+        uint32_t i = s0;
+        for (; i + 1 < e; i += 2) {
+            const float4 p0 = cpos[i], p1 = cpos[i + 1];
+            F = contact_term(F, p0, i, sp, sv, mind, cut2, cvel, ks, kd, w_index, group, w_group, vlimit);
+            F = contact_term(F, p1, i + 1, sp, sv, mind, cut2, cvel, ks, kd, w_index, group, w_group, vlimit);
         }
+        if (i < e)
+            F = contact_term(F, cpos[i], i, sp, sv, mind, cut2, cvel, ks, kd, w_index, group, w_group, vlimit);
     }
     return F;
 }
@@ -143,6 +167,73 @@ __device__ __forceinline__ F3 fluid_body_contact(const float4& pos, const float4
     return contact_stencil<false, true>(pos, vel, bc.fluid_radius, bc.pos, bc.vel, bc.body_radius, bc.ks, bc.kd,
                                         bc.relation, bc.start, bc.end, bc.grid, bc.vel != nullptr, false, 0, bc.eps,
                                         bc.count, bc.nvel);
+}
+
+// contact_stencil (any relation, LINEAR or REF_XOR grid) with the cells of
+// one sphere spread over LP consecutive lanes of a wavefront (LP a power of
+// two <= 32; sub = the lane's index in its group; every lane of the group
+// active and holding the same sphere).  Lane sub walks cells sub, sub + LP,
+// ... of the z, y, x stencil order, each into a sum of its own from zero
+// (contact_range), and after each round of LP cells the group adds the
+// round's sums in cell order, as contact_stencil adds them.  Sums equal to
+// zero are not added: the running sum starts at +0, so under
+// round-to-nearest it is never -0, and adding +0 or -0 to it changes no
+// bit.  The result is contact_stencil's, bit for bit, with the walks of
+// different cells (chains of dependent loads: bucket range, then spheres)
+// in flight together instead of one after another.
+template <uint32_t LP>
+__device__ __forceinline__ F3 contact_stencil_lanes(const float4& sp, const float4& sv, float sr,
+                                                    const float4* __restrict__ cpos, const float4* __restrict__ cvel,
+                                                    float cr, float ks, float kd, int3 rel,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ end, const GridDesc& g,
+                                                    bool w_index, bool group, int w_group, float eps, uint32_t limit,
+                                                    uint32_t vlimit, uint32_t sub) {
+    static_assert(LP >= 1 && LP <= 32 && (LP & (LP - 1)) == 0, "LP: a power of two up to 32");
+    F3 F = f3(0.0f, 0.0f, 0.0f);
+    const CellXYZ c = g.mode == FG_KEY_LINEAR ? grid_cell(sp.x, sp.y, sp.z, g) : CellXYZ{0, 0, 0};
+    const int nx = 2 * rel.x + 1, ny = 2 * rel.y + 1, cells = nx * ny * (2 * rel.z + 1);
+    const bool unit = rel.x == 1 && rel.y == 1 && rel.z == 1;
+    const uint32_t first = __lane_id() & ~(LP - 1u);
+    for (int r0 = 0; r0 < cells; r0 += (int)LP) {
+        const int k = r0 + (int)sub;
+        F3 part = f3(0.0f, 0.0f, 0.0f);
+        if (k < cells) {
+            const int x = unit ? k % 3 - 1 : k % nx - rel.x;
+            const int y = unit ? (k / 3) % 3 - 1 : (k / nx) % ny - rel.y;
+            const int z = unit ? k / 9 - 1 : k / (nx * ny) - rel.z;
+            const uint32_t key = contact_key(sp, c, x, y, z, g);
+            if (key != kEmpty) {  // a LINEAR cell outside the grid adds nothing
+                const uint32_t s0 = start[key], e = end[key];
+                part = contact_range(sp, sv, sr, cr, cpos, cvel, s0, s0 != kEmpty ? e : 0u, ks, kd, w_index, group,
+                                     w_group, eps, limit, vlimit);
+            }
+        }
+        const uint64_t nz = __ballot(part.x != 0.0f || part.y != 0.0f || part.z != 0.0f);
+        uint32_t m = (uint32_t)(nz >> first) & (LP == 32 ? 0xFFFFFFFFu : ((1u << LP) - 1u));
+        while (m) {
+            const int src = (int)first + __ffs((int)m) - 1;
+            m &= m - 1u;
+            F = add(F, f3(__shfl(part.x, src), __shfl(part.y, src), __shfl(part.z, src)));
+        }
+    }
+    return F;
+}
+
+// fluid_body_contact over LP lanes (contact_stencil_lanes).
+template <uint32_t LP>
+__device__ __forceinline__ F3 contact_cells(const float4& pos, const float4& vel, const BodyContact& bc,
+                                            uint32_t sub) {
+    if (bc.aabb) {
+        const float4 lo = bc.aabb[0], hi = bc.aabb[1];
+        if (pos.x < lo.x || pos.y < lo.y || pos.z < lo.z || pos.x > hi.x || pos.y > hi.y || pos.z > hi.z)
+            return f3(0.0f, 0.0f, 0.0f);
+    }
+    GridDesc g = bc.grid;
+    g.mode = FG_KEY_REF_XOR;  // body grids are keyed with the reference hash (body_grid in fg_rigid.hip)
+    return contact_stencil_lanes<LP>(pos, vel, bc.fluid_radius, bc.pos, bc.vel, bc.body_radius, bc.ks, bc.kd,
+                                     bc.relation, bc.start, bc.end, g, bc.vel != nullptr, false, 0, bc.eps, bc.count,
+                                     bc.nvel, sub);
 }
 
 // External force of K_FluidSim.cu:369-386: static grid first, then the

@@ -21,7 +21,15 @@ GridDesc fluid_grid_desc(const fg_common_fluid_params& cp, fg_float3 cs, uint32_
 
 #pragma clang fp contract(off)
 
-// K_RigidBodySim.cu:66-99, for body sphere i.
+// Lanes per body sphere of the body-force kernels: the stencil cells of a
+// sphere are walked by a group of kBodyLanes lanes (contact_stencil_lanes),
+// so a sphere's 27 chains of dependent loads are in flight together.
+constexpr uint32_t kBodyLanes = 8;
+
+// K_RigidBodySim.cu:66-99, for body sphere i, with its stencil walked by LP
+// lanes (sub: the lane's index in its group; LP = 1: one lane, the 27 cell
+// starts loaded together).
+template <uint32_t LP>
 __device__ __forceinline__ void body_sphere_from_fluid(uint32_t i, const float4* __restrict__ rb_pos,
                                                        const float4* __restrict__ rb_vel,
                                                        const float4* __restrict__ rb_cm, float4* __restrict__ out_f,
@@ -31,14 +39,20 @@ __device__ __forceinline__ void body_sphere_from_fluid(uint32_t i, const float4*
                                                        const uint32_t* __restrict__ sph_start,
                                                        const uint32_t* __restrict__ sph_end, uint32_t nsph,
                                                        float sph_radius, const GridDesc& g, int3 relation, float ks,
-                                                       float kd) {
+                                                       float kd, uint32_t sub = 0) {
     const float4 pos = rb_pos[i];
     const int b = f2i_sat(pos.w);
     const bool ok = b >= 0 && (uint32_t)b < nbodies;
     const float4 vel = ok ? rb_vel[b] : make_float4(0, 0, 0, 0);
     const float4 cm = ok ? rb_cm[b] : make_float4(0, 0, 0, 0);
-    const F3 F = contact_stencil<true>(pos, vel, radius, sph_pos, sph_vel, sph_radius, ks, kd, relation, sph_start, sph_end,
-                                 g, false, false, 0, 0.0001f, nsph, nsph);
+    F3 F;
+    if constexpr (LP == 1)
+        F = contact_stencil<true>(pos, vel, radius, sph_pos, sph_vel, sph_radius, ks, kd, relation, sph_start, sph_end,
+                                  g, false, false, 0, 0.0001f, nsph, nsph);
+    else
+        F = contact_stencil_lanes<LP>(pos, vel, radius, sph_pos, sph_vel, sph_radius, ks, kd, relation, sph_start,
+                                      sph_end, g, false, false, 0, 0.0001f, nsph, nsph, sub);
+    if (sub != 0) return;
     out_f[i] = make_float4(F.x, F.y, F.z, (float)b);
     const float4 dir = make_float4(pos.x - cm.x, pos.y - cm.y, pos.z - cm.z, pos.w - cm.w);
     float4 t = vec3_cross(dir, make_float4(F.x, F.y, F.z, 0.0f));
@@ -53,10 +67,11 @@ __global__ void __launch_bounds__(256)
                              const float4* __restrict__ sph_vel, const uint32_t* __restrict__ sph_start,
                              const uint32_t* __restrict__ sph_end, uint32_t nsph, float sph_radius, GridDesc g,
                              int3 relation, float ks, float kd) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    body_sphere_from_fluid(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, radius, sph_pos, sph_vel, sph_start,
-                           sph_end, nsph, sph_radius, g, relation, ks, kd);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = t / kBodyLanes;
+    if (i >= m) return;  // the same for every lane of a group
+    body_sphere_from_fluid<kBodyLanes>(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, radius, sph_pos, sph_vel,
+                                       sph_start, sph_end, nsph, sph_radius, g, relation, ks, kd, t % kBodyLanes);
 }
 
 __device__ __forceinline__ bool inside_box(const float4& p, const float4& lo, const float4& hi) {
@@ -73,15 +88,16 @@ __device__ __forceinline__ bool inside_box(const float4& p, const float4& lo, co
 // skipped walk would only have added +0.  ACCUM adds the force and torque to
 // what out_f/out_t hold (the fluid's part), else they are written.
 // For body sphere i; the broad phase looks at the bounds of bodies
-// [k0, k1) other than the sphere's own.
-template <bool ACCUM>
+// [k0, k1) other than the sphere's own.  The stencils are walked by LP
+// lanes (sub: the lane's index in its group).
+template <bool ACCUM, uint32_t LP>
 __device__ __forceinline__ void body_sphere_from_bodies(uint32_t i, const float4* __restrict__ rb_pos,
                                                         const float4* __restrict__ rb_vel,
                                                         const float4* __restrict__ rb_cm, float4* __restrict__ out_f,
                                                         float4* __restrict__ out_t, uint32_t nbodies, float radius,
                                                         const BodyContact& dyn, const BodyContact& st,
                                                         const float4* __restrict__ body_bounds, uint32_t k0,
-                                                        uint32_t k1) {
+                                                        uint32_t k1, uint32_t sub) {
     const float4 pos = rb_pos[i];
     const int b = f2i_sat(pos.w);
     const bool ok = b >= 0 && (uint32_t)b < nbodies;
@@ -93,14 +109,15 @@ __device__ __forceinline__ void body_sphere_from_bodies(uint32_t i, const float4
         for (uint32_t k = k0; !near && k < k1; k++)
             if ((int)k != b) near = inside_box(pos, body_bounds[2 * k], body_bounds[2 * k + 1]);
         if (near)
-            F = add(F, contact_stencil<false>(pos, vel, radius, dyn.pos, dyn.vel, dyn.body_radius, dyn.ks, dyn.kd,
-                                              dyn.relation, dyn.start, dyn.end, dyn.grid, true, true, b, dyn.eps,
-                                              dyn.count, dyn.nvel));
+            F = add(F, contact_stencil_lanes<LP>(pos, vel, radius, dyn.pos, dyn.vel, dyn.body_radius, dyn.ks, dyn.kd,
+                                                 dyn.relation, dyn.start, dyn.end, dyn.grid, true, true, b, dyn.eps,
+                                                 dyn.count, dyn.nvel, sub));
     }
     if (st.enabled && (st.aabb == nullptr || inside_box(pos, st.aabb[0], st.aabb[1])))
-        F = add(F, contact_stencil<false>(pos, vel, radius, st.pos, nullptr, st.body_radius, st.ks, st.kd,
-                                          st.relation, st.start, st.end, st.grid, false, false, 0, st.eps, st.count,
-                                          0u));
+        F = add(F, contact_stencil_lanes<LP>(pos, vel, radius, st.pos, nullptr, st.body_radius, st.ks, st.kd,
+                                             st.relation, st.start, st.end, st.grid, false, false, 0, st.eps,
+                                             st.count, 0u, sub));
+    if (sub != 0) return;
     const float4 dir = make_float4(pos.x - cm.x, pos.y - cm.y, pos.z - cm.z, pos.w - cm.w);
     const float4 t = vec3_cross(dir, make_float4(F.x, F.y, F.z, 0.0f));
     if (ACCUM) {
@@ -119,10 +136,11 @@ __global__ void __launch_bounds__(256)
                               const float4* __restrict__ rb_cm, float4* __restrict__ out_f,
                               float4* __restrict__ out_t, uint32_t m, uint32_t nbodies, float radius, BodyContact dyn,
                               BodyContact st, const float4* __restrict__ body_bounds, uint32_t nbounds) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    body_sphere_from_bodies<ACCUM>(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, radius, dyn, st, body_bounds,
-                                   0u, nbounds);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = t / kBodyLanes;
+    if (i >= m) return;  // the same for every lane of a group
+    body_sphere_from_bodies<ACCUM, kBodyLanes>(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, radius, dyn, st,
+                                               body_bounds, 0u, nbounds, t % kBodyLanes);
 }
 
 // K_RigidBodySim.cu:102-111
@@ -446,13 +464,15 @@ __global__ void __launch_bounds__(256)
                        const BatchBodyScene* __restrict__ scenes, const float4* __restrict__ sph_pos,
                        const float4* __restrict__ sph_vel, const uint32_t* __restrict__ sph_start,
                        const uint32_t* __restrict__ sph_end, uint32_t nsph) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = t / kBodyLanes;
+    if (i >= m) return;  // the same for every lane of a group
     const int b = f2i_sat(rb_pos[i].w);
     if (b < 0 || (uint32_t)b >= nbodies) return;
     const BatchBodyScene& d = scenes[body_scene[b]];
-    body_sphere_from_fluid(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, d.sphere_radius, sph_pos, sph_vel,
-                           sph_start, sph_end, nsph, d.fluid_radius, d.fluid, d.rb_to_fluid, d.ks, d.kd);
+    body_sphere_from_fluid<kBodyLanes>(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, d.sphere_radius, sph_pos,
+                                       sph_vel, sph_start, sph_end, nsph, d.fluid_radius, d.fluid, d.rb_to_fluid, d.ks,
+                                       d.kd, t % kBodyLanes);
 }
 
 // Scenes with one body have no body-body pass (the single-fluid simulator
@@ -462,8 +482,9 @@ __global__ void __launch_bounds__(256)
                         const float4* __restrict__ rb_cm, float4* __restrict__ out_f, float4* __restrict__ out_t,
                         uint32_t m, uint32_t nbodies, const uint32_t* __restrict__ body_scene,
                         const BatchBodyScene* __restrict__ scenes, const float4* __restrict__ body_bounds) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = t / kBodyLanes;
+    if (i >= m) return;  // the same for every lane of a group
     const int b = f2i_sat(rb_pos[i].w);
     if (b < 0 || (uint32_t)b >= nbodies) return;
     const BatchBodyScene& d = scenes[body_scene[b]];
@@ -474,8 +495,8 @@ __global__ void __launch_bounds__(256)
     dyn.eps = 0.001f;
     dyn.aabb = nullptr;
     const BodyContact st{};
-    body_sphere_from_bodies<true>(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, d.sphere_radius, dyn, st,
-                                  body_bounds, d.body_begin, d.body_end);
+    body_sphere_from_bodies<true, kBodyLanes>(i, rb_pos, rb_vel, rb_cm, out_f, out_t, nbodies, d.sphere_radius, dyn,
+                                              st, body_bounds, d.body_begin, d.body_end, t % kBodyLanes);
 }
 
 __global__ void k_bintegrate_rb(float4* __restrict__ pos, const float* __restrict__ mass, float4* __restrict__ vel,
@@ -518,10 +539,10 @@ int batch_body_forces(const float4* world, const float4* vel, const float4* cm, 
                       const float4* fluid_pos, const float4* fluid_vel, const uint32_t* fluid_start,
                       const uint32_t* fluid_end, uint32_t nfluid, const float4* body_bounds) {
     if (m == 0) return FG_OK;
-    hipLaunchKernelGGL(k_bbody_from_fluid, dim3(grid_for(m, 256)), dim3(256), 0, state().stream, world, vel, cm,
+    hipLaunchKernelGGL(k_bbody_from_fluid, dim3(grid_for((uint64_t)m * kBodyLanes, 256)), dim3(256), 0, state().stream, world, vel, cm,
                        out_f, out_t, m, nbodies, body_scene, scenes, fluid_pos, fluid_vel, fluid_start, fluid_end,
                        nfluid);
-    hipLaunchKernelGGL(k_bbody_from_bodies, dim3(grid_for(m, 256)), dim3(256), 0, state().stream, world, vel, cm,
+    hipLaunchKernelGGL(k_bbody_from_bodies, dim3(grid_for((uint64_t)m * kBodyLanes, 256)), dim3(256), 0, state().stream, world, vel, cm,
                        out_f, out_t, m, nbodies, body_scene, scenes, body_bounds);
     return check_launch("batch_body_forces") ? FG_OK : fg_peek_error();
 }
@@ -706,7 +727,7 @@ int fg_body_forces_from_fluid(fg_float4* spheres, fg_float4* bodyVelocities, fg_
     const State& st = state();
     // The fluid grid is walked with the key function it was built with.
     const GridDesc g = fluid_grid_desc(st.common, fluidCellSize, fluidNumBuckets, st.key_mode);
-    hipLaunchKernelGGL(k_body_forces_from_fluid, dim3(grid_for(numSpheres, 256)), dim3(256), 0, st.stream,
+    hipLaunchKernelGGL(k_body_forces_from_fluid, dim3(grid_for((uint64_t)numSpheres * kBodyLanes, 256)), dim3(256), 0, st.stream,
                        FG_F4(spheres), FG_F4(bodyVelocities), FG_F4(bodyCM), FG_F4(outForces), FG_F4(outTorques),
                        numSpheres, numBodies, radius, FG_F4(fluidOrderedPositions), FG_F4(fluidOrderedVelocities),
                        fluidGridStart, fluidGridEnd, numFluid, fluidRadius, g,
@@ -754,7 +775,7 @@ void gCalculateRigidBodyForcesFromRB(fg_float4* _in_RBPositions, fg_float4* _in_
     st.grid = body_grid(_in_CellSize, _in_numSRB > 0 ? (uint32_t)_in_SRBHashTableSize : 1u);
     st.count = 0xFFFFFFFFu;
     st.nvel = 0;
-    hipLaunchKernelGGL(k_body_forces_from_bodies<false>, dim3(grid_for(_in_NumParticles, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_body_forces_from_bodies<false>, dim3(grid_for((uint64_t)_in_NumParticles * kBodyLanes, 256)), dim3(256), 0,
                        state().stream, FG_F4(_in_RBPositions), FG_F4(_in_RBVelocites), FG_F4(_in_RBCMPosition),
                        FG_F4(_out_Forces), FG_F4(_out_torque), _in_NumParticles, 0xFFFFFFFFu, _in_Radius, dyn, st,
                        (const float4*)nullptr, 0u);
@@ -793,13 +814,13 @@ int fg_body_forces_from_bodies(const fg_float4* spheres, const fg_float4* bodyVe
     }
     const float4* bb = bodies ? reinterpret_cast<const float4*>(bodyBounds) : nullptr;
     if (accumulate)
-        hipLaunchKernelGGL(k_body_forces_from_bodies<true>, dim3(grid_for(numSpheres, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_body_forces_from_bodies<true>, dim3(grid_for((uint64_t)numSpheres * kBodyLanes, 256)), dim3(256), 0,
                            state().stream, reinterpret_cast<const float4*>(spheres),
                            reinterpret_cast<const float4*>(bodyVelocities), reinterpret_cast<const float4*>(bodyCM),
                            FG_F4(outForces), FG_F4(outTorques), numSpheres, numBodies, radius, dyn, st, bb,
                            numBodies);
     else
-        hipLaunchKernelGGL(k_body_forces_from_bodies<false>, dim3(grid_for(numSpheres, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_body_forces_from_bodies<false>, dim3(grid_for((uint64_t)numSpheres * kBodyLanes, 256)), dim3(256), 0,
                            state().stream, reinterpret_cast<const float4*>(spheres),
                            reinterpret_cast<const float4*>(bodyVelocities), reinterpret_cast<const float4*>(bodyCM),
                            FG_F4(outForces), FG_F4(outTorques), numSpheres, numBodies, radius, dyn, st, bb,

@@ -12,6 +12,21 @@ namespace fg {
 
 #pragma clang fp contract(off)
 
+// Walls and clamp of K_FluidSim.cu:293-303 applied to the acceleration
+// `total` (pair sum times mass, gravity and any body contact).
+__device__ __forceinline__ F3 walls_and_clamp(F3 total, const float4& pos, const float4& vel,
+                                              const fg_common_fluid_params& cp) {
+    F3 wall = walls_no_penetration(xyz(pos), xyz(vel), cp);
+    wall = add(wall, walls_no_slip(xyz(pos), xyz(vel), cp));
+    total = add(total, wall);
+    const float speed = len(total);
+    if (speed > cp.velocity_limit) {
+        const float t = cp.velocity_limit / speed;
+        total = mul(total, t);
+    }
+    return total;
+}
+
 // Acceleration after gravity, walls and clamp (K_FluidSim.cu:287-303), from
 // the pair sum `internal`.  NO_CONTACT: the caller knows the particle lies
 // outside every body grid's bounds, where fluid_external is exactly +0.
@@ -23,15 +38,17 @@ __device__ __forceinline__ F3 finish_accel(F3 internal, float m, const float4& p
     // K_FluidSim.cu:389-390: body contact forces are added after the mass
     // factor (the uncoupled kernel adds nothing here).
     if (COUPLED && bc.coupled) total = add(total, NO_CONTACT ? f3(0.0f, 0.0f, 0.0f) : fluid_external(pos, vel, bc));
-    F3 wall = walls_no_penetration(xyz(pos), xyz(vel), cp);
-    wall = add(wall, walls_no_slip(xyz(pos), xyz(vel), cp));
-    total = add(total, wall);
-    const float speed = len(total);
-    if (speed > cp.velocity_limit) {
-        const float t = cp.velocity_limit / speed;
-        total = mul(total, t);
-    }
-    return total;
+    return walls_and_clamp(total, pos, vel, cp);
+}
+
+// The same with the contact term `ext` computed by the caller (added when
+// `coupled`, as fluid_external is above).
+__device__ __forceinline__ F3 finish_accel_ext(F3 internal, float m, const float4& pos, const float4& vel,
+                                               const fg_common_fluid_params& cp, bool coupled, F3 ext) {
+    const F3 g = f3(cp.global_external_force.x, cp.global_external_force.y, cp.global_external_force.z);
+    F3 total = add(mul(internal, m), g);
+    if (coupled) total = add(total, ext);
+    return walls_and_clamp(total, pos, vel, cp);
 }
 
 // Whether body contact can act on a particle at pos: it lies inside the
