@@ -522,12 +522,14 @@ struct FlatRows {
 // in ascending sorted order):
 //  * row loop: the 9 rows in turn; a wavefront waits at every row for its
 //    lane with the most neighbours in that row;
-//  * flat (MASKS with a row table): the row masks are gathered first, then
-//    each lane walks its own neighbour stream across rows, two neighbours
-//    per iteration with their loads issued together, so a wavefront runs
-//    for its lane with the most neighbours overall rather than, row by row,
-//    for the lane with the most neighbours in that row.  A particle with a
-//    row longer than kFlatRowBits takes the row loop.
+//  * flat (MASKS with a row table): the rows with neighbours are gathered
+//    first, compacted in row order, then each lane walks its own neighbour
+//    stream across them, two neighbours per iteration with their loads
+//    issued together, so a wavefront runs for its lane with the most
+//    neighbours overall rather than, row by row, for the lane with the
+//    most neighbours in that row.  The next row's entry is read from LDS
+//    one row ahead.  A particle with a row longer than kFlatRowBits takes
+//    the row loop.
 template <bool MASKS, bool COUPLED = false, bool DEFER = false>
 __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __restrict__ sp,
                                                 const float4* __restrict__ sv, const float2* __restrict__ rho_p,
@@ -551,48 +553,54 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
         const float4* rec = MASKS ? nbr_records(masks, n) : nullptr;
         bool rowloop = true;
         if (MASKS && flat) {
+            // The rows with neighbours, compacted in row order: (first sorted
+            // index, mask without the particle itself).
             bool fits = true;
+            uint32_t nrows = 0;
 #pragma unroll
             for (int row = 0; row < 9; row++) {
                 const int dz = row / 3 - 1, dy = row % 3 - 1;
                 uint32_t rb = 0, re = 0;
-                uint32_t mm = 0u;
                 if (row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) {
                     if (re - rb <= kFlatRowBits) {
                         unsigned long long m64 = masks[(size_t)row * n + i];
                         // the particle itself is in its own row's mask; it adds nothing
                         if (i >= rb && i < re) m64 &= ~(1ull << (i - rb));
-                        mm = (uint32_t)m64;
+                        const uint32_t mm = (uint32_t)m64;
+                        if (mm) {
+                            flat->b(nrows) = rb;
+                            flat->m(nrows) = mm;
+                            nrows++;
+                        }
                     } else {
                         fits = false;
                     }
                 }
-                flat->b(row) = rb;
-                flat->m(row) = mm;
             }
             if (fits) {
                 rowloop = false;
-                int row = 0;
-                uint32_t cm = flat->m(0), cb = flat->b(0);
-                // The next neighbour of the stream (rows in order, ascending
-                // within a row), or kEmpty at its end.
+                // cm/cb: the current row's remaining mask and first index;
+                // nm/nb: the next row's, read from LDS one row ahead.
+                uint32_t k = 0;
+                uint32_t cm = nrows > 0 ? flat->m(0) : 0u, cb = nrows > 0 ? flat->b(0) : 0u;
+                uint32_t nm = nrows > 1 ? flat->m(1) : 0u, nb = nrows > 1 ? flat->b(1) : 0u;
+                // The next neighbour of the stream (cm != 0 on entry).
                 auto next = [&]() -> uint32_t {
-                    while (cm == 0u && row < 8) {
-                        row++;
-                        cm = flat->m(row);
-                        cb = flat->b(row);
-                    }
-                    if (cm == 0u) return kEmpty;
-                    const uint32_t k = (uint32_t)__builtin_ctz(cm);
+                    const uint32_t j = cb + (uint32_t)__builtin_ctz(cm);
                     cm &= cm - 1u;
-                    return cb + k;
+                    if (cm == 0u) {
+                        cm = nm;
+                        cb = nb;
+                        k++;
+                        nm = k + 1 < nrows ? flat->m(k + 1) : 0u;
+                        nb = k + 1 < nrows ? flat->b(k + 1) : 0u;
+                    }
+                    return j;
                 };
-                for (;;) {
+                while (cm != 0u) {
                     const uint32_t j1 = next();
-                    if (j1 == kEmpty) break;
-                    const uint32_t jn = next();
-                    const bool two = jn != kEmpty;
-                    const uint32_t j2 = two ? jn : j1;
+                    const bool two = cm != 0u;
+                    const uint32_t j2 = two ? next() : j1;
                     const float4 a1 = rec[2 * (size_t)j1], b1 = rec[2 * (size_t)j1 + 1];
                     const float4 a2 = rec[2 * (size_t)j2], b2 = rec[2 * (size_t)j2 + 1];
                     const float3 q1 = make_float3(a1.x, a1.y, a1.z), q2 = make_float3(a2.x, a2.y, a2.z);
@@ -602,7 +610,6 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                                  "v"(d2.y));
                     pair_term_masked(A, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s);
                     if (two) pair_term_masked(A, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s);
-                    if (!two) break;
                 }
             }
         }
