@@ -1,8 +1,8 @@
 """Throughput of the SPH step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--fluid-only] [--no-cpu-baseline]
-                    [--no-8m] [--no-33m] [--no-batch] [--batch-scenes S] [--no-mesh]
-                    [--only {33m,batched,mesh}]
+                    [--no-8m] [--no-33m] [--no-batch] [--batch-scenes S] [--no-mesh] [--no-tune]
+                    [--no-plain] [--only {33m,batched,mesh}]
 
 A step is one pass of the per-timestep pipeline over every particle of a
 synthetic dam-break scene, driven through the C++ HeterogeneousSimulator and
@@ -177,13 +177,15 @@ def drift_report(scene, c0, c1, steps):
             "energy_change_rel": float((e1 - e0) / abs(e0)) if e0 else None}
 
 
-def measure_batched(ens, steps, warmup, tune_steps=5):
-    """The batched form over an ensemble: the launch shape is autotuned once;
-    then the fixed shape (256-lane workgroups, 8-bit digits, row schedules)
-    and the tuned shape are each timed over REPEATS fresh simulators from the
-    ensemble's initial state (median reported); per-stage times of the tuned
-    shape; and the same scenes stepped one at a time through the
-    single-fluid simulator (the plain form's way to run them)."""
+def measure_batched(ens, steps, warmup, tune_steps=5, tune=True, plain=True):
+    """The batched form over an ensemble: the launch shape is autotuned once
+    (tune=False: the library's default shape is taken instead); then the
+    fixed shape (256-lane workgroups, 8-bit digits, row schedules, contact
+    walked in the force kernel) and the tuned shape are each timed over
+    REPEATS fresh simulators from the ensemble's initial state (median
+    reported); per-stage times of the tuned shape; and (plain=True) the same
+    scenes stepped one at a time through the single-fluid simulator (the
+    plain form's way to run them)."""
     out = {}
     bodies = ens.bodies or [None] * len(ens.scenes)
 
@@ -195,12 +197,16 @@ def measure_batched(ens, steps, warmup, tune_steps=5):
         return b
 
     try:
-        b = fresh()
-        try:
-            t = b.autotune(trial_steps=tune_steps)
-        finally:
-            b.close()
-        out["tuning"] = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
+        if tune:
+            b = fresh()
+            try:
+                t = b.autotune(trial_steps=tune_steps)
+            finally:
+                b.close()
+        else:
+            t = {"sph_block": 256, "sort_digit_bits": 8, "flat_density": False, "flat_forces": False,
+                 "queue_items": 0, "defer_contact": True, "trials": []}
+        out["tuning"] = {"autotuned": tune, "sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],
                          "flat_density": t["flat_density"], "flat_forces": t["flat_forces"],
                          "queue_items": t["queue_items"], "defer_contact": t["defer_contact"],
                          "trials_ms": [[a, c, fd, ff, q, dc, round(m, 4)] for a, c, fd, ff, q, dc, m in t["trials"]],
@@ -238,6 +244,9 @@ def measure_batched(ens, steps, warmup, tune_steps=5):
     finally:
         lib.load().fg_set_tuning(256, 8)
         lib.load().fg_set_schedule(0, 0)
+    out["info"] = info
+    if not plain:
+        return out
     # plain form: every scene alone through the single-fluid simulator
     k = max(5, steps // 4)
     total_ms = 0.0
@@ -255,7 +264,6 @@ def measure_batched(ens, steps, warmup, tune_steps=5):
         finally:
             sim.close()
     out["plain_per_scene"] = {"ms_per_step": total_ms, "value": ens.n / (total_ms * 1e-3), "steps": k}
-    out["info"] = info
     return out
 
 
@@ -391,7 +399,7 @@ def line_batched(a, plain=None):
     nsph = sum(0 if x is None else x.m for x in ens.bodies)
     log(f"ensemble: {len(ens.scenes)} scenes, n={ens.n:,}")
     kb = max(20, a.steps // 2)
-    rb = measure_batched(ens, kb, max(5, a.warmup // 2))
+    rb = measure_batched(ens, kb, max(5, a.warmup // 2), tune=a.tune, plain=a.plain)
     out = {
         "workload": f"{len(ens.scenes)} independent FP32 dam-break scenes, {ens.n:,} particles, "
                     f"{ens.num_buckets:,} buckets; counts log-uniform in [4096, 65536], edges in [0.5, 1.5] m, "
@@ -401,10 +409,11 @@ def line_batched(a, plain=None):
         "value_fixed": rb["fixed"]["value"], "value_tuned": rb["tuned"]["value"],
         "ms_per_step_fixed": rb["fixed"]["ms_per_step"], "ms_per_step_tuned": rb["tuned"]["ms_per_step"],
         "tuning": rb["tuning"], "stage_ms": rb["stage_ms"], "finite": rb["finite"],
-        "plain_per_scene_value": rb["plain_per_scene"]["value"],
-        "plain_per_scene_ms_per_step": rb["plain_per_scene"]["ms_per_step"],
-        "speedup_vs_plain_per_scene": rb["tuned"]["value"] / rb["plain_per_scene"]["value"],
     }
+    if "plain_per_scene" in rb:
+        out["plain_per_scene_value"] = rb["plain_per_scene"]["value"]
+        out["plain_per_scene_ms_per_step"] = rb["plain_per_scene"]["ms_per_step"]
+        out["speedup_vs_plain_per_scene"] = rb["tuned"]["value"] / rb["plain_per_scene"]["value"]
     if plain:
         out["speedup_vs_plain_1M"] = rb["tuned"]["value"] / plain
     return out
@@ -424,12 +433,15 @@ def main():
                     help="run the masked density/force passes as LDS tiles (fg_set_tiled) instead of row loops")
     ap.add_argument("--fluid-only", action="store_true", help="drop the rigid bodies of the configuration")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
-                    help="time the main line at the library's default launch shape instead of autotuning it")
+                    help="time the main line and the batched line at the library's default launch shape "
+                         "instead of autotuning them")
     ap.add_argument("--sph-block", type=int, default=0, choices=(0, 64, 128, 256, 512),
                     help="workgroup size of the fast density/force kernels (fg_set_tuning; 0: the library default)")
     ap.add_argument("--mixed", action="store_true",
                     help="the main line in the mixed-precision form (FG_PRECISION_MIXED) instead of plain FP32")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble line")
+    ap.add_argument("--no-plain", dest="plain", action="store_false",
+                    help="batched line without the scenes stepped one at a time")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-sampler line")
     ap.add_argument("--no-cfg0", action="store_true",

@@ -106,8 +106,14 @@ __device__ __forceinline__ void body_sphere_from_bodies(uint32_t i, const float4
     F3 F = f3(0.0f, 0.0f, 0.0f);
     if (dyn.enabled) {
         bool near = body_bounds == nullptr;
-        for (uint32_t k = k0; !near && k < k1; k++)
-            if ((int)k != b) near = inside_box(pos, body_bounds[2 * k], body_bounds[2 * k + 1]);
+        if (!near) {
+            // the group's lanes test the bodies in turn (lane sub: k0 + sub, k0 + sub + LP, ...)
+            bool hit = false;
+            for (uint32_t k = k0 + sub; !hit && k < k1; k += LP)
+                if ((int)k != b) hit = inside_box(pos, body_bounds[2 * k], body_bounds[2 * k + 1]);
+            const uint32_t first = __lane_id() & ~(LP - 1u);
+            near = ((__ballot(hit) >> first) & (LP == 32 ? 0xFFFFFFFFull : ((1ull << LP) - 1ull))) != 0ull;
+        }
         if (near)
             F = add(F, contact_stencil_lanes<LP>(pos, vel, radius, dyn.pos, dyn.vel, dyn.body_radius, dyn.ks, dyn.kd,
                                                  dyn.relation, dyn.start, dyn.end, dyn.grid, true, true, b, dyn.eps,
