@@ -177,7 +177,7 @@ def drift_report(scene, c0, c1, steps):
             "energy_change_rel": float((e1 - e0) / abs(e0)) if e0 else None}
 
 
-def measure_batched(ens, steps, warmup, tune_steps=5, tune=True, plain=True):
+def measure_batched(ens, steps, warmup, tune_steps=10, tune=True, plain=True):
     """The batched form over an ensemble: the launch shape is autotuned once
     (tune=False: the library's default shape is taken instead); then the
     fixed shape (256-lane workgroups, 8-bit digits, row schedules, contact
@@ -495,7 +495,7 @@ def main():
         sim = new_sim(scene, fused, prec)
         try:
             sim.step(max(2, a.warmup // 4))
-            t = sim.autotune(trial_steps=5)
+            t = sim.autotune(trial_steps=20)
         finally:
             sim.close()
         tuning = {"sph_block": t["sph_block"], "sort_digit_bits": t["sort_digit_bits"],

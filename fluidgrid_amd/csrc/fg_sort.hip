@@ -5,8 +5,9 @@
 // 469, 542).  Ties keep their input order, so the result is the unique stable
 // permutation of the keys.
 //
-// Design (wave64, gfx950): a 256-lane workgroup owns a tile of 4096
-// consecutive elements; each digit pass is three launches,
+// Design (wave64, gfx950): a 256-lane workgroup owns a tile of 2048 or 4096
+// consecutive elements (8 or 16 per lane, by size); each digit pass is three
+// launches,
 //  * upsweep: per-tile digit counts (per-wave LDS histograms; a run of equal
 //    digits in a wavefront is counted once, by its first lane), stored
 //    digit-major;
@@ -28,8 +29,20 @@
 namespace fg {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;
+// Items per lane: 8 (2048-element tiles) up to kSmallTileMaxN keys, where
+// twice as many tiles fill the GPU better and each tile's chain of loads,
+// ranking and stores is shorter; 16 (4096-element tiles) above, where the
+// per-tile counts and their scan would grow.  Measured per sort (22-bit keys
+// at 1M: 63 us against 75; 25-bit at 8M: 230 against 227; 27-bit at 33.5M:
+// 1023 against 849).
+constexpr int kSortItemsSmall = 8;
+constexpr int kSortItemsLarge = 16;
+constexpr uint32_t kSmallTileMaxN = 1u << 22;
+static int sort_items(uint32_t n) { return n <= kSmallTileMaxN ? kSortItemsSmall : kSortItemsLarge; }
+static uint32_t sort_tiles(uint32_t n) {
+    const uint32_t tile = (uint32_t)(kSortThreads * sort_items(n));
+    return (n + tile - 1) / tile;
+}
 constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kMaxPasses = 4;
 constexpr int kMaxRadix = 256;
@@ -84,7 +97,7 @@ static SortLayout make_layout(uint32_t n, int keyBits) {
     if (keyBits < 1) keyBits = 1;
     if (keyBits > 32) keyBits = 32;
     const int max_passes = plan_passes(keyBits, kMinDigitBits);
-    const size_t tiles = (n + kSortTile - 1) / kSortTile;
+    const size_t tiles = sort_tiles(n);
     size_t o = 0;
     L.counters = o; o = align256(o + sizeof(uint32_t) * (kMaxPasses + 1));
     L.totals = o; o = align256(o + sizeof(uint32_t) * kMaxRadix);
@@ -106,7 +119,9 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 }
 
-// Upsweep: digit counts of tile blockIdx.x, into counts[d * tiles + tile].
+// Upsweep: digit counts of tile blockIdx.x (ITEMS keys per lane), into
+// counts[d * tiles + tile].
+template <int ITEMS>
 __global__ void __launch_bounds__(kSortThreads) k_sort_upsweep(const uint32_t* __restrict__ keys, uint32_t n,
                                                                int shift, int bits, uint32_t* __restrict__ counts,
                                                                uint32_t tiles) {
@@ -116,15 +131,15 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_upsweep(const uint32_t* _
     __syncthreads();
     const uint32_t tile = blockIdx.x;
     const uint32_t dmask = (1u << bits) - 1u;
-    const uint32_t wbase = tile * (uint32_t)kSortTile + (uint32_t)wave * 64u * kSortItems;
-    uint32_t key[kSortItems];
+    const uint32_t wbase = tile * (uint32_t)(kSortThreads * ITEMS) + (uint32_t)wave * 64u * ITEMS;
+    uint32_t key[ITEMS];
 #pragma unroll
-    for (int k = 0; k < kSortItems; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
         key[k] = idx < n ? keys[idx] : 0u;
     }
 #pragma unroll
-    for (int k = 0; k < kSortItems; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
         const bool valid = idx < n;
         const uint32_t d = valid ? (key[k] >> shift) & dmask : 0xFFFFFFFFu;
@@ -172,26 +187,33 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 }
 
 // Scan: row blockIdx.x (one digit) of the tile counts becomes the exclusive
-// offsets of the tiles within that digit; totals[d] its sum.
+// offsets of the tiles within that digit; totals[d] its sum.  Lane t owns
+// the run of `per` consecutive tiles starting at t*per: one workgroup scan
+// of the run sums, then each lane writes its run's offsets.
 __global__ void __launch_bounds__(kSortThreads) k_sort_scan(uint32_t* __restrict__ counts, uint32_t tiles,
                                                             uint32_t* __restrict__ totals) {
     __shared__ uint32_t s_w[kSortWaves];
     uint32_t* row = counts + (size_t)blockIdx.x * tiles;
-    uint32_t carry = 0;
-    for (uint32_t c0 = 0; c0 < tiles; c0 += kSortThreads) {
-        const uint32_t t = c0 + threadIdx.x;
-        const uint32_t v = t < tiles ? row[t] : 0u;
-        uint32_t sum;
-        const uint32_t ex = block_exclusive_scan(v, s_w, &sum);
-        if (t < tiles) row[t] = carry + ex;
-        carry += sum;
+    const uint32_t per = (tiles + kSortThreads - 1) / kSortThreads;
+    const uint32_t t0 = threadIdx.x * per;
+    const uint32_t t1 = min(t0 + per, tiles);
+    uint32_t sum = 0;
+#pragma unroll 4
+    for (uint32_t t = t0; t < t1; t++) sum += row[t];
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, s_w, &total);
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint32_t c = row[t];
+        row[t] = run;
+        run += c;
     }
-    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
 }
 
 // Downsweep of one digit pass.  Element e of tile T is base + w*64*ITEMS +
 // k*64 + lane, so (wave, item, lane) order is index order and the ranking
 // is stable.
+template <int ITEMS>
 __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __restrict__ kin,
                                                             const uint32_t* __restrict__ vin,
                                                             uint32_t* __restrict__ kout,
@@ -199,8 +221,9 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __re
                                                             int bits, const uint32_t* __restrict__ counts,
                                                             uint32_t tiles, const uint32_t* __restrict__ totals,
                                                             int values_iota) {
-    __shared__ uint32_t s_keys[kSortTile];
-    __shared__ uint32_t s_vals[kSortTile];
+    constexpr int TILE = kSortThreads * ITEMS;
+    __shared__ uint32_t s_keys[TILE];
+    __shared__ uint32_t s_vals[TILE];
     __shared__ uint32_t s_wcnt[kSortWaves][kMaxRadix];
     __shared__ uint32_t s_tstart[kMaxRadix];
     __shared__ uint32_t s_gbase[kMaxRadix];
@@ -211,12 +234,12 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __re
     for (int i = tid; i < kSortWaves * kMaxRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = blockIdx.x;
-    const uint32_t base = tile * (uint32_t)kSortTile;
-    const uint32_t wbase = base + (uint32_t)wave * 64u * kSortItems;
+    const uint32_t base = tile * (uint32_t)TILE;
+    const uint32_t wbase = base + (uint32_t)wave * 64u * ITEMS;
 
-    uint32_t key[kSortItems], val[kSortItems], rnk[kSortItems];
+    uint32_t key[ITEMS], val[ITEMS], rnk[ITEMS];
 #pragma unroll
-    for (int k = 0; k < kSortItems; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
         const bool valid = idx < n;
         key[k] = valid ? kin[idx] : 0u;
@@ -229,7 +252,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __re
     const uint32_t my_total = d_own < radix ? totals[d_own] : 0u;
     const uint64_t lt = lanemask_lt();
 #pragma unroll
-    for (int k = 0; k < kSortItems; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
         const bool valid = idx < n;
         const uint32_t d = (key[k] >> shift) & dmask;
@@ -274,7 +297,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __re
 
     // Stage the tile in digit order.
 #pragma unroll
-    for (int k = 0; k < kSortItems; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const uint32_t idx = wbase + (uint32_t)k * 64u + (uint32_t)lane;
         if (idx < n) {
             const uint32_t d = (key[k] >> shift) & dmask;
@@ -284,7 +307,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_pass(const uint32_t* __re
         }
     }
     __syncthreads();
-    const uint32_t count = (n - base) < (uint32_t)kSortTile ? (n - base) : (uint32_t)kSortTile;
+    const uint32_t count = (n - base) < (uint32_t)TILE ? (n - base) : (uint32_t)TILE;
     for (uint32_t q = tid; q < count; q += kSortThreads) {
         const uint32_t kk = s_keys[q];
         const uint32_t d = (kk >> shift) & dmask;
@@ -359,7 +382,8 @@ static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, voi
     uint32_t* t1v = reinterpret_cast<uint32_t*>(w + L.t1v);
     uint32_t* t2k = plan.passes >= 3 ? reinterpret_cast<uint32_t*>(w + L.t2k) : nullptr;
     uint32_t* t2v = plan.passes >= 3 ? reinterpret_cast<uint32_t*>(w + L.t2v) : nullptr;
-    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    const uint32_t tiles = sort_tiles(n);
+    const bool small = sort_items(n) == kSortItemsSmall;
 
     const uint32_t* src_k = keys;
     const uint32_t* src_v = vals;
@@ -374,11 +398,12 @@ static int run_sort(uint32_t* keys, uint32_t* vals, uint32_t n, int keyBits, voi
         } else {
             dk = t2k; dv = t2v;
         }
-        hipLaunchKernelGGL(k_sort_upsweep, dim3(tiles), dim3(kSortThreads), 0, st, src_k, n, plan.shift[p],
-                           plan.bits[p], counts, tiles);
+        hipLaunchKernelGGL(small ? k_sort_upsweep<kSortItemsSmall> : k_sort_upsweep<kSortItemsLarge>, dim3(tiles),
+                           dim3(kSortThreads), 0, st, src_k, n, plan.shift[p], plan.bits[p], counts, tiles);
         hipLaunchKernelGGL(k_sort_scan, dim3(1u << plan.bits[p]), dim3(kSortThreads), 0, st, counts, tiles, totals);
-        hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(kSortThreads), 0, st, src_k, src_v, dk, dv, n,
-                           plan.shift[p], plan.bits[p], counts, tiles, totals, (p == 0 && iota) ? 1 : 0);
+        hipLaunchKernelGGL(small ? k_sort_pass<kSortItemsSmall> : k_sort_pass<kSortItemsLarge>, dim3(tiles),
+                           dim3(kSortThreads), 0, st, src_k, src_v, dk, dv, n, plan.shift[p], plan.bits[p], counts,
+                           tiles, totals, (p == 0 && iota) ? 1 : 0);
         src_k = dk;
         src_v = dv;
     }

@@ -574,14 +574,20 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
     };
     // one discarded trial first: the clocks of an idle GPU ramp up over it
     (void)time_shape(mTuning, false);
-    // Coordinate descent: workgroup size, schedules, work queue, digit width.
+    // Coordinate descent from the current shape: workgroup size, schedules,
+    // work queue, contact, digit width.  A candidate replaces the incumbent
+    // only when it is faster by more than kTuneMargin, about the spread of
+    // repeated trials of one shape.
+    constexpr float kTuneMargin = 0.01f;
     BatchTuning best = mTuning;
-    float best_ms = 1e30f;
+    float best_ms = time_shape(best);
+    auto better = [&](float ms) { return ms < best_ms * (1.0f - kTuneMargin); };
     for (int blk : {64, 128, 256, 512}) {
         BatchTuning t = best;
         t.sphBlock = blk;
+        if (t.sphBlock == best.sphBlock) continue;
         const float ms = time_shape(t);
-        if (ms < best_ms) { best_ms = ms; best = t; }
+        if (better(ms)) { best_ms = ms; best = t; }
     }
     for (int sched = 0; sched < 4; sched++) {
         BatchTuning t = best;
@@ -589,27 +595,27 @@ BatchTuning BatchedSimulator::autotune(int trialSteps, std::vector<std::pair<Bat
         t.flatForces = (sched & 2) != 0;
         if (t.flatDensity == best.flatDensity && t.flatForces == best.flatForces) continue;
         const float ms = time_shape(t);
-        if (ms < best_ms) { best_ms = ms; best = t; }
+        if (better(ms)) { best_ms = ms; best = t; }
     }
     for (int items : {0, 1, 4}) {
         BatchTuning t = best;
         t.queueItems = items;
         if (t.queueItems == best.queueItems) continue;
         const float ms = time_shape(t);
-        if (ms < best_ms) { best_ms = ms; best = t; }
+        if (better(ms)) { best_ms = ms; best = t; }
     }
     if (mBodyCount) {
         BatchTuning t = best;
         t.deferContact = !best.deferContact;
         const float ms = time_shape(t);
-        if (ms < best_ms) { best_ms = ms; best = t; }
+        if (better(ms)) { best_ms = ms; best = t; }
     }
     for (int bits : {8, 7, 6}) {
         BatchTuning t = best;
         t.sortDigitBits = bits;
         if (t.sortDigitBits == best.sortDigitBits && !trials) continue;
         const float ms = time_shape(t);
-        if (ms < best_ms) { best_ms = ms; best = t; }
+        if (better(ms)) { best_ms = ms; best = t; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

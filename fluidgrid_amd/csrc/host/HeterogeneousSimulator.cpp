@@ -455,10 +455,14 @@ StepTuning HeterogeneousSimulator::autotune(int trialSteps, std::vector<std::pai
     StepTuning best = currentTuning();
     try {
         (void)time_shape(best, false);  // the clocks of an idle GPU ramp up over the first trial
-        float best_ms = 1e30f;
+        // Coordinate descent from the current shape; a candidate replaces the
+        // incumbent only when it is faster by more than kTuneMargin, about the
+        // spread of repeated trials of one shape.
+        constexpr float kTuneMargin = 0.01f;
+        float best_ms = time_shape(best, true);
         auto consider = [&](const StepTuning& t) {
             const float ms = time_shape(t, true);
-            if (ms < best_ms) {
+            if (ms < best_ms * (1.0f - kTuneMargin)) {
                 best_ms = ms;
                 best = t;
             }
@@ -466,7 +470,7 @@ StepTuning HeterogeneousSimulator::autotune(int trialSteps, std::vector<std::pai
         for (int blk : {64, 128, 256, 512}) {
             StepTuning t = best;
             t.sphBlock = blk;
-            consider(t);
+            if (t.sphBlock != best.sphBlock) consider(t);
         }
         const StepTuning base = best;
         for (int sched = 1; sched < 4; sched++) {

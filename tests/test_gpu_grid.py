@@ -54,8 +54,9 @@ def test_hash_outside_and_nan_positions(fglib, orc):
         assert np.array_equal(key, okey), mode
 
 
+# Sizes up to 4,194,304 keys run 2048-element tiles, larger ones 4096-element tiles.
 SORT_CASES = [(1, 1), (100, 8), (4095, 16), (4096, 16), (4097, 3), (65537, 22), (300_001, 25), (1_000_003, 27),
-              (200_000, 32), (50_000, 9)]
+              (200_000, 32), (50_000, 9), (4_194_304, 23), (4_194_305, 23), (5_000_011, 25)]
 
 
 @pytest.mark.parametrize("n,bits", SORT_CASES)
