@@ -15,6 +15,11 @@ oracle is tested in test_gpu_rigid.py and test_gpu_bodies.py.
   argument checks    bodies for a missing scene, a body without spheres, a
                      second set for a scene and bodies after the first step
                      are refused
+  configuration 3    the full 256-scene ensemble with its 479 bodies for
+                     500 steps: finite, active flags kept, the sorted ids a
+                     permutation, densities at least 1, and three scenes
+                     (several bodies, one body, the largest scene) bit for
+                     bit as when stepped alone
 """
 import numpy as np
 import pytest
@@ -173,3 +178,35 @@ def test_add_bodies_argument_checks(fglib):
         L.fg_last_error()
     finally:
         b.close()
+
+
+def test_configuration_3_full_ensemble_500_steps(fglib):
+    ens = scenes.ensemble(bodies=True)
+    steps = 500
+    b = fglib.BatchSimulator(ens)
+    try:
+        for sc, bs in enumerate(ens.bodies):
+            if bs is not None:
+                b.add_bodies(sc, bs)
+        b.step(steps)
+        pos, vel = b.state()
+        bst = b.body_state()
+        rp, pid = b.density()
+    finally:
+        b.close()
+    w0 = np.concatenate([sc.pos[:, 3] for sc in ens.scenes])
+    assert np.array_equal(pos[:, 3], w0)
+    assert np.isfinite(pos).all() and np.isfinite(vel).all() and np.isfinite(bst["pos"]).all()
+    assert np.array_equal(np.sort(pid), np.arange(ens.n, dtype=np.uint32))
+    assert np.all(rp[:, 0] >= 1.0)
+    counts = [0 if x is None else x.b for x in ens.bodies]
+    sizes = [sc.n for sc in ens.scenes]
+    picks = {counts.index(max(counts)), counts.index(1), int(np.argmax(sizes))}
+    p_off = np.concatenate([[0], np.cumsum(sizes)])
+    b_off = np.concatenate([[0], np.cumsum(counts)])
+    for s in sorted(picks):
+        p1, v1, b1 = _alone(fglib, ens.scenes[s], ens.bodies[s], steps)
+        assert np.array_equal(pos[p_off[s]:p_off[s + 1]], p1), s
+        assert np.array_equal(vel[p_off[s]:p_off[s + 1]], v1), s
+        for k in ("pos", "vel", "orient", "L") if b1 is not None else ():
+            assert np.array_equal(bst[k][b_off[s]:b_off[s + 1]], b1[k]), (s, k)
