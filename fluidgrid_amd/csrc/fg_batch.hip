@@ -140,6 +140,8 @@ __global__ void __launch_bounds__(BS)
                    float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, WorkQueue q) {
     __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
     __shared__ uint32_t s_mm[FLAT ? 9 * BS : 1];
+    __shared__ float4 s_stage[FLAT ? 1 : (BS / 64) * 2 * kStageRecs];
+    float4* stage = FLAT ? nullptr : s_stage + (threadIdx.x / 64) * 2 * kStageRecs;
     const Contacts none{};
     FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_particle<BS, PERSIST>(q, n, [&](uint32_t i) {
@@ -147,7 +149,7 @@ __global__ void __launch_bounds__(BS)
             const fg_scene_desc& d = scenes[s];
             if (!scene_stencil_covers_h(d.common)) return refuse_scene_particle(i, id, n, vout);
             forces_particle<true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, d.mass, d.viscosity, d.common,
-                                  d.kernels, d.keyBase, masks, none, FLAT ? &fr : nullptr);
+                                  d.kernels, d.keyBase, masks, none, FLAT ? &fr : nullptr, nullptr, stage);
         });
     });
 }
@@ -167,6 +169,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5)))
                            const BatchBodyScene* __restrict__ bodies, WorkQueue q) {
     __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
     __shared__ uint32_t s_mm[FLAT ? 9 * BS : 1];
+    __shared__ float4 s_stage[FLAT ? 1 : (BS / 64) * 2 * kStageRecs];
+    float4* stage = FLAT ? nullptr : s_stage + (threadIdx.x / 64) * 2 * kStageRecs;
     FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_particle<BS, PERSIST>(q, n, [&](uint32_t i) {
         for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
@@ -176,7 +180,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5)))
             c.dyn = bodies[s].dyn;
             c.coupled = c.dyn.enabled;
             forces_particle<true, true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, d.mass, d.viscosity,
-                                        d.common, d.kernels, d.keyBase, masks, c, FLAT ? &fr : nullptr);
+                                        d.common, d.kernels, d.keyBase, masks, c, FLAT ? &fr : nullptr, nullptr,
+                                        stage);
         });
     });
 }
@@ -196,6 +201,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7)))
                             const BatchBodyScene* __restrict__ bodies, DeferredContact dc, WorkQueue q) {
     __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
     __shared__ uint32_t s_mm[FLAT ? 9 * BS : 1];
+    __shared__ float4 s_stage[FLAT ? 1 : (BS / 64) * 2 * kStageRecs];
+    float4* stage = FLAT ? nullptr : s_stage + (threadIdx.x / 64) * 2 * kStageRecs;
     FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_particle<BS, PERSIST>(q, n, [&](uint32_t i) {
         for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
@@ -205,7 +212,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7)))
             c.dyn = bodies[s].dyn;
             c.coupled = c.dyn.enabled;
             forces_particle<true, true, true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, d.mass, d.viscosity,
-                                              d.common, d.kernels, d.keyBase, masks, c, FLAT ? &fr : nullptr, &dc);
+                                              d.common, d.kernels, d.keyBase, masks, c, FLAT ? &fr : nullptr, &dc,
+                                              stage);
         });
     });
 }

@@ -221,9 +221,12 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8)))
                   float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
                   fg_common_fluid_params cp, fg_kernels_cache kc, const unsigned long long* __restrict__ masks,
                   Contacts bc) {
+    // one LDS slice of kStageRecs neighbour records per wavefront (staged row loop)
+    __shared__ float4 s_stage[MASKS ? (BS / 64) * 2 * kStageRecs : 1];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
-    forces_particle<MASKS, COUPLED>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u, masks, bc);
+    forces_particle<MASKS, COUPLED>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u, masks, bc,
+                                    nullptr, nullptr, MASKS ? s_stage + (threadIdx.x / 64) * 2 * kStageRecs : nullptr);
 }
 
 // Flat schedule (row table in LDS); same results as k_forces_fast<true, COUPLED>.

@@ -433,6 +433,42 @@ __device__ __forceinline__ void pair_term_rp(PairAcc& A, uint32_t i, uint32_t j,
 }
 #pragma clang fp contract(fast)
 
+// The neighbours of one row from its mask mm (self bit cleared), bit k
+// being record src[first + k], in ascending order, two per iteration.  The
+// mask is walked as two 32-bit words, low word first (the same ascending
+// order): 32-bit bit scans and clears, and rows of at most 32 candidates
+// never enter the second loop.
+__device__ __forceinline__ void walk_row_mask(PairAcc& A, unsigned long long mm, uint32_t first,
+                                              const float4* __restrict__ src, const float4& pos, const float4& vel,
+                                              float pri, float h, float s) {
+    uint32_t word = (uint32_t)mm, base = first;
+    for (int half = 0; half < 2; half++) {
+        while (word) {
+            const uint32_t k1 = (uint32_t)__builtin_ctz(word);
+            word &= word - 1u;
+            const bool two = word != 0u;
+            const uint32_t k2 = two ? (uint32_t)__builtin_ctz(word) : k1;
+            word &= word - 1u;  // (0 stays 0)
+            const uint32_t j1 = base + k1, j2 = base + k2;
+            const float4 a1 = src[2 * (size_t)j1], b1 = src[2 * (size_t)j1 + 1];
+            const float4 a2 = src[2 * (size_t)j2], b2 = src[2 * (size_t)j2 + 1];
+            const float3 q1 = make_float3(a1.x, a1.y, a1.z), q2 = make_float3(a2.x, a2.y, a2.z);
+            const float3 v1 = make_float3(b1.x, b1.y, b1.z), v2 = make_float3(b2.x, b2.y, b2.z);
+            const float2 d1 = make_float2(a1.w, b1.w), d2 = make_float2(a2.w, b2.w);
+            // Keeps the second pair's loads beside the first pair's (one
+            // memory round trip per iteration): without a use here the
+            // compiler sinks them into the `two` branch below.
+            asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(v2.x), "v"(v2.y), "v"(v2.z), "v"(d2.x),
+                         "v"(d2.y));
+            pair_term_masked(A, pos, vel, pri, q1, v1, d1.x, d1.y, h, s);
+            if (two) pair_term_masked(A, pos, vel, pri, q2, v2, d2.x, d2.y, h, s);
+        }
+        word = (uint32_t)(mm >> 32);  // then the high word
+        base = first + 32u;
+        if (!word) break;
+    }
+}
+
 // The row loop of the force pass: the 9 stencil rows in turn, each row's
 // neighbours from its mask (ascending, two per iteration) or, without a mask
 // or for a row longer than 64 candidates, by re-testing every candidate.
@@ -465,39 +501,84 @@ __device__ __forceinline__ void forces_rowloop(PairAcc& A, uint32_t i, const flo
             unsigned long long mm = mrow;
             // the particle itself is in its own row's mask; it adds nothing
             if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
-            // The mask as two 32-bit words, low word first (the same
-            // ascending order): 32-bit bit scans and clears, and rows of
-            // at most 32 candidates never enter the second loop.
-            uint32_t word = (uint32_t)mm, base = rb;
-            for (int half = 0; half < 2; half++) {
-                while (word) {
-                    const uint32_t k1 = (uint32_t)__builtin_ctz(word);
-                    word &= word - 1u;
-                    const bool two = word != 0u;
-                    const uint32_t k2 = two ? (uint32_t)__builtin_ctz(word) : k1;
-                    word &= word - 1u;  // (0 stays 0)
-                    const uint32_t j1 = base + k1, j2 = base + k2;
-                    const float4 a1 = rec[2 * (size_t)j1], b1 = rec[2 * (size_t)j1 + 1];
-                    const float4 a2 = rec[2 * (size_t)j2], b2 = rec[2 * (size_t)j2 + 1];
-                    const float3 q1 = make_float3(a1.x, a1.y, a1.z), q2 = make_float3(a2.x, a2.y, a2.z);
-                    const float3 v1 = make_float3(b1.x, b1.y, b1.z), v2 = make_float3(b2.x, b2.y, b2.z);
-                    const float2 d1 = make_float2(a1.w, b1.w), d2 = make_float2(a2.w, b2.w);
-                    // Keeps the second pair's loads beside the first pair's (one
-                    // memory round trip per iteration): without a use here the
-                    // compiler sinks them into the `two` branch below.
-                    asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(v2.x), "v"(v2.y), "v"(v2.z),
-                                 "v"(d2.x), "v"(d2.y));
-                    pair_term_masked(A, pos, vel, dpi.y, q1, v1, d1.x, d1.y, h, s);
-                    if (two) pair_term_masked(A, pos, vel, dpi.y, q2, v2, d2.x, d2.y, h, s);
-                }
-                word = (uint32_t)(mm >> 32);  // then the high word
-                base = rb + 32u;
-                if (!word) break;
-            }
+            walk_row_mask(A, mm, rb, rec, pos, vel, dpi.y, h, s);
         } else {
             for (uint32_t j = rb; j < re; j++)
                 pair_term_rp(A, i, j, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), rho_p[j], h, s, hhu);
         }
+    }
+}
+
+// The masked row loop with each row's neighbour records staged in LDS, for
+// a whole wavefront (every lane present; a lane of an inactive particle
+// takes part with no rows).  The lanes of a wavefront are consecutive
+// sorted particles, so for each of the 9 rows the union of their ranges is
+// one short run of sorted indices [lo, hi): the wavefront loads those
+// records once, coalesced, into its LDS slice `stage` (kStageRecs records)
+// and each lane reads its neighbours from there instead of gathering them
+// from memory, where lanes of a row touch the same records many times.  A
+// row whose run does not fit is read from memory.  Same pairs, same order,
+// same sums as forces_rowloop<true>.
+constexpr uint32_t kStageRecs = 128;
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool active, const float4& pos,
+                                                   const float4& vel, float2 dpi, const float4* __restrict__ sp,
+                                                   const float4* __restrict__ sv, const float2* __restrict__ rho_p,
+                                                   const uint32_t* __restrict__ start,
+                                                   const uint32_t* __restrict__ end, uint32_t n,
+                                                   const fg_common_fluid_params& cp, uint32_t key_base,
+                                                   const unsigned long long* __restrict__ masks,
+                                                   float4* __restrict__ stage) {
+    const float h = cp.kernel_radius, hh = h * h;
+    const float s = cp.simulation_scale;
+    const float hu = h / s, hhu = hu * hu;
+    const float cut = hh * 1.0001f;
+    const RowWalker w = make_rows(pos, cp);
+    const float4* rec = nbr_records(masks, n);
+    for (int row = 0; row < 9; row++) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        unsigned long long mrow = 0ull;
+        uint32_t rb = 0, re = 0;
+        bool has = false;
+        if (active) {
+            mrow = masks[(size_t)row * n + i];
+            asm volatile("" ::"v"(mrow));
+            has = row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re);
+        }
+        // the wavefront's run of this row: min of the starts, max of the ends
+        uint32_t lo = has ? rb : 0xFFFFFFFFu, hi = has ? re : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+            hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+        }
+        const bool staged = hi > lo && hi - lo <= kStageRecs;  // the same on every lane
+        if (staged) {
+            for (uint32_t k = __lane_id(); k < hi - lo; k += 64u) {
+                stage[2 * k] = rec[2 * (size_t)(lo + k)];
+                stage[2 * k + 1] = rec[2 * (size_t)(lo + k) + 1];
+            }
+            wave_lds_sync();
+        }
+        if (has) {
+            if (re - rb <= kRowMaskBits) {
+                unsigned long long mm = mrow;
+                // the particle itself is in its own row's mask; it adds nothing
+                if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
+                if (staged)
+                    walk_row_mask(A, mm, rb - lo, stage, pos, vel, dpi.y, h, s);
+                else
+                    walk_row_mask(A, mm, rb, rec, pos, vel, dpi.y, h, s);
+            } else {
+                for (uint32_t j = rb; j < re; j++)
+                    pair_term_rp(A, i, j, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), rho_p[j], h, s, hhu);
+            }
+        }
+        if (staged) wave_lds_sync();  // every lane's reads of this row before the next row's stores
     }
 }
 
@@ -521,7 +602,9 @@ struct FlatRows {
 // Two schedules give bit-identical sums (rows in order, neighbours of a row
 // in ascending sorted order):
 //  * row loop: the 9 rows in turn; a wavefront waits at every row for its
-//    lane with the most neighbours in that row;
+//    lane with the most neighbours in that row.  Given an LDS slice
+//    (`stage`) and a full wavefront, each row's records are staged there
+//    first (forces_rows_staged);
 //  * flat (MASKS with a row table): the rows with neighbours are gathered
 //    first, compacted in row order, then each lane walks its own neighbour
 //    stream across them, two neighbours per iteration with their loads
@@ -539,19 +622,28 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
                                                 const fg_common_fluid_params& cp, const fg_kernels_cache& kc,
                                                 uint32_t key_base, const unsigned long long* __restrict__ masks,
                                                 const Contacts& bc, FlatRows* flat = nullptr,
-                                                const DeferredContact* defer = nullptr) {
+                                                const DeferredContact* defer = nullptr,
+                                                float4* __restrict__ stage = nullptr) {
     const float4 pos = sp[i];
     const float4 vel = sv[i];
+    const bool active = pos.w > 0.0f;
     F3 a = f3(0.0f, 0.0f, 0.0f);
-    if (pos.w > 0.0f) {
+    PairAcc A{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // The staged row loop needs every lane of the wavefront (the run of
+    // each row is a wavefront reduction); otherwise the rows are read from
+    // memory below.
+    const bool staged_rows = MASKS && stage != nullptr && flat == nullptr && __ballot(1) == ~0ull;
+    if (staged_rows)
+        forces_rows_staged(A, i, active, pos, vel, active ? rho_p[i] : make_float2(1.0f, 0.0f), sp, sv, rho_p, start,
+                           end, n, cp, key_base, masks, stage);
+    if (active) {
         const float h = cp.kernel_radius, hh = h * h;
         const float s = cp.simulation_scale;
         const float cut = hh * 1.0001f;
         const float2 dpi = rho_p[i];
         const RowWalker w = make_rows(pos, cp);
-        PairAcc A{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         const float4* rec = MASKS ? nbr_records(masks, n) : nullptr;
-        bool rowloop = true;
+        bool rowloop = !staged_rows;
         if (MASKS && flat) {
             // The rows with neighbours, compacted in row order: (first sorted
             // index, mask without the particle itself).
