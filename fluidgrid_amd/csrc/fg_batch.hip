@@ -117,6 +117,8 @@ __global__ void __launch_bounds__(BS)
                     unsigned long long* __restrict__ masks, uint32_t n, WorkQueue q) {
     __shared__ uint32_t s_rb[FLAT ? 9 * BS : 1];
     __shared__ uint32_t s_re[FLAT ? 9 * BS : 1];
+    __shared__ float4 s_stage[FLAT ? 1 : (BS / 64) * kStagePos];
+    float4* stage = FLAT ? nullptr : s_stage + (threadIdx.x / 64) * kStagePos;
     FlatRanges fr{s_rb, s_re, (uint32_t)BS, (uint32_t)threadIdx.x};
     for_each_particle<BS, PERSIST>(q, n, [&](uint32_t i) {
         for_each_scene(scene_of[i], nscenes, [&](uint32_t s) {
@@ -126,7 +128,7 @@ __global__ void __launch_bounds__(BS)
                 return;
             }
             density_particle<true>(i, sp, sv, rho_p, start, end, n, d.gasConstant, d.restDensity, d.mass, d.common,
-                                   d.kernels, d.keyBase, masks, FLAT ? &fr : nullptr);
+                                   d.kernels, d.keyBase, masks, FLAT ? &fr : nullptr, stage);
         });
     });
 }

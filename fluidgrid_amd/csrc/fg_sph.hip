@@ -190,9 +190,12 @@ __global__ void __launch_bounds__(BS)
                    const uint32_t* __restrict__ start,
                    const uint32_t* __restrict__ end, uint32_t n, float k, float rho0, float m,
                    fg_common_fluid_params cp, fg_kernels_cache kc, unsigned long long* __restrict__ masks) {
+    // one LDS slice of kStagePos positions per wavefront (staged row loop)
+    __shared__ float4 s_stage[(BS / 64) * kStagePos];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
-    density_particle<MASKS>(i, sp, sv, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks);
+    density_particle<MASKS>(i, sp, sv, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks, nullptr,
+                            s_stage + (threadIdx.x / 64) * kStagePos);
 }
 
 // Flat schedule (row table in LDS); same results as k_density_fast<MASKS>.

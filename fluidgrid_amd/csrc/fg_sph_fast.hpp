@@ -213,6 +213,40 @@ __device__ __forceinline__ float3 ld_xyz(const float4* p) {
     return make_float3(f[0], f[1], f[2]);
 }
 
+// LDS staging of neighbour rows, per wavefront.  The lanes of a wavefront
+// are consecutive sorted particles, so for each of the 9 stencil rows the
+// union of their candidate ranges is one short run of sorted indices
+// [lo, hi): the wavefront loads it once, coalesced, into its own LDS slice
+// and the lanes read their candidates from there instead of gathering them
+// from memory, where the lanes of a row touch the same particles many
+// times.  A row whose run does not fit its slice is read from memory.  The
+// density pass stages positions (kStagePos float4 per wavefront), the force
+// pass neighbour records (kStageRecs records of two float4).
+constexpr uint32_t kStagePos = 256;
+constexpr uint32_t kStageRecs = 128;
+
+// Orders a wavefront's LDS stores before its other lanes' loads (and loads
+// before the next stores): the lanes of one wavefront need no workgroup
+// barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The run [lo, hi) of one row over the wavefront: the minimum of the lanes'
+// starts and the maximum of their ends (lanes without the row give nothing).
+// Every lane of the wavefront must call it.
+__device__ __forceinline__ void wave_run(bool has, uint32_t rb, uint32_t re, uint32_t& lo, uint32_t& hi) {
+    lo = has ? rb : 0xFFFFFFFFu;
+    hi = has ? re : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
+    }
+}
+
 // Poly6 sum over four candidates; bit u of the returned nibble is set when
 // candidate u lies within h.
 //
@@ -255,6 +289,92 @@ __device__ __forceinline__ void store_density(uint32_t i, float rho, float k, fl
 }
 #pragma clang fp contract(fast)
 
+// xyz of a position staged in LDS (a 16-byte read; w unused).
+__device__ __forceinline__ float3 ld_xyz_lds(const float4* p) {
+    const float4 q = *p;
+    return make_float3(q.x, q.y, q.z);
+}
+
+// One stencil row of the density pass: the len candidates src[0, len) in
+// groups of four in sorted order (a short last group padded with far
+// points) added to acc, and the row's mask (0 for rows longer than 64
+// candidates, which are never read through their mask).  src is the sorted
+// positions from the row's first particle, in memory or (LDS) staged.
+template <bool MASKS, bool LDS>
+__device__ __forceinline__ unsigned long long density_row(const float4& p, const float4* __restrict__ pp,
+                                                          uint32_t len, float h2u, float hhu, float& acc) {
+    auto ld = [](const float4* q) { return LDS ? ld_xyz_lds(q) : ld_xyz(q); };
+    unsigned long long msk = 0ull;
+    const uint32_t n4 = len & ~3u;
+    uint32_t kk = 0;
+    if (n4) {
+        // Software-pipelined groups of four: the next group's loads are
+        // in flight while the current group is evaluated.
+        float3 a0 = ld(pp), a1 = ld(pp + 1), a2 = ld(pp + 2), a3 = ld(pp + 3);
+        for (; kk + 4 < n4; kk += 4) {
+            pp += 4;
+            const float3 b0 = ld(pp), b1 = ld(pp + 1), b2 = ld(pp + 2), b3 = ld(pp + 3);
+            const uint32_t nib = density4(p, a0, a1, a2, a3, h2u, hhu, acc);
+            if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
+            a0 = b0; a1 = b1; a2 = b2; a3 = b3;
+        }
+        const uint32_t nib = density4(p, a0, a1, a2, a3, h2u, hhu, acc);
+        if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
+        kk += 4;
+        pp += 4;
+    }
+    if (kk < len) {  // 1-3 remaining candidates; the others are masked out
+        const uint32_t rest = len - kk;
+        const float3 far = make_float3(3.0e38f, 3.0e38f, 3.0e38f);
+        const float3 t0 = ld(pp);
+        const float3 t1 = rest > 1 ? ld(pp + 1) : far;
+        const float3 t2 = rest > 2 ? ld(pp + 2) : far;
+        const uint32_t nib = density4(p, t0, t1, t2, far, h2u, hhu, acc);
+        if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
+    }
+    return len <= kRowMaskBits ? msk : 0ull;
+}
+
+// The row loop of the density pass with each row's candidate positions
+// staged in the wavefront's LDS slice `stage` (kStagePos positions), for a
+// whole wavefront (every lane present; a lane of an inactive particle takes
+// part with no rows and writes no masks).  Same candidates, same order,
+// same sums and masks as the row loop of density_particle.
+template <bool MASKS>
+__device__ __forceinline__ float density_rows_staged(uint32_t i, bool active, const float4& p,
+                                                     const float4* __restrict__ sp,
+                                                     const uint32_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ end, uint32_t n,
+                                                     const fg_common_fluid_params& cp, uint32_t key_base,
+                                                     unsigned long long* __restrict__ masks,
+                                                     float4* __restrict__ stage) {
+    const float s = cp.simulation_scale, s2 = s * s;
+    const float h = cp.kernel_radius, hh = h * h;
+    const float hu = h / s, hhu = hu * hu, h2u = cp.kernel_radius_squared / s2;
+    const float cut = hh * 1.0001f;
+    const RowWalker w = make_rows(p, cp);
+    float acc = 0.0f;
+    for (int row = 0; row < 9; row++) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        uint32_t rb = 0, re = 0;
+        const bool has = active && row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re);
+        uint32_t lo, hi;
+        wave_run(has, rb, re, lo, hi);
+        const bool staged = hi > lo && hi - lo <= kStagePos;  // the same on every lane
+        if (staged) {
+            for (uint32_t k = __lane_id(); k < hi - lo; k += 64u) stage[k] = sp[lo + k];
+            wave_lds_sync();
+        }
+        unsigned long long msk = 0ull;
+        if (has)
+            msk = staged ? density_row<MASKS, true>(p, stage + (rb - lo), re - rb, h2u, hhu, acc)
+                         : density_row<MASKS, false>(p, sp + rb, re - rb, h2u, hhu, acc);
+        if (MASKS && active) masks[(size_t)row * n + i] = msk;
+        if (staged) wave_lds_sync();  // every lane's reads of this row before the next row's stores
+    }
+    return acc;
+}
+
 // Per-lane row table of the flat density schedule, in LDS: first and
 // one-past-last sorted index of each of the 9 rows (equal for a culled row).
 struct FlatRanges {
@@ -273,7 +393,9 @@ struct FlatRanges {
 // in groups of four in sorted order, a short last group padded with far
 // points): the row loop, where a wavefront waits at every row for its lane
 // with the longest row, and the flat schedule (with a row table), where each
-// lane walks its own candidate stream across rows.
+// lane walks its own candidate stream across rows.  Given an LDS slice
+// (`stage`) and a full wavefront, the row loop reads each row's candidates
+// from LDS (density_rows_staged).
 template <bool MASKS>
 __device__ __forceinline__ void density_particle(uint32_t i, const float4* __restrict__ sp,
                                                  const float4* __restrict__ sv, float2* __restrict__ rho_p, const uint32_t* __restrict__ start,
@@ -281,10 +403,23 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
                                                  float m, const fg_common_fluid_params& cp,
                                                  const fg_kernels_cache& kc, uint32_t key_base,
                                                  unsigned long long* __restrict__ masks,
-                                                 FlatRanges* flat = nullptr) {
+                                                 FlatRanges* flat = nullptr, float4* __restrict__ stage = nullptr) {
     const float4 p = sp[i];
     float4* rec = MASKS ? nbr_records(masks, n) : nullptr;
     const float4 v = MASKS ? sv[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // The staged row loop needs every lane of the wavefront (the run of each
+    // row is a wavefront reduction); otherwise the rows are read from memory.
+    if (stage != nullptr && flat == nullptr && __ballot(1) == ~0ull) {
+        const bool active = p.w > 0.0f;
+        const float acc = density_rows_staged<MASKS>(i, active, p, sp, start, end, n, cp, key_base, masks, stage);
+        if (active) {
+            const float s2 = cp.simulation_scale * cp.simulation_scale;
+            store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * acc, k, rho0, rho_p, rec, p, v);
+        } else if (MASKS) {
+            store_record(i, rho_p[i], p, v, rec);
+        }
+        return;
+    }
     if (!(p.w > 0.0f)) {
         // (rho, p) is left as it is; neighbours still read it
         if (MASKS) store_record(i, rho_p[i], p, v, rec);
@@ -340,40 +475,8 @@ __device__ __forceinline__ void density_particle(uint32_t i, const float4* __res
         const int dz = row / 3 - 1, dy = row % 3 - 1;
         unsigned long long msk = 0ull;
         uint32_t rb, re;
-        if (row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re)) {
-            const uint32_t len = re - rb, n4 = len & ~3u;
-            const bool encoded = len <= kRowMaskBits;
-            const float4* pp = sp + rb;
-            uint32_t kk = 0;
-            if (n4) {
-                // Software-pipelined groups of four: the next group's loads are
-                // in flight while the current group is evaluated.
-                float3 a0 = ld_xyz(pp), a1 = ld_xyz(pp + 1), a2 = ld_xyz(pp + 2), a3 = ld_xyz(pp + 3);
-                for (; kk + 4 < n4; kk += 4) {
-                    pp += 4;
-                    const float3 b0 = ld_xyz(pp), b1 = ld_xyz(pp + 1), b2 = ld_xyz(pp + 2), b3 = ld_xyz(pp + 3);
-                    const uint32_t nib = density4(p, a0, a1, a2, a3, h2u, hhu, acc);
-                    if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
-                    a0 = b0; a1 = b1; a2 = b2; a3 = b3;
-                }
-                const uint32_t nib = density4(p, a0, a1, a2, a3, h2u, hhu, acc);
-                if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
-                kk += 4;
-                pp += 4;
-            }
-            if (kk < len) {  // 1-3 remaining candidates; the others are masked out
-                const uint32_t rest = len - kk;
-                const float3 far = make_float3(3.0e38f, 3.0e38f, 3.0e38f);
-                const float3 t0 = ld_xyz(pp);
-                const float3 t1 = rest > 1 ? ld_xyz(pp + 1) : far;
-                const float3 t2 = rest > 2 ? ld_xyz(pp + 2) : far;
-                const uint32_t nib = density4(p, t0, t1, t2, far, h2u, hhu, acc);
-                if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
-            }
-            if (!encoded) msk = 0ull;
-        }
-        // Rows longer than 64 particles are never read through their mask
-        // (it is written as 0).
+        if (row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re))
+            msk = density_row<MASKS, false>(p, sp + rb, re - rb, h2u, hhu, acc);
         if (MASKS) masks[(size_t)row * n + i] = msk;
     }
     store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * acc, k, rho0, rho_p, rec, p, v);
@@ -509,22 +612,10 @@ __device__ __forceinline__ void forces_rowloop(PairAcc& A, uint32_t i, const flo
     }
 }
 
-// The masked row loop with each row's neighbour records staged in LDS, for
-// a whole wavefront (every lane present; a lane of an inactive particle
-// takes part with no rows).  The lanes of a wavefront are consecutive
-// sorted particles, so for each of the 9 rows the union of their ranges is
-// one short run of sorted indices [lo, hi): the wavefront loads those
-// records once, coalesced, into its LDS slice `stage` (kStageRecs records)
-// and each lane reads its neighbours from there instead of gathering them
-// from memory, where lanes of a row touch the same records many times.  A
-// row whose run does not fit is read from memory.  Same pairs, same order,
-// same sums as forces_rowloop<true>.
-constexpr uint32_t kStageRecs = 128;
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// The masked row loop with each row's neighbour records staged in the
+// wavefront's LDS slice `stage` (kStageRecs records), for a whole wavefront
+// (every lane present; a lane of an inactive particle takes part with no
+// rows).  Same pairs, same order, same sums as forces_rowloop<true>.
 __device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool active, const float4& pos,
                                                    const float4& vel, float2 dpi, const float4* __restrict__ sp,
                                                    const float4* __restrict__ sv, const float2* __restrict__ rho_p,
@@ -549,13 +640,8 @@ __device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool 
             asm volatile("" ::"v"(mrow));
             has = row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re);
         }
-        // the wavefront's run of this row: min of the starts, max of the ends
-        uint32_t lo = has ? rb : 0xFFFFFFFFu, hi = has ? re : 0u;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
-            hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
-        }
+        uint32_t lo, hi;
+        wave_run(has, rb, re, lo, hi);
         const bool staged = hi > lo && hi - lo <= kStageRecs;  // the same on every lane
         if (staged) {
             for (uint32_t k = __lane_id(); k < hi - lo; k += 64u) {
