@@ -234,10 +234,30 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The run [lo, hi) of one row over the wavefront: the minimum of the lanes'
-// starts and the maximum of their ends (lanes without the row give nothing).
-// Every lane of the wavefront must call it.
-__device__ __forceinline__ void wave_run(bool has, uint32_t rb, uint32_t re, uint32_t& lo, uint32_t& hi) {
+// Whether one row is staged for the wavefront, and its run [lo, hi).  The
+// sorted ranges of consecutive lanes are ordered (a later lane's cell has a
+// key no lower), so the run is the start of the first lane with the row and
+// the end of the last one: two lane reads, where a min/max reduction over 64
+// lanes is six dependent cross-lane steps.  A lane whose range falls outside
+// that run sends the whole row to memory, so the result never depends on
+// the ordering.  Every lane of the wavefront must call it.
+__device__ __forceinline__ bool wave_run(bool has, uint32_t rb, uint32_t re, uint32_t cap, uint32_t& lo,
+                                         uint32_t& hi) {
+    const unsigned long long hm = __ballot(has);
+    lo = hi = 0u;
+    if (hm == 0ull) return false;
+    lo = (uint32_t)__builtin_amdgcn_readlane((int)rb, __builtin_ctzll(hm));
+    hi = (uint32_t)__builtin_amdgcn_readlane((int)re, 63 - __builtin_clzll(hm));
+    const bool inside = !has || (rb >= lo && re <= hi);
+    return __ballot(!inside) == 0ull && hi - lo <= cap;
+}
+
+// The same run as a min/max reduction over the 64 lanes (six dependent
+// cross-lane steps, no lane order assumed).  The force pass uses it: at its
+// 64-register budget the two lane reads above cost a register spill that is
+// reloaded at every row.
+__device__ __forceinline__ bool wave_run_minmax(bool has, uint32_t rb, uint32_t re, uint32_t cap, uint32_t& lo,
+                                                uint32_t& hi) {
     lo = has ? rb : 0xFFFFFFFFu;
     hi = has ? re : 0u;
 #pragma unroll
@@ -245,6 +265,7 @@ __device__ __forceinline__ void wave_run(bool has, uint32_t rb, uint32_t re, uin
         lo = min(lo, (uint32_t)__shfl_xor((int)lo, o));
         hi = max(hi, (uint32_t)__shfl_xor((int)hi, o));
     }
+    return hi > lo && hi - lo <= cap;
 }
 
 // Poly6 sum over four candidates; bit u of the returned nibble is set when
@@ -359,8 +380,7 @@ __device__ __forceinline__ float density_rows_staged(uint32_t i, bool active, co
         uint32_t rb = 0, re = 0;
         const bool has = active && row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re);
         uint32_t lo, hi;
-        wave_run(has, rb, re, lo, hi);
-        const bool staged = hi > lo && hi - lo <= kStagePos;  // the same on every lane
+        const bool staged = wave_run(has, rb, re, kStagePos, lo, hi);  // the same on every lane
         if (staged) {
             for (uint32_t k = __lane_id(); k < hi - lo; k += 64u) stage[k] = sp[lo + k];
             wave_lds_sync();
@@ -641,8 +661,7 @@ __device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool 
             has = row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re);
         }
         uint32_t lo, hi;
-        wave_run(has, rb, re, lo, hi);
-        const bool staged = hi > lo && hi - lo <= kStageRecs;  // the same on every lane
+        const bool staged = wave_run_minmax(has, rb, re, kStageRecs, lo, hi);  // the same on every lane
         if (staged) {
             for (uint32_t k = __lane_id(); k < hi - lo; k += 64u) {
                 stage[2 * k] = rec[2 * (size_t)(lo + k)];
