@@ -328,7 +328,14 @@ __device__ __forceinline__ unsigned long long density_row(const float4& p, const
     unsigned long long msk = 0ull;
     const uint32_t n4 = len & ~3u;
     uint32_t kk = 0;
-    if (n4) {
+    if (LDS) {
+        // From LDS the loads return quickly: no pipelining, whose register
+        // copies cost a quarter of the loop's instructions.
+        for (; kk < n4; kk += 4, pp += 4) {
+            const uint32_t nib = density4(p, ld(pp), ld(pp + 1), ld(pp + 2), ld(pp + 3), h2u, hhu, acc);
+            if (MASKS) msk |= (unsigned long long)nib << (kk & 63u);
+        }
+    } else if (n4) {
         // Software-pipelined groups of four: the next group's loads are
         // in flight while the current group is evaluated.
         float3 a0 = ld(pp), a1 = ld(pp + 1), a2 = ld(pp + 2), a3 = ld(pp + 3);
@@ -557,13 +564,17 @@ __device__ __forceinline__ void pair_term_rp(PairAcc& A, uint32_t i, uint32_t j,
 #pragma clang fp contract(fast)
 
 // The neighbours of one row from its mask mm (self bit cleared), bit k
-// being record src[first + k], in ascending order, two per iteration.  The
-// mask is walked as two 32-bit words, low word first (the same ascending
-// order): 32-bit bit scans and clears, and rows of at most 32 candidates
-// never enter the second loop.
+// being record first + k, in ascending order, two per iteration.  Record j
+// is (pa[STRIDE j], pb[STRIDE j]): the interleaved records in memory
+// (STRIDE 2, pb = pa + 1) or the two halves staged apart in LDS (STRIDE 1),
+// where 16-byte reads of consecutive records then fall in distinct banks.
+// The mask is walked as two 32-bit words, low word first (the same
+// ascending order): 32-bit bit scans and clears, and rows of at most 32
+// candidates never enter the second loop.
+template <int STRIDE>
 __device__ __forceinline__ void walk_row_mask(PairAcc& A, unsigned long long mm, uint32_t first,
-                                              const float4* __restrict__ src, const float4& pos, const float4& vel,
-                                              float pri, float h, float s) {
+                                              const float4* __restrict__ pa, const float4* __restrict__ pb,
+                                              const float4& pos, const float4& vel, float pri, float h, float s) {
     uint32_t word = (uint32_t)mm, base = first;
     for (int half = 0; half < 2; half++) {
         while (word) {
@@ -573,8 +584,8 @@ __device__ __forceinline__ void walk_row_mask(PairAcc& A, unsigned long long mm,
             const uint32_t k2 = two ? (uint32_t)__builtin_ctz(word) : k1;
             word &= word - 1u;  // (0 stays 0)
             const uint32_t j1 = base + k1, j2 = base + k2;
-            const float4 a1 = src[2 * (size_t)j1], b1 = src[2 * (size_t)j1 + 1];
-            const float4 a2 = src[2 * (size_t)j2], b2 = src[2 * (size_t)j2 + 1];
+            const float4 a1 = pa[STRIDE * (size_t)j1], b1 = pb[STRIDE * (size_t)j1];
+            const float4 a2 = pa[STRIDE * (size_t)j2], b2 = pb[STRIDE * (size_t)j2];
             const float3 q1 = make_float3(a1.x, a1.y, a1.z), q2 = make_float3(a2.x, a2.y, a2.z);
             const float3 v1 = make_float3(b1.x, b1.y, b1.z), v2 = make_float3(b2.x, b2.y, b2.z);
             const float2 d1 = make_float2(a1.w, b1.w), d2 = make_float2(a2.w, b2.w);
@@ -624,7 +635,7 @@ __device__ __forceinline__ void forces_rowloop(PairAcc& A, uint32_t i, const flo
             unsigned long long mm = mrow;
             // the particle itself is in its own row's mask; it adds nothing
             if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
-            walk_row_mask(A, mm, rb, rec, pos, vel, dpi.y, h, s);
+            walk_row_mask<2>(A, mm, rb, rec, rec + 1, pos, vel, dpi.y, h, s);
         } else {
             for (uint32_t j = rb; j < re; j++)
                 pair_term_rp(A, i, j, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), rho_p[j], h, s, hhu);
@@ -633,7 +644,8 @@ __device__ __forceinline__ void forces_rowloop(PairAcc& A, uint32_t i, const flo
 }
 
 // The masked row loop with each row's neighbour records staged in the
-// wavefront's LDS slice `stage` (kStageRecs records), for a whole wavefront
+// wavefront's LDS slice `stage` (kStageRecs records: their first halves,
+// then their second halves), for a whole wavefront
 // (every lane present; a lane of an inactive particle takes part with no
 // rows).  Same pairs, same order, same sums as forces_rowloop<true>.
 __device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool active, const float4& pos,
@@ -664,8 +676,8 @@ __device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool 
         const bool staged = wave_run_minmax(has, rb, re, kStageRecs, lo, hi);  // the same on every lane
         if (staged) {
             for (uint32_t k = __lane_id(); k < hi - lo; k += 64u) {
-                stage[2 * k] = rec[2 * (size_t)(lo + k)];
-                stage[2 * k + 1] = rec[2 * (size_t)(lo + k) + 1];
+                stage[k] = rec[2 * (size_t)(lo + k)];
+                stage[kStageRecs + k] = rec[2 * (size_t)(lo + k) + 1];
             }
             wave_lds_sync();
         }
@@ -675,9 +687,9 @@ __device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool 
                 // the particle itself is in its own row's mask; it adds nothing
                 if (i >= rb && i < re) mm &= ~(1ull << (i - rb));
                 if (staged)
-                    walk_row_mask(A, mm, rb - lo, stage, pos, vel, dpi.y, h, s);
+                    walk_row_mask<1>(A, mm, rb - lo, stage, stage + kStageRecs, pos, vel, dpi.y, h, s);
                 else
-                    walk_row_mask(A, mm, rb, rec, pos, vel, dpi.y, h, s);
+                    walk_row_mask<2>(A, mm, rb, rec, rec + 1, pos, vel, dpi.y, h, s);
             } else {
                 for (uint32_t j = rb; j < re; j++)
                     pair_term_rp(A, i, j, pos, vel, dpi.y, ld_xyz(sp + j), ld_xyz(sv + j), rho_p[j], h, s, hhu);
