@@ -213,12 +213,14 @@ __global__ void __launch_bounds__(BS)
     density_particle<MASKS>(i, sp, sv, rho_p, start, end, n, k, rho0, m, cp, kc, 0u, masks, &fr);
 }
 
-// amdgpu_waves_per_eu(8): the coupled variant carries two body-grid
-// descriptors in scalar registers (106 SGPRs, occupancy 7 without the hint);
-// fitted to 8 waves per SIMD it spills scalars to vector lanes outside the
-// pair loop and runs 5-6% faster with the same results.
+// amdgpu_waves_per_eu(7): the masked kernels load each stencil row's cell
+// bounds and mask a row ahead (forces_rows_staged<true>), which needs about
+// 70 registers: at 8 waves per SIMD that spills to scratch and the pass is
+// 25% slower, at 7 it is 4-7% faster than the row-by-row loads at 8.  The
+// hint also holds the coupled variant, whose two body-grid descriptors take
+// scalar registers (106 SGPRs without it), to 7 waves.
 template <bool MASKS, bool COUPLED, int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7)))
     k_forces_fast(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                   const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, float4* __restrict__ vout,
                   float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, float m, float mu,
@@ -228,8 +230,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8)))
     __shared__ float4 s_stage[MASKS ? (BS / 64) * 2 * kStageRecs : 1];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
-    forces_particle<MASKS, COUPLED>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u, masks, bc,
-                                    nullptr, nullptr, MASKS ? s_stage + (threadIdx.x / 64) * 2 * kStageRecs : nullptr);
+    forces_particle<MASKS, COUPLED, false, true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u,
+                                                 masks, bc, nullptr, nullptr,
+                                                 MASKS ? s_stage + (threadIdx.x / 64) * 2 * kStageRecs : nullptr);
 }
 
 // Flat schedule (row table in LDS); same results as k_forces_fast<true, COUPLED>.

@@ -130,15 +130,22 @@ __device__ __forceinline__ RowWalker make_rows(const float4& p, const fg_common_
     return w;
 }
 
-// Sorted range of row (dy,dz) after culling; returns false when empty.  The
-// grid's buckets start at key_base of the key space (0 for a single fluid;
-// the scene's first bucket in a batch).
-// `cut` is h^2 enlarged by a small margin so that rounding in the cell
-// bounds can never drop a pair with r < h.
-__device__ __forceinline__ bool row_range(const RowWalker& w, int dy, int dz, float cut,
-                                          const fg_common_fluid_params& cp, uint32_t key_base,
-                                          const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                                          uint32_t n, uint32_t& rb, uint32_t& re) {
+// Sorted range of row (dy,dz) after culling, in two halves so that a row's
+// loads can be issued a row ahead (row_fetch) and used later (row_finish).
+// The grid's buckets start at key_base of the key space (0 for a single
+// fluid; the scene's first bucket in a batch).  `cut` is h^2 enlarged by a
+// small margin so that rounding in the cell bounds can never drop a pair
+// with r < h.
+struct RowFetch {
+    uint32_t s0, s1, e1;  // start of the first cell, start and end of the last
+    bool ok;              // the row is not culled (and s0, s1, e1 are loaded)
+};
+
+// The row's cells cx-1..cx+1 after culling by distance, as keys k0..k1;
+// false for a culled row.
+__device__ __forceinline__ bool row_keys(const RowWalker& w, int dy, int dz, float cut,
+                                         const fg_common_fluid_params& cp, uint32_t key_base, uint32_t& k0,
+                                         uint32_t& k1) {
     const int cy = w.cy + dy, cz = w.cz + dz;
     if (cy < 0 || cz < 0 || cy >= cp.bucket_size.y || cz >= cp.bucket_size.z) return false;
     // selects, not an array indexed by the row: the row loops are not always
@@ -150,17 +157,42 @@ __device__ __forceinline__ bool row_range(const RowWalker& w, int dy, int dz, fl
     const bool useL = w.cx > 0 && gyz + w.gx_lo < cut;                      // cell cx-1
     const bool useR = w.cx + 1 < cp.bucket_size.x && gyz + w.gx_hi < cut;  // cell cx+1
     const uint32_t kb = key_base + linear_key(0, cy, cz, cp) + (uint32_t)w.cx;
-    const uint32_t k0 = useL ? kb - 1u : kb, k1 = useR ? kb + 1u : kb;
-    // The row's particles are sorted[start[k0], end[k1]) when its end cells
-    // are occupied, as in the bulk of the fluid: three loads in one round
-    // trip.  Otherwise the first occupied cell from the left and the last
-    // from the right bound it (end[] of an empty cell is stale and unused).
-    const uint32_t s0 = start[k0], s1 = start[k1], e1 = end[k1];
-    if (s0 != kEmpty && s1 != kEmpty) {
-        rb = s0;
-        re = min(e1, n);
+    k0 = useL ? kb - 1u : kb;
+    k1 = useR ? kb + 1u : kb;
+    return true;
+}
+
+// The row's particles are sorted[start[k0], end[k1]) when its end cells are
+// occupied, as in the bulk of the fluid: three loads in one round trip.
+__device__ __forceinline__ RowFetch row_fetch(const RowWalker& w, int dy, int dz, float cut,
+                                              const fg_common_fluid_params& cp, uint32_t key_base,
+                                              const uint32_t* __restrict__ start, const uint32_t* __restrict__ end) {
+    RowFetch f{kEmpty, kEmpty, 0u, false};
+    uint32_t k0, k1;
+    if (row_keys(w, dy, dz, cut, cp, key_base, k0, k1)) {
+        f.s0 = start[k0];
+        f.s1 = start[k1];
+        f.e1 = end[k1];
+        f.ok = true;
+    }
+    return f;
+}
+
+// The range from the fetched loads; when an end cell is empty, the first
+// occupied cell from the left and the last from the right bound it (end[]
+// of an empty cell is stale and unused).  Returns false when empty.
+__device__ __forceinline__ bool row_finish(const RowFetch& f, const RowWalker& w, int dy, int dz, float cut,
+                                           const fg_common_fluid_params& cp, uint32_t key_base,
+                                           const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                           uint32_t n, uint32_t& rb, uint32_t& re) {
+    if (!f.ok) return false;
+    if (f.s0 != kEmpty && f.s1 != kEmpty) {
+        rb = f.s0;
+        re = min(f.e1, n);
         return rb < re;
     }
+    uint32_t k0 = 0, k1 = 0;
+    row_keys(w, dy, dz, cut, cp, key_base, k0, k1);
     uint32_t b = kEmpty, e = 0;
     for (uint32_t k = k0; k <= k1; k++) {
         const uint32_t sk = start[k];
@@ -173,6 +205,14 @@ __device__ __forceinline__ bool row_range(const RowWalker& w, int dy, int dz, fl
     rb = b;
     re = min(e, n);
     return rb < re;
+}
+
+__device__ __forceinline__ bool row_range(const RowWalker& w, int dy, int dz, float cut,
+                                          const fg_common_fluid_params& cp, uint32_t key_base,
+                                          const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                          uint32_t n, uint32_t& rb, uint32_t& re) {
+    return row_finish(row_fetch(w, dy, dz, cut, cp, key_base, start, end), w, dy, dz, cut, cp, key_base, start,
+                      end, n, rb, re);
 }
 
 // Blocks are handed to the eight XCDs round-robin; remapping them so that each
@@ -382,16 +422,20 @@ __device__ __forceinline__ float density_rows_staged(uint32_t i, bool active, co
     const float cut = hh * 1.0001f;
     const RowWalker w = make_rows(p, cp);
     float acc = 0.0f;
+    // The next row's cell bounds are loaded a row ahead, after this row's
+    // staging loads, so they arrive while this row is walked.
+    RowFetch next = active ? row_fetch(w, -1, -1, cut, cp, key_base, start, end) : RowFetch{kEmpty, kEmpty, 0u, false};
     for (int row = 0; row < 9; row++) {
         const int dz = row / 3 - 1, dy = row % 3 - 1;
+        const RowFetch f = next;
         uint32_t rb = 0, re = 0;
-        const bool has = active && row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re);
+        const bool has = active && row_finish(f, w, dy, dz, cut, cp, key_base, start, end, n, rb, re);
         uint32_t lo, hi;
         const bool staged = wave_run(has, rb, re, kStagePos, lo, hi);  // the same on every lane
-        if (staged) {
+        if (staged)
             for (uint32_t k = __lane_id(); k < hi - lo; k += 64u) stage[k] = sp[lo + k];
-            wave_lds_sync();
-        }
+        if (row < 8 && active) next = row_fetch(w, (row + 1) % 3 - 1, (row + 1) / 3 - 1, cut, cp, key_base, start, end);
+        if (staged) wave_lds_sync();
         unsigned long long msk = 0ull;
         if (has)
             msk = staged ? density_row<MASKS, true>(p, stage + (rb - lo), re - rb, h2u, hhu, acc)
@@ -648,6 +692,10 @@ __device__ __forceinline__ void forces_rowloop(PairAcc& A, uint32_t i, const flo
 // then their second halves), for a whole wavefront
 // (every lane present; a lane of an inactive particle takes part with no
 // rows).  Same pairs, same order, same sums as forces_rowloop<true>.
+// AHEAD: the next row's cell bounds and mask are loaded a row ahead, after
+// this row's staging loads, so they arrive while this row is walked (about
+// five more registers; the single-fluid kernel runs it at 7 waves per SIMD).
+template <bool AHEAD>
 __device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool active, const float4& pos,
                                                    const float4& vel, float2 dpi, const float4* __restrict__ sp,
                                                    const float4* __restrict__ sv, const float2* __restrict__ rho_p,
@@ -662,12 +710,21 @@ __device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool 
     const float cut = hh * 1.0001f;
     const RowWalker w = make_rows(pos, cp);
     const float4* rec = nbr_records(masks, n);
+    RowFetch next{kEmpty, kEmpty, 0u, false};
+    unsigned long long next_mask = 0ull;
+    if (AHEAD && active) {
+        next = row_fetch(w, -1, -1, cut, cp, key_base, start, end);
+        next_mask = masks[i];
+    }
     for (int row = 0; row < 9; row++) {
         const int dz = row / 3 - 1, dy = row % 3 - 1;
         unsigned long long mrow = 0ull;
         uint32_t rb = 0, re = 0;
         bool has = false;
-        if (active) {
+        if (AHEAD) {
+            mrow = next_mask;
+            has = active && row_finish(next, w, dy, dz, cut, cp, key_base, start, end, n, rb, re);
+        } else if (active) {
             mrow = masks[(size_t)row * n + i];
             asm volatile("" ::"v"(mrow));
             has = row_range(w, dy, dz, cut, cp, key_base, start, end, n, rb, re);
@@ -679,8 +736,12 @@ __device__ __forceinline__ void forces_rows_staged(PairAcc& A, uint32_t i, bool 
                 stage[k] = rec[2 * (size_t)(lo + k)];
                 stage[kStageRecs + k] = rec[2 * (size_t)(lo + k) + 1];
             }
-            wave_lds_sync();
         }
+        if (AHEAD && row < 8 && active) {
+            next = row_fetch(w, (row + 1) % 3 - 1, (row + 1) / 3 - 1, cut, cp, key_base, start, end);
+            next_mask = masks[(size_t)(row + 1) * n + i];
+        }
+        if (staged) wave_lds_sync();
         if (has) {
             if (re - rb <= kRowMaskBits) {
                 unsigned long long mm = mrow;
@@ -730,7 +791,7 @@ struct FlatRows {
 //    most neighbours in that row.  The next row's entry is read from LDS
 //    one row ahead.  A particle with a row longer than kFlatRowBits takes
 //    the row loop.
-template <bool MASKS, bool COUPLED = false, bool DEFER = false>
+template <bool MASKS, bool COUPLED = false, bool DEFER = false, bool AHEAD = false>
 __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __restrict__ sp,
                                                 const float4* __restrict__ sv, const float2* __restrict__ rho_p,
                                                 const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
@@ -751,8 +812,8 @@ __device__ __forceinline__ void forces_particle(uint32_t i, const float4* __rest
     // memory below.
     const bool staged_rows = MASKS && stage != nullptr && flat == nullptr && __ballot(1) == ~0ull;
     if (staged_rows)
-        forces_rows_staged(A, i, active, pos, vel, active ? rho_p[i] : make_float2(1.0f, 0.0f), sp, sv, rho_p, start,
-                           end, n, cp, key_base, masks, stage);
+        forces_rows_staged<AHEAD>(A, i, active, pos, vel, active ? rho_p[i] : make_float2(1.0f, 0.0f), sp, sv, rho_p,
+                                  start, end, n, cp, key_base, masks, stage);
     if (active) {
         const float h = cp.kernel_radius, hh = h * h;
         const float s = cp.simulation_scale;
