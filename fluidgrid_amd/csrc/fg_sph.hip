@@ -261,9 +261,12 @@ __global__ void __launch_bounds__(BS)
                     const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, uint32_t n, float k,
                     float rho0, float m, fg_common_fluid_params cp, fg_kernels_cache kc,
                     unsigned long long* __restrict__ masks) {
+    // per wavefront: kMixedStage + 4 pair records, then kMixedStage positions
+    __shared__ float4 s_stage[(BS / 64) * (2 * kMixedStage + 4)];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
-    density_particle_mixed(i, sp, sv, rho_p, start, end, n, k, rho0, m, cp, kc, masks);
+    density_particle_mixed(i, sp, sv, rho_p, start, end, n, k, rho0, m, cp, kc, masks,
+                           s_stage + (threadIdx.x / 64) * (2 * kMixedStage + 4));
 }
 
 // LDS-tiled passes (fg_sph_tile.hpp).  Tile b of the grid is the sorted

@@ -133,41 +133,23 @@ __device__ __forceinline__ void neumaier_add(float& sum, float& comp, float x) {
 }
 #pragma clang fp contract(fast)
 
-// Density of sorted particle i (i < n) in the mixed form: the row masks and
-// the neighbour record of the FP32 pass (density_particle<true>), and
-// rho = m kPoly6 s^6 sum (h^2 - r^2)^3 over the same pairs, summed in
-// ascending sorted order within rows in order, compensated.
-__device__ __forceinline__ void density_particle_mixed(uint32_t i, const float4* __restrict__ sp,
-                                                       const float4* __restrict__ sv, float2* __restrict__ rho_p,
-                                                       const uint32_t* __restrict__ start,
-                                                       const uint32_t* __restrict__ end, uint32_t n, float k,
-                                                       float rho0, float m, const fg_common_fluid_params& cp,
-                                                       const fg_kernels_cache& kc,
-                                                       unsigned long long* __restrict__ masks) {
-    const float4 p = sp[i];
-    float4* rec = nbr_records(masks, n);
-    const float4 v = sv[i];
-    if (!(p.w > 0.0f)) {
-        store_record(i, rho_p[i], p, v, rec);
-        return;
-    }
-    const float s = cp.simulation_scale, s2 = s * s;
-    const float h = cp.kernel_radius, hh = h * h;
-    const float hu = h / s, hhu = hu * hu, h2u = cp.kernel_radius_squared / s2;
-    const float cut = hh * 1.0001f;
-    const RowWalker w = make_rows(p, cp);
-    const HalfGeom g = half_geom(cp);
-    const float3 o = half_offsets(p, cp, g);
-    const half2v tx = {(_Float16)o.x, (_Float16)o.x};
-    const half2v per = {(_Float16)g.per, (_Float16)g.per};
-    const half2v ncut = {(_Float16)(-g.cut), (_Float16)(-g.cut)};
-    const uint4* __restrict__ hr = nbr_half(masks, n);
-    float sum = 0.0f, comp = 0.0f;
+// One stencil row of the mixed density pass for one particle: the cull of
+// the row's candidates from their FP16 pair records, then the exact FP32
+// test and compensated sum of the survivors.  H and P are the pair records
+// and sorted positions from the row's first candidate, in memory or (LDS)
+// staged.
+struct MixedRow {
+    float4 p;
+    half2v tx, per, ncut;
+    float hhu, h2u;
+    float sum, comp;
 
-    // The survivors among candidates base + (bits of word), four per
-    // iteration with their loads issued together; returns the bits of those
-    // within h (the FP32 test of density4).
-    auto exact = [&](uint32_t word, uint32_t base) -> uint32_t {
+    // The survivors among candidates P[bits of word], four per iteration
+    // with their loads issued together; returns the bits of those within h
+    // (the FP32 test of density4).
+    template <bool LDS>
+    __device__ __forceinline__ uint32_t exact(uint32_t word, const float4* __restrict__ P) {
+        auto ld = [](const float4* q) { return LDS ? ld_xyz_lds(q) : ld_xyz(q); };
         uint32_t hits = 0u;
         while (word) {
             const uint32_t k0 = (uint32_t)__builtin_ctz(word);
@@ -181,8 +163,8 @@ __device__ __forceinline__ void density_particle_mixed(uint32_t i, const float4*
             const bool v3 = word != 0u;
             const uint32_t k3 = v3 ? (uint32_t)__builtin_ctz(word) : k0;
             word &= word - 1u;
-            const float3 q0 = ld_xyz(sp + base + k0), q1 = ld_xyz(sp + base + k1);
-            const float3 q2 = ld_xyz(sp + base + k2), q3 = ld_xyz(sp + base + k3);
+            const float3 q0 = ld(P + k0), q1 = ld(P + k1);
+            const float3 q2 = ld(P + k2), q3 = ld(P + k3);
             const float r0 = exact_r2(p, q0), r1 = exact_r2(p, q1);
             const float r2 = exact_r2(p, q2), r3 = exact_r2(p, q3);
             if (r0 < hhu) {
@@ -203,42 +185,112 @@ __device__ __forceinline__ void density_particle_mixed(uint32_t i, const float4*
             }
         }
         return hits;
-    };
-    // Survivor bits of the cnt (<= 32) candidates from j0.  (Keeping the
+    }
+
+    // Survivor bits of the cnt (<= 32) candidates from H[0].  (Keeping the
     // next group's records in flight, eight candidates per iteration, needs
     // 76 VGPRs and was no faster: the loop is bound by issue, not latency.)
-    auto cull_word = [&](uint32_t j0, uint32_t cnt, half2v ty, half2v tz) -> uint32_t {
+    __device__ __forceinline__ uint32_t cull_word(const uint4* __restrict__ H, uint32_t cnt, half2v ty,
+                                                  half2v tz) const {
         uint32_t word = 0u;
         for (uint32_t t = 0; t < cnt; t += 4) {
-            const uint4 r0 = hr[j0 + t], r1 = hr[j0 + t + 2];
+            const uint4 r0 = H[t], r1 = H[t + 2];
             word |= cull_quad(r0, r1, tx, ty, tz, per, ncut) << t;
         }
         return cnt < 32u ? word & ((1u << cnt) - 1u) : word;
-    };
+    }
 
-#pragma unroll
+    // The row's mask: 64 candidates at a time; a row longer than 64 is never
+    // read through its mask (the force pass re-tests it), which is 0.
+    template <bool LDS>
+    __device__ __forceinline__ unsigned long long row(const uint4* __restrict__ H, const float4* __restrict__ P,
+                                                      uint32_t len, half2v ty, half2v tz) {
+        unsigned long long msk = 0ull;
+        for (uint32_t c0 = 0; c0 < len; c0 += 64u) {
+            const uint32_t l = min(len - c0, 64u);
+            const uint32_t w0 = cull_word(H + c0, min(l, 32u), ty, tz);
+            const uint32_t w1 = l > 32u ? cull_word(H + c0 + 32u, l - 32u, ty, tz) : 0u;
+            const uint32_t e0 = exact<LDS>(w0, P + c0);
+            const uint32_t e1 = exact<LDS>(w1, P + c0 + 32u);
+            msk = (unsigned long long)e0 | ((unsigned long long)e1 << 32);
+        }
+        return len > kRowMaskBits ? 0ull : msk;
+    }
+};
+
+// Density of sorted particle i (i < n) in the mixed form: the row masks and
+// the neighbour record of the FP32 pass (density_particle<true>), and
+// rho = m kPoly6 s^6 sum (h^2 - r^2)^3 over the same pairs, summed in
+// ascending sorted order within rows in order, compensated.
+//
+// Given an LDS slice (`stage`, kMixedStage pair records then kMixedStage
+// positions) and a full wavefront, each row's run of pair records and
+// positions is staged there first, as in density_rows_staged; every lane
+// then takes part (an inactive particle's lane with no rows).
+constexpr uint32_t kMixedStage = 128;
+__device__ __forceinline__ void density_particle_mixed(uint32_t i, const float4* __restrict__ sp,
+                                                       const float4* __restrict__ sv, float2* __restrict__ rho_p,
+                                                       const uint32_t* __restrict__ start,
+                                                       const uint32_t* __restrict__ end, uint32_t n, float k,
+                                                       float rho0, float m, const fg_common_fluid_params& cp,
+                                                       const fg_kernels_cache& kc,
+                                                       unsigned long long* __restrict__ masks,
+                                                       float4* __restrict__ stage = nullptr) {
+    const float4 p = sp[i];
+    float4* rec = nbr_records(masks, n);
+    const float4 v = sv[i];
+    const bool active = p.w > 0.0f;
+    const bool staged_rows = stage != nullptr && __ballot(1) == ~0ull;
+    if (!active && !staged_rows) {
+        store_record(i, rho_p[i], p, v, rec);
+        return;
+    }
+    const float s = cp.simulation_scale, s2 = s * s;
+    const float h = cp.kernel_radius, hh = h * h;
+    const float hu = h / s, hhu = hu * hu, h2u = cp.kernel_radius_squared / s2;
+    const float cut = hh * 1.0001f;
+    const RowWalker w = make_rows(p, cp);
+    const HalfGeom g = half_geom(cp);
+    const float3 o = half_offsets(p, cp, g);
+    const uint4* __restrict__ hr = nbr_half(masks, n);
+    MixedRow M{p,
+               half2v{(_Float16)o.x, (_Float16)o.x},
+               half2v{(_Float16)g.per, (_Float16)g.per},
+               half2v{(_Float16)(-g.cut), (_Float16)(-g.cut)},
+               hhu,
+               h2u,
+               0.0f,
+               0.0f};
+    uint4* hst = reinterpret_cast<uint4*>(stage);
+    const float4* pst = stage + (kMixedStage + 4u);
+
     for (int row = 0; row < 9; row++) {
         const int dz = row / 3 - 1, dy = row % 3 - 1;
+        uint32_t rb = 0, re = 0;
+        const bool has = active && row_range(w, dy, dz, cut, cp, 0u, start, end, n, rb, re);
+        uint32_t lo = 0, hi = 0;
+        const bool staged = staged_rows && wave_run(has, rb, re, kMixedStage, lo, hi);  // the same on every lane
+        if (staged) {
+            // the pair records of the run and the two after it (a quad of
+            // candidates reads the records t and t + 2), and the positions
+            for (uint32_t q = __lane_id(); q < hi - lo + 2u; q += 64u) hst[q] = hr[lo + q];
+            for (uint32_t q = __lane_id(); q < hi - lo; q += 64u) stage[kMixedStage + 4u + q] = sp[lo + q];
+            wave_lds_sync();
+        }
         unsigned long long msk = 0ull;
-        uint32_t rb, re;
-        if (row_range(w, dy, dz, cut, cp, 0u, start, end, n, rb, re)) {
+        if (has) {
             const half2v ty = {(_Float16)(o.y - (float)dy * g.wy), (_Float16)(o.y - (float)dy * g.wy)};
             const half2v tz = {(_Float16)(o.z - (float)dz * g.wz), (_Float16)(o.z - (float)dz * g.wz)};
-            // 64 candidates at a time; a row longer than 64 is never read
-            // through its mask (the force pass re-tests it), which is 0.
-            for (uint32_t c0 = rb; c0 < re; c0 += 64u) {
-                const uint32_t len = min(re - c0, 64u);
-                const uint32_t w0 = cull_word(c0, min(len, 32u), ty, tz);
-                const uint32_t w1 = len > 32u ? cull_word(c0 + 32u, len - 32u, ty, tz) : 0u;
-                const uint32_t e0 = exact(w0, c0);
-                const uint32_t e1 = exact(w1, c0 + 32u);
-                msk = (unsigned long long)e0 | ((unsigned long long)e1 << 32);
-            }
-            if (re - rb > kRowMaskBits) msk = 0ull;
+            msk = staged ? M.row<true>(hst + (rb - lo), pst + (rb - lo), re - rb, ty, tz)
+                         : M.row<false>(hr + rb, sp + rb, re - rb, ty, tz);
         }
-        masks[(size_t)row * n + i] = msk;
+        if (active) masks[(size_t)row * n + i] = msk;
+        if (staged) wave_lds_sync();  // every lane's reads of this row before the next row's stores
     }
-    store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * (sum + comp), k, rho0, rho_p, rec, p, v);
+    if (active)
+        store_density(i, m * kc.kPoly6 * (s2 * s2 * s2) * (M.sum + M.comp), k, rho0, rho_p, rec, p, v);
+    else
+        store_record(i, rho_p[i], p, v, rec);
 }
 
 }  // namespace fg

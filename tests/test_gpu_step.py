@@ -133,14 +133,18 @@ def test_trajectory_100_steps_appmain_ref_xor(fglib, orc):
     assert err <= tol
 
 
-@pytest.mark.parametrize("which", ["cfg0", "compressed", "coupled"])
+@pytest.mark.parametrize("which", ["cfg0", "compressed", "coupled", "ragged"])
 def test_schedules_and_block_sizes_bit_identical(fglib, which):
     """The LDS-tiled passes and the flat and row-by-row schedules of the
     density and force passes, at every workgroup size, give the same bits
     after 10 graph steps ('compressed' has rows longer than the mask width,
     which the flat force schedule hands to the row loop and the tiled passes
-    to their mask fallback, and tiles past the LDS capacity; 'coupled' runs
-    the coupled force kernels with a torus resting in the surface)."""
+    to their mask fallback, runs longer than the row loop's LDS slices, and
+    tiles past the LDS capacity; 'coupled' runs the coupled force kernels
+    with a torus resting in the surface; 'ragged' has a particle count that
+    is not a multiple of 64, so the last wavefront is partial and reads its
+    rows from memory, and a cluster of inactive particles, so full
+    wavefronts stage rows for lanes that have none)."""
     bodies = None
     if which == "cfg0":
         s = scenes.config_scene(0)
@@ -149,6 +153,17 @@ def test_schedules_and_block_sizes_bit_identical(fglib, which):
         top = float(s.pos[:, 1].max())
         tor = scenes.torus_spheres(0.06, 0.02, 0.01) + np.array([0.12, top, 0.12])
         bodies = scenes.make_bodies([tor], [0.5], 0.02, s.common)
+    elif which == "ragged":
+        s = scenes.small_dam_break(lattice=(23, 9, 21), h=0.03, box=(0.8, 0.8, 0.8))
+        assert s.n % 64 != 0
+        # Every 11th particle inactive, moved into a cluster far (> 10 h)
+        # from the fluid: an inactive particle's (rho, p) is never computed,
+        # so one within h of the fluid would feed its neighbours the initial
+        # zero density (non-finite forces, whatever the schedule).
+        idx = np.arange(0, s.n, 11)
+        g = np.stack(np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij"), -1).reshape(-1, 3)
+        s.pos[idx, :3] = 0.6 + 0.015 * g[: len(idx)]
+        s.pos[idx, 3] = 0.0
     else:
         s = scenes.small_dam_break(lattice=(16, 16, 16), h=0.04)
         s.pos[:, :3] = 0.3 + (s.pos[:, :3] - 0.3) * 0.55
