@@ -38,6 +38,11 @@ struct State {
     // LDS-tiled single-fluid passes (density lists + list-walking forces);
     // they take precedence over the two schedules above when switched on.
     bool tiled = false;
+    // Coupled masked force pass (row loop): body contact of the particles
+    // inside a body grid's bounds deferred to a second kernel that walks each
+    // one's body-grid cells on a pair of lanes (fg_set_contact_defer);
+    // results are identical.
+    bool defer_contact = true;
     // Arithmetic of the fused density pass (fg_set_precision): FP32, or FP16
     // culling with exact FP32 survivors; the latter takes precedence over the
     // schedules above for density.

@@ -141,6 +141,20 @@ int fg_set_tiled(int enable) {
     return FG_OK;
 }
 
+int fg_set_contact_defer(int enable) {
+    state().defer_contact = enable != 0;
+    return FG_OK;
+}
+
+int fg_get_contact_defer(int* enable) {
+    if (!enable) {
+        set_error(FG_ERR_INVALID_ARGUMENT, "fg_get_contact_defer: null pointer");
+        return FG_ERR_INVALID_ARGUMENT;
+    }
+    *enable = state().defer_contact ? 1 : 0;
+    return FG_OK;
+}
+
 int fg_set_precision(int mode) {
     if (mode != FG_PRECISION_FP32 && mode != FG_PRECISION_MIXED) {
         set_error(FG_ERR_INVALID_ARGUMENT, "fg_set_precision: unknown mode %d", mode);

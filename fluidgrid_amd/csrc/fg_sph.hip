@@ -236,7 +236,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7)))
 }
 
 // Flat schedule (row table in LDS); same results as k_forces_fast<true, COUPLED>.
-// The occupancy hint is the one of k_forces_fast, for the same reason.
+// amdgpu_waves_per_eu(8): the coupled variant's two body-grid descriptors
+// take scalar registers (106 SGPRs without the hint, occupancy 7).
 template <bool COUPLED, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8)))
     k_forces_flat(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
@@ -251,6 +252,52 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8)))
     FlatRows fr{s_rb, s_mm, (uint32_t)BS, (uint32_t)threadIdx.x};
     forces_particle<true, COUPLED>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u, masks, bc,
                                    &fr);
+}
+
+// The coupled masked force pass with body contact deferred: a particle
+// inside a body grid's expanded bounds stores its pair sum and joins a list
+// (forces_particle with DEFER); the others finish here without the contact
+// walk, which leaves the wavefronts that never meet a body free of it.  Row
+// loop with row-ahead loads, as k_forces_fast<true, true>.
+template <int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7)))
+    k_forces_deferred(const float4* __restrict__ sp, const float4* __restrict__ sv, const float2* __restrict__ rho_p,
+                      const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                      float4* __restrict__ vout, float4* __restrict__ pout, const uint32_t* __restrict__ id,
+                      uint32_t n, float m, float mu, fg_common_fluid_params cp, fg_kernels_cache kc,
+                      const unsigned long long* __restrict__ masks, Contacts bc, DeferredContact dc) {
+    __shared__ float4 s_stage[(BS / 64) * 2 * kStageRecs];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+    if (i >= n) return;
+    forces_particle<true, true, true, true>(i, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu, cp, kc, 0u, masks,
+                                            bc, nullptr, &dc, s_stage + (threadIdx.x / 64) * 2 * kStageRecs);
+}
+
+// Second pass of the deferred form: body contact (fluid_external: the
+// static grid, then the dynamic one, summed from zero), walls, clamp and
+// integration of the listed particles from their pair sums.  Each particle
+// takes a pair of lanes that walk its body-grid cells together
+// (contact_cells, bit-identical to the one-lane walk).
+constexpr uint32_t kFinishLanes = 2;
+__global__ void __launch_bounds__(256)
+    k_contact_finish(const float4* __restrict__ sp, const float4* __restrict__ sv, float4* __restrict__ vout,
+                     float4* __restrict__ pout, const uint32_t* __restrict__ id, uint32_t n, float m,
+                     fg_common_fluid_params cp, Contacts bc, DeferredContact dc) {
+    const uint32_t cnt = min(*dc.count, n);
+    const uint32_t sub = threadIdx.x % kFinishLanes;
+    const uint32_t groups = gridDim.x * (blockDim.x / kFinishLanes);
+    // k is the same on both lanes of a group, so a group is active or idle as a whole
+    for (uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) / kFinishLanes; k < cnt; k += groups) {
+        const uint32_t i = dc.list[k];
+        const float4 pos = sp[i], vel = sv[i];
+        F3 e = f3(0.0f, 0.0f, 0.0f);
+        if (bc.st.enabled) e = add(e, contact_cells<kFinishLanes>(pos, vel, bc.st, sub));
+        if (bc.dyn.enabled) e = add(e, contact_cells<kFinishLanes>(pos, vel, bc.dyn, sub));
+        if (sub != 0) continue;
+        const float4 ia = dc.acc[i];
+        const F3 a = finish_accel_ext(f3(ia.x, ia.y, ia.z), m, pos, vel, cp, bc.coupled != 0, e);
+        write_back(id[i], n, pos, vel, a, cp, vout, pout);
+    }
 }
 
 // Mixed-precision density (fg_sph_mixed.hpp): the FP32 pass's masks and
@@ -525,7 +572,32 @@ int launch_forces(float4* sp, float4* sv, float2* rho_p, uint32_t* start, uint32
                           st.kernels, masks, bc);
     } else if (use_fast()) {
         const unsigned long long* mk = masks;
-        if (coupled && mk)
+        if (coupled && mk && st.defer_contact) {
+            // The deferred particles' pair sums and list live in the neighbour
+            // buffer's list area (used otherwise only by the tiled passes) and
+            // the count in its count area; the count is cleared by a kernel,
+            // not a memset, so the step can be captured in a graph.
+            char* base = reinterpret_cast<char*>(const_cast<unsigned long long*>(mk));
+            DeferredContact dc{reinterpret_cast<float4*>(base + nbr_list_offset(n)),
+                               reinterpret_cast<uint32_t*>(base + nbr_list_offset(n) + 16 * (size_t)n),
+                               reinterpret_cast<uint32_t*>(base + nbr_count_offset(n))};
+            launch_fill_u32(dc.count, 0u, 1, st.stream);
+            switch (st.sph_block) {
+                case 64: FG_SPH_LAUNCH_BS(k_forces_deferred<64>, 64, n, sp, sv, rho_p, start, end, vout, pout, id, n,
+                                          m, mu, st.common, st.kernels, mk, bc, dc); break;
+                case 128: FG_SPH_LAUNCH_BS(k_forces_deferred<128>, 128, n, sp, sv, rho_p, start, end, vout, pout, id,
+                                           n, m, mu, st.common, st.kernels, mk, bc, dc); break;
+                case 512: FG_SPH_LAUNCH_BS(k_forces_deferred<512>, 512, n, sp, sv, rho_p, start, end, vout, pout, id,
+                                           n, m, mu, st.common, st.kernels, mk, bc, dc); break;
+                default: FG_SPH_LAUNCH_BS(k_forces_deferred<256>, 256, n, sp, sv, rho_p, start, end, vout, pout, id,
+                                          n, m, mu, st.common, st.kernels, mk, bc, dc);
+            }
+            // up to one lane pair per particle (the count is known only on the device)
+            unsigned blocks = grid_for((uint64_t)n * kFinishLanes, 256);
+            if (blocks > 8192) blocks = 8192;
+            hipLaunchKernelGGL(k_contact_finish, dim3(blocks), dim3(256), 0, st.stream, sp, sv, vout, pout, id, n, m,
+                               st.common, bc, dc);
+        } else if (coupled && mk)
             FG_SPH_LAUNCH2(k_forces_fast, true, true, n, sp, sv, rho_p, start, end, vout, pout, id, n, m, mu,
                            st.common, st.kernels, mk, bc);
         else if (coupled)

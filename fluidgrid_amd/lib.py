@@ -141,6 +141,8 @@ SIGNATURES = {
     "fg_set_schedule": (i32, [i32, i32]),
     "fg_get_schedule": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fg_set_tiled": (i32, [i32]),
+    "fg_set_contact_defer": (i32, [i32]),
+    "fg_get_contact_defer": (i32, [C.POINTER(C.c_int)]),
     "fg_set_precision": (i32, [i32]),
     "fg_get_precision": (i32, []),
     "fg_get_tiled": (i32, [C.POINTER(C.c_int)]),

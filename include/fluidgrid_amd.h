@@ -306,6 +306,16 @@ int   fg_get_schedule(int* flatDensity, int* flatForces);
  * runs. */
 int   fg_set_tiled(int enable);
 int   fg_get_tiled(int* enable);
+/* Coupled force pass with neighbour masks, row-by-row schedule (default on):
+ * the particles inside a body grid's bounds hand their pair sum to a second
+ * kernel that adds body contact (walking each one's body-grid cells on two
+ * lanes), walls and clamp, and integrates them; the others finish in the
+ * force pass without the contact walk.  No counterpart in the reference: a
+ * launch-shape knob like fg_set_schedule; every result bit is the same.  It
+ * uses the neighbour buffer's list area, which only the LDS-tiled passes
+ * use otherwise. */
+int   fg_set_contact_defer(int enable);
+int   fg_get_contact_defer(int* enable);
 int   fg_set_precision(int mode);
 int   fg_get_precision(void);
 int   fg_get_common_fluid_params(fg_common_fluid_params* out);

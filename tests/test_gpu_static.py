@@ -209,3 +209,28 @@ def test_static_steps_bit_identical_eager_overlap_graph(fglib):
         assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
         for k in ref[2]:
             assert np.array_equal(ref[2][k], got[2][k]), k
+
+
+@pytest.mark.parametrize("with_torus", [False, True])
+def test_contact_deferred_bit_identical_to_inline(fglib, with_torus):
+    """The coupled force pass with body contact deferred to a second kernel
+    (fg_set_contact_defer(1), the default: particles inside a body grid's
+    bounds finish there, their body-grid cells walked by two lanes) gives the
+    bits of the inline contact walk, with the static cylinder alone and with
+    the dynamic torus too, at two workgroup sizes."""
+    s, b = _scene(with_torus=with_torus)
+    L = fglib.load()
+    try:
+        L.fg_set_contact_defer(0)
+        ref = _trajectory(fglib, s, b, 8)
+        L.fg_set_contact_defer(1)
+        for blk in (256, 64):
+            L.fg_set_tuning(blk, 8)
+            got = _trajectory(fglib, s, b, 8)
+            assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]), blk
+            if with_torus:
+                for k in ref[2]:
+                    assert np.array_equal(ref[2][k], got[2][k]), (blk, k)
+    finally:
+        L.fg_set_contact_defer(1)
+        L.fg_set_tuning(256, 8)
