@@ -230,3 +230,21 @@ def test_reference_prototypes_pass_arguments_intact(tmp_path):
     assert ("(numParticles=12345 mass=0.25 kViscosity=3.5 sph_particle_radius=0.125 cell_size_relation=(1,2,3) "
             "srb_particle_radius=0.0625 srb_cell_size=(0.5,0.75,1.25) kspring=-2000 kdamping=-50 "
             "srb_num_buckets=4096 rb_num_buckets=777 useStaticRB=1 useDynamicRB=0)") in lines["srb"]
+
+
+def test_contact_defer_switch_round_trips_on_the_host():
+    """fg_set_contact_defer / fg_get_contact_defer: process-wide state, on by
+    default, read back as set; a null output pointer is refused."""
+    import ctypes as C
+    L = lib.load()
+    v = C.c_int(-1)
+    assert L.fg_get_contact_defer(C.byref(v)) == 0 and v.value == 1
+    try:
+        assert L.fg_set_contact_defer(0) == 0
+        assert L.fg_get_contact_defer(C.byref(v)) == 0 and v.value == 0
+        assert L.fg_set_contact_defer(7) == 0
+        assert L.fg_get_contact_defer(C.byref(v)) == 0 and v.value == 1
+        assert L.fg_get_contact_defer(None) == 1  # FG_ERR_INVALID_ARGUMENT
+        assert L.fg_last_error() == 1 and L.fg_last_error() == 0
+    finally:
+        L.fg_set_contact_defer(1)
