@@ -56,6 +56,7 @@ sys.path.insert(0, ROOT)
 from fluidgrid_amd import abi, lib, scenes  # noqa: E402
 
 HBM_PEAK = 8.0e12
+DEFAULT_SPH_BLOCK = 64  # the library's default workgroup size of the fast SPH kernels
 METRIC = "particle-steps/s at 1M & 8M; % HBM peak; batched/mixed vs plain FP32"
 STAGE_BYTES = dict(hash=24, sort=16, cells=72, density=24, forces=60, integrate=48)
 OCC_BYTES = dict(cells=8, density=8, forces=8)
@@ -242,7 +243,7 @@ def measure_batched(ens, steps, warmup, tune_steps=10, tune=True, plain=True):
             out[label] = {"ms_per_step": 1e3 * dt / steps, "value": ens.n * steps / dt,
                           "repeats_ms": [round(x, 4) for x in ev_all]}
     finally:
-        lib.load().fg_set_tuning(256, 8)
+        lib.load().fg_set_tuning(DEFAULT_SPH_BLOCK, 8)
         lib.load().fg_set_schedule(0, 0)
     out["info"] = info
     if not plain:
@@ -507,7 +508,7 @@ def main():
     if a.tune:
         # the other lines run with the library's default shape
         L = lib.load()
-        lib.check(L.fg_set_tuning(256, 8), "fg_set_tuning")
+        lib.check(L.fg_set_tuning(DEFAULT_SPH_BLOCK, 8), "fg_set_tuning")
         L.fg_set_schedule(0, 0)
         L.fg_set_tiled(0)
     n = scene.n
@@ -557,7 +558,7 @@ def main():
             "form": ("mixed precision density (FP16 cull, exact FP32 survivors), " if a.mixed else "plain FP32, ")
                     + ("fused forces+integrate, HIP graph replay" if fused else "unfused"),
             "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
-            "tuning": tuning if tuning is not None else "library default shape (256-lane workgroups, "
+            "tuning": tuning if tuning is not None else "library default shape (64-lane workgroups, "
                                                          "row loops, 8-bit digits)",
             "step_algorithmic_GBps": step_gbs / 1e9,
             "step_hbm_frac": step_gbs / HBM_PEAK,

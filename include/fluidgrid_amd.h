@@ -287,7 +287,7 @@ int   fg_get_key_mode(void);
 int   fg_set_math_mode(int mode);
 int   fg_get_math_mode(void);
 /* Launch shape (process-wide): workgroup size of the fast density/force
- * kernels (64, 128, 256 or 512; default 256) and the widest radix digit of
+ * kernels (64, 128, 256 or 512; default 64) and the widest radix digit of
  * the sort (6..8 bits; default 8).  Results do not depend on either. */
 int   fg_set_tuning(int sphBlock, int sortDigitBits);
 int   fg_get_tuning(int* sphBlock, int* sortDigitBits);
